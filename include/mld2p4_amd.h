@@ -1,0 +1,138 @@
+/*
+ * mld2p4_amd.h — C-ABI of the MI355X engine for MLD2P4's preconditioner
+ * application phase.
+ *
+ * Plain C: opaque handle, plain pointers and sizes, int return codes
+ * (0 = success).  This is the layer a Fortran 2003 host binds with
+ * iso_c_binding, following the reference's own C-binding precedent
+ * (mlprec/mld_d_umf_solver.F90:51,62,89-121 <-> mlprec/impl/mld_dumf_interface.c:
+ * opaque c_ptr handle, integer(c_int) return, value scalars).
+ *
+ * Matrices are passed as 0-based CSR with int64 row pointers, int32 column
+ * indices (strictly ascending within each row) and double values, in host
+ * memory; they are copied, the caller keeps ownership.  Vectors of apply /
+ * krylov are DEVICE pointers (double, length n) unless the name ends in
+ * _host.  Error codes: 3111 not initialised, 3112 not built, 1121 smoother
+ * without solver, 2 ILU pivot too small / unknown preconditioner type,
+ * -1 invalid argument, -2 invalid level, -20 option not provided by this
+ * engine, -30 device error, -40 internal error.  mldhip_get_error returns
+ * the text of the last error of a handle.
+ */
+#ifndef MLD2P4_AMD_H
+#define MLD2P4_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mldhip_prec mldhip_prec;
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Create a preconditioner object bound to HIP device `device` (-1: host
+ * only; builds run but apply/krylov return -30).  Replaces the allocation
+ * of a type(mld_dprec_type) variable (mlprec/mld_d_prec_type.f90:84-142). */
+int mldhip_create(mldhip_prec** p, int device);
+/* prec%free (mlprec/mld_d_prec_type.f90:137 free method). */
+int mldhip_destroy(mldhip_prec* p);
+
+/* ---- configuration ------------------------------------------------------ */
+/* prec%init(ptype, info): mlprec/impl/mld_dprecinit.F90:86-218.
+ * ptype: "NOPREC", "JACOBI", "L1-JACOBI", "BJAC", "ML" (also "GS", "FBGS",
+ * "AS": accepted, their smoothers are not provided -> -20 at build). */
+int mldhip_init(mldhip_prec* p, const char* ptype);
+/* prec%set(what, val, info [,ilev, ilmax, pos]):
+ * mlprec/impl/mld_dcprecset.F90:78-453 (integer), :494-556 (string),
+ * :597-693 (real).  ilev/ilmax <= 0 mean "absent"; pos may be NULL. */
+int mldhip_seti(mldhip_prec* p, const char* what, int val, int ilev, int ilmax, const char* pos);
+int mldhip_setc(mldhip_prec* p, const char* what, const char* val, int ilev, int ilmax, const char* pos);
+int mldhip_setr(mldhip_prec* p, const char* what, double val, int ilev, int ilmax, const char* pos);
+/* Value type of the uploaded hierarchy: 0 = FP64 (default), 1 = FP32
+ * (mixed form; vectors of apply stay FP64).  No reference counterpart:
+ * the reference's FP32 twin is the separate mld_sprec_type. */
+int mldhip_set_precision(mldhip_prec* p, int kind);
+/* Storage of uploaded operators: 0 = automatic (default), 1 = CSR, 2 = ELL. */
+int mldhip_set_format(mldhip_prec* p, int fmt);
+
+/* ---- build -------------------------------------------------------------- */
+/* prec%hierarchy_build(a, desc, info): mlprec/impl/mld_d_hierarchy_bld.f90:66-490. */
+int mldhip_hierarchy_build(mldhip_prec* p, int n, const int64_t* irp, const int* ja, const double* val);
+/* prec%smoothers_build(a, desc, info, amold, vmold): mlprec/impl/mld_d_smoothers_bld.f90
+ * :66-307; on a device handle this also uploads the hierarchy to HBM once
+ * (the amold/vmold conversion of mld_d_base_onelev_cnv.f90:56-61). */
+int mldhip_smoothers_build(mldhip_prec* p);
+/* prec%build = hierarchy_build + smoothers_build (mlprec/impl/mld_dprecbld.f90:135,143). */
+int mldhip_build(mldhip_prec* p, int n, const int64_t* irp, const int* ja, const double* val);
+
+/* ---- apply (the hot path) ------------------------------------------------ */
+/* prec%apply(x, y, desc, info, trans): y = op(M^-1) x;
+ * mlprec/impl/mld_dprecaply.f90:305-442 (mld_dprecaply2_vect) ->
+ * mlprec/impl/mld_dmlprec_aply.f90:203-357.  x, y: device, length n.
+ * trans: 'N' or 'T' ('C' == 'T' for real data).  Enqueued on the handle's
+ * stream; returns after enqueueing (call mldhip_synchronize to wait). */
+int mldhip_apply(mldhip_prec* p, const double* x, double* y, char trans);
+/* Same with host vectors (copies over PCIe, synchronous). */
+int mldhip_apply_host(mldhip_prec* p, const double* x, double* y, char trans);
+
+/* psb_krylov(method, a, prec, b, x, eps, desc, info, itmax, iter, err,
+ * istop) call site (tests/pdegen/mld_d_pde3d.f90:753-755) with the matrix
+ * given to hierarchy_build.  method: "CG", "FCG", "BICGSTAB".  istop = 2
+ * (||r||/||b||).  b, x: device; x holds the initial guess on entry.
+ * Synchronous. */
+int mldhip_krylov(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,
+                  int istop, int* iter, double* err);
+int mldhip_krylov_host(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,
+                       int istop, int* iter, double* err);
+
+/* ---- streams ------------------------------------------------------------ */
+int mldhip_set_stream(mldhip_prec* p, void* hip_stream);
+int mldhip_synchronize(mldhip_prec* p);
+
+/* ---- inspection (hierarchy, smoother data) ------------------------------ */
+/* Number of levels (size(prec%precv)). */
+int mldhip_get_nlev(mldhip_prec* p, int* nlev);
+/* info[0..9] = n, nnz(A), nnz(P), nnz(R), nnz(ND pre), naggr, sweeps_pre,
+ * sweeps_post, smoother solver code, post solver code (-1 if none). */
+int mldhip_get_level_info(mldhip_prec* p, int lev, int64_t* info);
+/* Copy out a level operator: what 0 = A, 1 = P, 2 = R, 3 = ND, 4 = L, 5 = U
+ * (pre smoother), 13/14/15 = ND/L/U of a separate post smoother.  Any of
+ * irp/ja/val may be NULL to skip; sizes from mldhip_get_csr_size. */
+int mldhip_get_csr_size(mldhip_prec* p, int lev, int what, int64_t* nr, int64_t* nc, int64_t* nnz);
+int mldhip_get_level_csr(mldhip_prec* p, int lev, int what, int64_t* irp, int* ja, double* val);
+/* Aggregation map of level lev (>= 1) onto level lev-1 rows (0-based ids). */
+int mldhip_get_aggr(mldhip_prec* p, int lev, int* ilaggr);
+/* Smoother diagonal data (1/diag, 1/l1-diag or ILU 1/pivot), length n. */
+int mldhip_get_level_dv(mldhip_prec* p, int lev, int which, double* dv);
+int mldhip_get_omega(mldhip_prec* p, int lev, double* omega);
+/* Operator complexity sum nnz(A_l)/nnz(A_1) (mld_d_prec_type.f90:465-497). */
+int mldhip_get_complexity(mldhip_prec* p, double* cplx);
+int mldhip_get_device_bytes(mldhip_prec* p, int64_t* bytes);
+int mldhip_get_error(mldhip_prec* p, char* buf, int len);
+
+/* ---- single-kernel entry points (parity tests of the device kernels) ---- */
+/* y = M x with the device copy of a level operator (what 0 = A, 1 = P,
+ * 2 = R, 3 = ND): the psb_spmm / psb_map_X2Y / psb_map_Y2X products of
+ * mld_dmlprec_aply.f90:674-709.  FP64 handles only. */
+int mldhip_level_spmv(mldhip_prec* p, int lev, int what, const double* x, double* y);
+/* One smoother application sm%apply(1, x, beta, y, trans, sweeps, init='Z')
+ * of level lev (mlprec/impl/smoother/mld_d_jac_smoother_apply_vect.f90:38-215);
+ * which 0 = pre smoother, 1 = post smoother.  FP64 handles only. */
+int mldhip_level_smooth(mldhip_prec* p, int lev, int which, const double* x, int beta, double* y, int sweeps,
+                        char trans);
+
+/* ---- measurement (no reference counterpart: the reference times phases
+ * with psb_wtime in its drivers, tests/pdegen/mld_d_pde3d.f90:721-770) ---- */
+/* Average duration in ms of one cycle step of level lev, timed with HIP
+ * events on the handle's stream over reps back-to-back runs.  kind:
+ * 0 residual r = x - A y, 1 pre smoother, 2 restriction, 3 prolongation,
+ * 4 post smoother, 5 Krylov SpMV with its dot (level 0). */
+int mldhip_time_step(mldhip_prec* p, int lev, int kind, int reps, double* avg_ms);
+/* Device storage of a level operator (what 0 A, 1 P, 2 R, 3 ND):
+ * fmt 0 = CSR with `lanes` lanes per row, 1 = ELL of width `width`. */
+int mldhip_get_op_format(mldhip_prec* p, int lev, int what, int* fmt, int* lanes, int* width);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MLD2P4_AMD_H */

@@ -1,0 +1,334 @@
+"""Python host mirror of MLD2P4's preconditioner interface over the MI355X engine.
+
+The class ``MldPrec`` keeps the reference's method names and argument meaning
+(``mlprec/mld_d_prec_type.f90:84-142``):
+
+    p = MldPrec()                 # type(mld_dprec_type) :: p
+    p.init('ML')                  # call p%init('ML', info)
+    p.set('SMOOTHER_TYPE', 'JACOBI')
+    p.hierarchy_build(a)          # call p%hierarchy_build(a, desc, info)
+    p.smoothers_build()           # call p%smoothers_build(a, desc, info)
+    p.apply(x, y)                 # call p%apply(x, y, desc, info)
+    x, it, err = psb_krylov('CG', a, p, b, x0, eps=1e-6)
+
+Errors are raised as ``MldError`` carrying the reference's ``info`` code.
+The compute path is the HIP library ``_build/libmld2p4_amd.so``; there is no
+CPU fallback: without the library (or without a GPU for apply/krylov) the
+calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libmld2p4_amd.so")
+_lib = None
+
+
+class MldError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"info={code}: {msg}")
+        self.info = code
+
+
+def build(jobs=8):
+    """Compile the HIP library for gfx950 (make -C this directory)."""
+    subprocess.run(["make", "-s", "-j", str(jobs), "-C", _HERE], check=True)
+
+
+def lib():
+    """Load the engine library (it must have been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MldError(-30, f"engine library not built: {LIB_PATH} (run build())")
+        L = C.CDLL(LIB_PATH)
+        vp, cp, dp, ip, i64p = C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int64)
+        sig = {
+            "mldhip_create": [C.POINTER(vp), C.c_int],
+            "mldhip_destroy": [vp],
+            "mldhip_init": [vp, cp],
+            "mldhip_seti": [vp, cp, C.c_int, C.c_int, C.c_int, cp],
+            "mldhip_setc": [vp, cp, cp, C.c_int, C.c_int, cp],
+            "mldhip_setr": [vp, cp, C.c_double, C.c_int, C.c_int, cp],
+            "mldhip_set_precision": [vp, C.c_int],
+            "mldhip_set_format": [vp, C.c_int],
+            "mldhip_hierarchy_build": [vp, C.c_int, i64p, ip, dp],
+            "mldhip_smoothers_build": [vp],
+            "mldhip_build": [vp, C.c_int, i64p, ip, dp],
+            "mldhip_apply": [vp, vp, vp, C.c_char],
+            "mldhip_apply_host": [vp, dp, dp, C.c_char],
+            "mldhip_krylov": [vp, cp, vp, vp, C.c_double, C.c_int, C.c_int, ip, dp],
+            "mldhip_krylov_host": [vp, cp, dp, dp, C.c_double, C.c_int, C.c_int, ip, dp],
+            "mldhip_set_stream": [vp, vp],
+            "mldhip_synchronize": [vp],
+            "mldhip_get_nlev": [vp, ip],
+            "mldhip_get_level_info": [vp, C.c_int, i64p],
+            "mldhip_get_csr_size": [vp, C.c_int, C.c_int, i64p, i64p, i64p],
+            "mldhip_get_level_csr": [vp, C.c_int, C.c_int, i64p, ip, dp],
+            "mldhip_get_aggr": [vp, C.c_int, ip],
+            "mldhip_get_level_dv": [vp, C.c_int, C.c_int, dp],
+            "mldhip_get_omega": [vp, C.c_int, dp],
+            "mldhip_get_complexity": [vp, dp],
+            "mldhip_get_device_bytes": [vp, i64p],
+            "mldhip_get_error": [vp, C.c_char_p, C.c_int],
+            "mldhip_level_spmv": [vp, C.c_int, C.c_int, vp, vp],
+            "mldhip_level_smooth": [vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, C.c_char],
+            "mldhip_time_step": [vp, C.c_int, C.c_int, C.c_int, dp],
+            "mldhip_get_op_format": [vp, C.c_int, C.c_int, ip, ip, ip],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or an int address)."""
+    if isinstance(t, int):
+        return C.c_void_p(t)
+    return C.c_void_p(t.data_ptr())
+
+
+def _b(s):
+    return None if s is None else s.encode()
+
+
+def csr_arrays(a):
+    """(n, irp int64, ja int32, val float64) from a scipy matrix or a tuple."""
+    if isinstance(a, tuple):
+        irp, ja, val = a
+    else:
+        m = a.tocsr()
+        m.sort_indices()
+        irp, ja, val = m.indptr, m.indices, m.data
+    irp = np.ascontiguousarray(irp, dtype=np.int64)
+    ja = np.ascontiguousarray(ja, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    return len(irp) - 1, irp, ja, val
+
+
+LEVEL_OPS = {"A": 0, "P": 1, "R": 2, "ND": 3, "L": 4, "U": 5, "ND2": 13, "L2": 14, "U2": 15}
+
+
+class MldPrec:
+    """type(mld_dprec_type) on the MI355X engine (one handle per object)."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        rc = lib().mldhip_create(C.byref(self.h), device)
+        if rc:
+            raise MldError(rc, f"cannot create preconditioner on device {device}")
+        self.device = device
+        self.n = None
+
+    # -- errors -----------------------------------------------------------
+    def _check(self, rc, what):
+        if rc:
+            buf = C.create_string_buffer(512)
+            lib().mldhip_get_error(self.h, buf, 512)
+            raise MldError(rc, f"{what}: {buf.value.decode(errors='replace')}")
+
+    def free(self):
+        if self.h:
+            lib().mldhip_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # -- configuration ------------------------------------------------------
+    def init(self, ptype: str):
+        self._check(lib().mldhip_init(self.h, ptype.encode()), f"init({ptype})")
+        return self
+
+    def set(self, what: str, val, ilev: int | None = None, ilmax: int | None = None, pos: str | None = None):
+        il = -1 if ilev is None else int(ilev)
+        im = -1 if ilmax is None else int(ilmax)
+        if isinstance(val, str):
+            rc = lib().mldhip_setc(self.h, what.encode(), val.encode(), il, im, _b(pos))
+        elif isinstance(val, (float, np.floating)):
+            rc = lib().mldhip_setr(self.h, what.encode(), float(val), il, im, _b(pos))
+        else:
+            rc = lib().mldhip_seti(self.h, what.encode(), int(val), il, im, _b(pos))
+        self._check(rc, f"set({what}={val!r})")
+        return self
+
+    def set_precision(self, kind: str):
+        """'double' (FP64 hierarchy) or 'mixed' (FP32 hierarchy, FP64 vectors)."""
+        self._check(lib().mldhip_set_precision(self.h, {"double": 0, "mixed": 1}[kind]), "set_precision")
+        return self
+
+    def set_format(self, fmt: str):
+        self._check(lib().mldhip_set_format(self.h, {"auto": 0, "csr": 1, "ell": 2}[fmt]), "set_format")
+        return self
+
+    # -- build ----------------------------------------------------------------
+    def hierarchy_build(self, a):
+        n, irp, ja, val = csr_arrays(a)
+        self._keep = (irp, ja, val)
+        rc = lib().mldhip_hierarchy_build(self.h, n, irp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          ja.ctypes.data_as(C.POINTER(C.c_int)), _dp(val))
+        self._check(rc, "hierarchy_build")
+        self.n = n
+        return self
+
+    def smoothers_build(self):
+        self._check(lib().mldhip_smoothers_build(self.h), "smoothers_build")
+        return self
+
+    def build(self, a):
+        return self.hierarchy_build(a).smoothers_build()
+
+    # -- apply ----------------------------------------------------------------
+    def apply(self, x, y=None, trans: str = "N"):
+        """y = op(M^-1) x.  torch CUDA tensors (float64) stay on the device;
+        numpy arrays go through the host entry point."""
+        if isinstance(x, np.ndarray):
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            y = np.zeros_like(x) if y is None else y
+            self._check(lib().mldhip_apply_host(self.h, _dp(x), _dp(y), trans.encode()), "apply")
+            return y
+        import torch
+
+        if y is None:
+            y = torch.empty_like(x)
+        self.use_torch_stream()
+        self._check(lib().mldhip_apply(self.h, _ptr(x), _ptr(y), trans.encode()), "apply")
+        return y
+
+    def use_torch_stream(self):
+        import torch
+
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        lib().mldhip_set_stream(self.h, C.c_void_p(s))
+
+    def synchronize(self):
+        self._check(lib().mldhip_synchronize(self.h), "synchronize")
+
+    # -- inspection --------------------------------------------------------------
+    @property
+    def nlev(self):
+        v = C.c_int(0)
+        self._check(lib().mldhip_get_nlev(self.h, C.byref(v)), "get_nlev")
+        return v.value
+
+    def level_info(self, lev):
+        info = np.zeros(10, dtype=np.int64)
+        self._check(lib().mldhip_get_level_info(self.h, lev, info.ctypes.data_as(C.POINTER(C.c_int64))),
+                    "get_level_info")
+        keys = ["n", "nnz_a", "nnz_p", "nnz_r", "nnz_nd", "naggr", "sweeps_pre", "sweeps_post", "solver",
+                "solver_post"]
+        return dict(zip(keys, (int(v) for v in info)))
+
+    def level_csr(self, lev, what="A"):
+        code = LEVEL_OPS[what]
+        nr, nc, nz = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(lib().mldhip_get_csr_size(self.h, lev, code, C.byref(nr), C.byref(nc), C.byref(nz)),
+                    f"level_csr({lev},{what})")
+        irp = np.zeros(nr.value + 1, dtype=np.int64)
+        ja = np.zeros(nz.value, dtype=np.int32)
+        val = np.zeros(nz.value)
+        self._check(lib().mldhip_get_level_csr(self.h, lev, code, irp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                               ja.ctypes.data_as(C.POINTER(C.c_int)), _dp(val)),
+                    "get_level_csr")
+        import scipy.sparse as sp
+
+        return sp.csr_matrix((val, ja, irp), shape=(nr.value, nc.value))
+
+    def aggr(self, lev):
+        n = self.level_info(lev - 1)["n"]
+        out = np.zeros(n, dtype=np.int32)
+        self._check(lib().mldhip_get_aggr(self.h, lev, out.ctypes.data_as(C.POINTER(C.c_int))), "get_aggr")
+        return out
+
+    def dv(self, lev, which=0):
+        out = np.zeros(self.level_info(lev)["n"])
+        self._check(lib().mldhip_get_level_dv(self.h, lev, which, _dp(out)), "get_level_dv")
+        return out
+
+    def omega(self, lev):
+        v = C.c_double()
+        self._check(lib().mldhip_get_omega(self.h, lev, C.byref(v)), "get_omega")
+        return v.value
+
+    def complexity(self):
+        v = C.c_double()
+        self._check(lib().mldhip_get_complexity(self.h, C.byref(v)), "get_complexity")
+        return v.value
+
+    def device_bytes(self):
+        v = C.c_int64()
+        self._check(lib().mldhip_get_device_bytes(self.h, C.byref(v)), "get_device_bytes")
+        return v.value
+
+    def descr(self):
+        """Text description of the hierarchy (prec%descr analogue)."""
+        lines = [f"Multilevel preconditioner: {self.nlev} levels, operator complexity {self.complexity():.4f}"]
+        for l in range(self.nlev):
+            i = self.level_info(l)
+            lines.append(f"  level {l + 1}: n={i['n']} nnz(A)={i['nnz_a']} nnz(P)={i['nnz_p']} "
+                         f"sweeps={i['sweeps_pre']}/{i['sweeps_post']} solver={i['solver']}")
+        return "\n".join(lines)
+
+    # -- single kernels ---------------------------------------------------------
+    def level_spmv(self, lev, what, x, y):
+        self.use_torch_stream()
+        self._check(lib().mldhip_level_spmv(self.h, lev, LEVEL_OPS[what], _ptr(x), _ptr(y)), "level_spmv")
+        return y
+
+    STEP_KINDS = {"residual": 0, "pre": 1, "restrict": 2, "prolong": 3, "post": 4, "krylov_spmv": 5}
+
+    def time_step(self, lev, kind, reps=20):
+        """Average ms of one cycle step (HIP events on the handle's stream)."""
+        self.use_torch_stream()
+        v = C.c_double()
+        self._check(lib().mldhip_time_step(self.h, lev, self.STEP_KINDS[kind], reps, C.byref(v)), "time_step")
+        return v.value
+
+    def op_format(self, lev, what):
+        f, l, w = C.c_int(), C.c_int(), C.c_int()
+        self._check(lib().mldhip_get_op_format(self.h, lev, LEVEL_OPS[what], C.byref(f), C.byref(l), C.byref(w)),
+                    "get_op_format")
+        return {"fmt": ("CSR", "ELL")[f.value], "lanes": l.value, "width": w.value}
+
+    def level_smooth(self, lev, x, y, beta=0, sweeps=1, trans="N", which=0):
+        self.use_torch_stream()
+        self._check(lib().mldhip_level_smooth(self.h, lev, which, _ptr(x), beta, _ptr(y), sweeps, trans.encode()),
+                    "level_smooth")
+        return y
+
+
+def psb_krylov(method, a, prec: MldPrec, b, x=None, eps=1e-6, itmax=500, istop=2):
+    """psb_krylov(method, a, prec, b, x, eps, desc, info, itmax, iter, err,
+    istop) with the matrix `a` given to prec.hierarchy_build (the device
+    keeps its copy).  Returns (x, iter, err)."""
+    it = C.c_int(0)
+    err = C.c_double(0.0)
+    if isinstance(b, np.ndarray):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b) if x is None else np.ascontiguousarray(x, dtype=np.float64).copy()
+        rc = lib().mldhip_krylov_host(prec.h, method.encode(), _dp(b), _dp(x), eps, itmax, istop, C.byref(it),
+                                      C.byref(err))
+    else:
+        import torch
+
+        x = torch.zeros_like(b) if x is None else x
+        prec.use_torch_stream()
+        rc = lib().mldhip_krylov(prec.h, method.encode(), _ptr(b), _ptr(x), eps, itmax, istop, C.byref(it),
+                                 C.byref(err))
+    prec._check(rc, f"krylov({method})")
+    return x, it.value, err.value

@@ -1,0 +1,537 @@
+// Host build phase: smoothed-aggregation hierarchy and smoother data.
+//   hierarchy          : mlprec/impl/mld_d_hierarchy_bld.f90:66-413
+//   aggregation        : mlprec/impl/mld_d_dec_map_bld.f90:70-321,
+//                        mlprec/impl/mld_daggrmap_bld.f90:126-180
+//   smoothed P, R, Ac  : mlprec/impl/mld_daggrmat_smth_asb.f90:159-401
+//   smoothers          : mlprec/impl/mld_d_smoothers_bld.f90:156-169,
+//                        mlprec/impl/smoother/mld_d_jac_smoother_bld.f90:73-103,
+//                        mlprec/impl/solver/mld_d_diag_solver_bld.f90:74-92
+//   ILU(0)             : mlprec/impl/mld_dilu0_fact.f90:284-458,551-660
+// Products are computed row-parallel (OpenMP) with each output entry
+// accumulated in the order of the left factor's stored row, so the result
+// does not depend on the thread count.  Built with -ffp-contract=off.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mld_host.hpp"
+
+namespace mld {
+
+static std::vector<double> diag_of(const HCsr& a) {
+  std::vector<double> d(a.nr, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < a.nr; ++i)
+    for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k)
+      if (a.ja[k] == i) d[i] = a.val[k];
+  return d;
+}
+
+// Greedy three-phase decoupled aggregation (Vanek-Mandel-Brezina) with the
+// reference's strength test |a_ij| > theta*sqrt(|a_ii a_jj|), natural or
+// descending-degree visiting order, phase-two snapshot and first-maximum
+// tie-break.  Output ids are 0-based.
+int dec_map_bld(int iorder, double theta, const HCsr& a, std::vector<int>& ilaggr, int& naggr) {
+  const int nr = a.nr;
+  const int64_t nz = a.nnz();
+  std::vector<double> diag = diag_of(a);
+  // strength flag of every stored entry (in range and strong)
+  std::vector<unsigned char> strong(nz);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nr; ++i)
+    for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
+      int j = a.ja[k];
+      strong[k] = (j >= 0 && j < nr) &&
+                  (std::fabs(a.val[k]) > theta * std::sqrt(std::fabs(diag[i] * diag[j])));
+    }
+  std::vector<int> order(nr);
+  for (int i = 0; i < nr; ++i) order[i] = i;
+  if (iorder == ORD_DESC_DEG) {
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
+      return (a.irp[p + 1] - a.irp[p]) > (a.irp[q + 1] - a.irp[q]);
+    });
+  }
+  const int FREE = 0;  // aggregate ids are 1-based during the build
+  std::vector<int> agg(nr, FREE);
+  int cnt = 0;
+  // phase 1
+  for (int t = 0; t < nr; ++t) {
+    int i = order[t];
+    if (agg[i] != FREE) continue;
+    bool disjoint = true;
+    for (int64_t k = a.irp[i]; k < a.irp[i + 1] && disjoint; ++k)
+      if (strong[k] && agg[a.ja[k]] != FREE) disjoint = false;
+    if (!disjoint) continue;
+    ++cnt;
+    for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k)
+      if (strong[k]) agg[a.ja[k]] = cnt;
+    agg[i] = cnt;
+  }
+  // phase 2 (decisions on the phase-1 snapshot)
+  std::vector<int> snap(agg);
+  for (int t = 0; t < nr; ++t) {
+    int i = order[t];
+    if (agg[i] != FREE) continue;
+    double best = 0.0;
+    int64_t kbest = -1;
+    for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
+      if (!strong[k] || snap[a.ja[k]] <= 0) continue;
+      double c = std::fabs(a.val[k]);
+      if (c > best) { best = c; kbest = k; }
+    }
+    if (kbest >= 0) agg[i] = agg[a.ja[kbest]];
+  }
+  // phase 3
+  for (int t = 0; t < nr; ++t) {
+    int i = order[t];
+    if (agg[i] != FREE) continue;
+    ++cnt;
+    for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k)
+      if (strong[k] && agg[a.ja[k]] == FREE) agg[a.ja[k]] = cnt;
+    agg[i] = cnt;
+  }
+  ilaggr.resize(nr);
+  for (int i = 0; i < nr; ++i) {
+    if (agg[i] <= 0) return E_INTERNAL;
+    ilaggr[i] = agg[i] - 1;
+  }
+  naggr = cnt;
+  return 0;
+}
+
+// Row-parallel Gustavson product C = A*B; every C_ij is the sum over A's
+// stored row i, in that order, of a_ik*b_kj.  Columns of C ascending.
+static void spgemm(const HCsr& A, const HCsr& B, HCsr& C) {
+  const int nr = A.nr, nc = B.nc;
+  C.nr = nr; C.nc = nc;
+  C.irp.assign((size_t)nr + 1, 0);
+  std::vector<int64_t> cnt(nr);
+#pragma omp parallel
+  {
+    std::vector<int> mark(nc, -1);
+#pragma omp for schedule(dynamic, 4096)
+    for (int i = 0; i < nr; ++i) {
+      int64_t c = 0;
+      for (int64_t ka = A.irp[i]; ka < A.irp[i + 1]; ++ka) {
+        int k = A.ja[ka];
+        for (int64_t kb = B.irp[k]; kb < B.irp[k + 1]; ++kb) {
+          int j = B.ja[kb];
+          if (mark[j] != i) { mark[j] = i; ++c; }
+        }
+      }
+      cnt[i] = c;
+    }
+  }
+  for (int i = 0; i < nr; ++i) C.irp[i + 1] = C.irp[i] + cnt[i];
+  C.ja.resize(C.irp[nr]);
+  C.val.resize(C.irp[nr]);
+#pragma omp parallel
+  {
+    std::vector<int> mark(nc, -1);
+    std::vector<double> acc(nc, 0.0);
+    std::vector<int> cols;
+#pragma omp for schedule(dynamic, 4096)
+    for (int i = 0; i < nr; ++i) {
+      cols.clear();
+      for (int64_t ka = A.irp[i]; ka < A.irp[i + 1]; ++ka) {
+        int k = A.ja[ka];
+        double ak = A.val[ka];
+        for (int64_t kb = B.irp[k]; kb < B.irp[k + 1]; ++kb) {
+          int j = B.ja[kb];
+          if (mark[j] != i) { mark[j] = i; acc[j] = 0.0; cols.push_back(j); }
+          acc[j] = acc[j] + ak * B.val[kb];
+        }
+      }
+      std::sort(cols.begin(), cols.end());
+      int64_t p = C.irp[i];
+      for (int j : cols) { C.ja[p] = j; C.val[p] = acc[j]; ++p; }
+    }
+  }
+}
+
+// Transpose; each output row lists its columns in ascending order.
+static void transpose(const HCsr& A, HCsr& T) {
+  T.nr = A.nc; T.nc = A.nr;
+  T.irp.assign((size_t)T.nr + 1, 0);
+  for (int64_t k = 0; k < A.nnz(); ++k) T.irp[A.ja[k] + 1]++;
+  for (int i = 0; i < T.nr; ++i) T.irp[i + 1] += T.irp[i];
+  T.ja.resize(A.nnz()); T.val.resize(A.nnz());
+  std::vector<int64_t> pos(T.irp.begin(), T.irp.end() - 1);
+  for (int i = 0; i < A.nr; ++i)
+    for (int64_t k = A.irp[i]; k < A.irp[i + 1]; ++k) {
+      int64_t q = pos[A.ja[k]]++;
+      T.ja[q] = i; T.val[q] = A.val[k];
+    }
+}
+
+int aggrmat_smth_asb(const HCsr& a, const std::vector<int>& ilaggr, int naggr, LevelParms& parms, HCsr& ac,
+                     HCsr& p, HCsr& r) {
+  if (parms.aggr_prol != SMOOTH_PROL) return E_UNSUPPORTED;  // only SMOOTHED is provided
+  const int n = a.nr;
+  std::vector<double> dinv = diag_of(a);
+  const bool filter = (parms.aggr_filter == FILTER_MAT);
+  // S = (I - omega D^-1 A_or_Af), built in A's (or Af's) stored order
+  HCsr s;
+  if (filter) {
+    // filtered matrix: weak entries removed and lumped on the diagonal
+    s.nr = s.nc = n; s.irp.assign((size_t)n + 1, 0);
+    std::vector<double> tmpv;
+    for (int i = 0; i < n; ++i) {
+      double lump = 0.0;
+      int64_t jd = -1;
+      int64_t start = (int64_t)s.ja.size();
+      for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
+        int j = a.ja[k];
+        double v = a.val[k];
+        if (j == i) jd = (int64_t)s.ja.size();
+        if (std::fabs(v) < parms.aggr_thresh * std::sqrt(std::fabs(dinv[i] * dinv[j]))) {
+          lump = lump + v;
+          v = 0.0;
+        }
+        s.ja.push_back(j); s.val.push_back(v);
+      }
+      if (jd >= 0) s.val[jd] = s.val[jd] - lump;
+      // drop exact zeros (clean_zeros)
+      int64_t w = start;
+      for (int64_t k = start; k < (int64_t)s.ja.size(); ++k)
+        if (s.val[k] != 0.0) { s.ja[w] = s.ja[k]; s.val[w] = s.val[k]; ++w; }
+      s.ja.resize(w); s.val.resize(w);
+      s.irp[i + 1] = w;
+    }
+  }
+  for (int i = 0; i < n; ++i) dinv[i] = (dinv[i] != 0.0) ? 1.0 / dinv[i] : 1.0;
+  // D^-1 A row scaling and the max-norm estimate ||D^-1 A||_inf
+  HCsr da;
+  da.nr = a.nr; da.nc = a.nc; da.irp = a.irp; da.ja = a.ja; da.val.resize(a.nnz());
+  double anorm = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : anorm)
+  for (int i = 0; i < n; ++i) {
+    double rs = 0.0;
+    for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
+      da.val[k] = a.val[k] * dinv[i];
+      rs = rs + std::fabs(da.val[k]);
+    }
+    anorm = std::max(anorm, rs);
+  }
+  double omega;
+  if (parms.aggr_omega_alg == EIG_EST) {
+    if (parms.aggr_eig != MAX_NORM) return E_UNSUPPORTED;
+    omega = 4.0 / (3.0 * anorm);
+    parms.aggr_omega_val = omega;
+  } else if (parms.aggr_omega_alg == USER_CHOICE) {
+    omega = parms.aggr_omega_val;
+  } else {
+    return E_BAD_ARG;
+  }
+  if (filter) {
+    for (int i = 0; i < n; ++i)
+      for (int64_t k = s.irp[i]; k < s.irp[i + 1]; ++k) s.val[k] = s.val[k] * dinv[i];
+  } else {
+    s = std::move(da);
+  }
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i)
+    for (int64_t k = s.irp[i]; k < s.irp[i + 1]; ++k)
+      s.val[k] = (s.ja[k] == i) ? 1.0 - omega * s.val[k] : -(omega * s.val[k]);
+  // tentative prolongator: row i has a single 1.0 in column ilaggr[i]
+  HCsr pt;
+  pt.nr = n; pt.nc = naggr;
+  pt.irp.resize((size_t)n + 1);
+  for (int i = 0; i <= n; ++i) pt.irp[i] = i;
+  pt.ja.assign(ilaggr.begin(), ilaggr.end());
+  pt.val.assign(n, 1.0);
+  spgemm(s, pt, p);        // P  = (I - omega D^-1 A) Ptilde
+  HCsr ap;
+  spgemm(a, p, ap);        // AP = A P
+  transpose(p, r);         // R  = P^T
+  spgemm(r, ap, ac);       // Ac = R (A P)
+  return 0;
+}
+
+// IKJ ILU(0) restricted to pattern(A): L strict lower with l_ik stored,
+// U strict upper scaled by 1/pivot, d = 1/pivot.
+int ilu0_factor(const HCsr& a, HCsr& l, HCsr& u, std::vector<double>& d) {
+  const int m = a.nr;
+  l.clear(); u.clear();
+  l.nr = l.nc = u.nr = u.nc = m;
+  l.irp.assign((size_t)m + 1, 0); u.irp.assign((size_t)m + 1, 0);
+  d.assign(m, 0.0);
+  for (int i = 0; i < m; ++i) {
+    double piv = 0.0;
+    for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
+      int j = a.ja[k];
+      if (j < i && j >= 0) { l.ja.push_back(j); l.val.push_back(a.val[k]); }
+      else if (j == i) piv = a.val[k];
+      else if (j > i && j < m) { u.ja.push_back(j); u.val.push_back(a.val[k]); }
+    }
+    l.irp[i + 1] = (int64_t)l.ja.size();
+    u.irp[i + 1] = (int64_t)u.ja.size();
+    const int64_t lb = l.irp[i], le = l.irp[i + 1], ub = u.irp[i], ue = u.irp[i + 1];
+    for (int64_t kk = lb; kk < le; ++kk) {
+      const double lik = l.val[kk];  // unscaled l_ik used for the updates
+      const int k = l.ja[kk];
+      l.val[kk] = lik * d[k];
+      int64_t sl = kk + 1, su = ub;  // merge cursors into row i of L and U
+      for (int64_t jj = u.irp[k]; jj < u.irp[k + 1]; ++jj) {
+        const int j = u.ja[jj];
+        if (j < i) {
+          while (sl < le && l.ja[sl] < j) ++sl;
+          if (sl < le && l.ja[sl] == j) { l.val[sl] = l.val[sl] - lik * u.val[jj]; ++sl; }
+        } else if (j == i) {
+          piv = piv - lik * u.val[jj];
+        } else {
+          while (su < ue && u.ja[su] < j) ++su;
+          if (su < ue && u.ja[su] == j) { u.val[su] = u.val[su] - lik * u.val[jj]; ++su; }
+        }
+      }
+    }
+    if (std::fabs(piv) < 2.220446049250313e-16) return E_PIVOT;
+    piv = 1.0 / piv;
+    d[i] = piv;
+    for (int64_t kk = ub; kk < ue; ++kk) u.val[kk] = u.val[kk] * piv;
+  }
+  return 0;
+}
+
+// Level sets of a triangular solve: rows in the same level depend only on
+// rows of earlier levels.  Rows within a level are kept in ascending order.
+void level_schedule(const HCsr& t, bool lower, std::vector<int>& ptr, std::vector<int>& rows) {
+  const int n = t.nr;
+  std::vector<int> lev(n, 0);
+  int nlev = 0;
+  if (lower) {
+    for (int i = 0; i < n; ++i) {
+      int lv = 0;
+      for (int64_t k = t.irp[i]; k < t.irp[i + 1]; ++k) lv = std::max(lv, lev[t.ja[k]] + 1);
+      lev[i] = lv;
+      nlev = std::max(nlev, lv + 1);
+    }
+  } else {
+    for (int i = n - 1; i >= 0; --i) {
+      int lv = 0;
+      for (int64_t k = t.irp[i]; k < t.irp[i + 1]; ++k) lv = std::max(lv, lev[t.ja[k]] + 1);
+      lev[i] = lv;
+      nlev = std::max(nlev, lv + 1);
+    }
+  }
+  ptr.assign((size_t)nlev + 1, 0);
+  for (int i = 0; i < n; ++i) ptr[lev[i] + 1]++;
+  for (int l = 0; l < nlev; ++l) ptr[l + 1] += ptr[l];
+  rows.resize(n);
+  std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+  for (int i = 0; i < n; ++i) rows[pos[lev[i]]++] = i;
+}
+
+static int dense_lu(const HCsr& a, std::vector<double>& lu, std::vector<int>& piv) {
+  const int n = a.nr;
+  lu.assign((size_t)n * n, 0.0);
+  piv.resize(n);
+  for (int i = 0; i < n; ++i)
+    for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) lu[(size_t)i * n + a.ja[k]] += a.val[k];
+  for (int k = 0; k < n; ++k) {
+    int pr = k;
+    double mx = std::fabs(lu[(size_t)k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(lu[(size_t)i * n + k]) > mx) { mx = std::fabs(lu[(size_t)i * n + k]); pr = i; }
+    piv[k] = pr;
+    if (mx == 0.0) return E_PIVOT;
+    if (pr != k)
+      for (int j = 0; j < n; ++j) std::swap(lu[(size_t)k * n + j], lu[(size_t)pr * n + j]);
+    const double inv = 1.0 / lu[(size_t)k * n + k];
+#pragma omp parallel for schedule(static) if (n > 512)
+    for (int i = k + 1; i < n; ++i) {
+      double f = lu[(size_t)i * n + k] * inv;
+      lu[(size_t)i * n + k] = f;
+      for (int j = k + 1; j < n; ++j) lu[(size_t)i * n + j] = lu[(size_t)i * n + j] - f * lu[(size_t)k * n + j];
+    }
+  }
+  return 0;
+}
+
+static int build_smoother(const HCsr& a, const SmootherSpec& spec, SmootherData& sd) {
+  sd = SmootherData();
+  sd.smoother = spec.smoother;
+  sd.solver = spec.solver;
+  const int n = a.nr;
+  if (spec.smoother == NOPREC) return 0;
+  if (spec.smoother != JAC) return E_UNSUPPORTED;  // AS: not provided
+  if (spec.solver < 0) return E_NO_SOLVER;
+  sd.nd.nr = sd.nd.nc = n;
+  sd.nd.irp.assign((size_t)n + 1, 0);
+  switch (spec.solver) {
+    case F_NONE:
+      return 0;
+    case DIAG_SCALE:
+    case L1_DIAG: {
+      const bool l1 = (spec.solver == L1_DIAG);
+      std::vector<double> d = diag_of(a);
+      // ND: A without its diagonal (clip_diag); for l1-Jacobi A - D_l1,
+      // i.e. the diagonal entry replaced by -sum_{j!=i}|a_ij|.
+      std::vector<int64_t> cnt(n);
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < n; ++i) {
+        int64_t c = 0;
+        for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k)
+          if (a.ja[k] != i) ++c;
+        cnt[i] = c + (l1 ? 1 : 0);  // l1: the diagonal slot is always present
+      }
+      for (int i = 0; i < n; ++i) sd.nd.irp[i + 1] = sd.nd.irp[i] + cnt[i];
+      sd.nd.ja.resize(sd.nd.irp[n]);
+      sd.nd.val.resize(sd.nd.irp[n]);
+      sd.dv.resize(n);
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < n; ++i) {
+        int64_t q = sd.nd.irp[i];
+        double dd = d[i];
+        if (l1) {
+          double s = 0.0;
+          bool has = false;
+          for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k)
+            if (a.ja[k] != i) s = s + std::fabs(a.val[k]); else has = true;
+          dd = dd + s;
+          bool placed = false;
+          for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
+            int j = a.ja[k];
+            if (!has && !placed && j > i) { sd.nd.ja[q] = i; sd.nd.val[q] = -s; ++q; placed = true; }
+            if (j == i) { sd.nd.ja[q] = i; sd.nd.val[q] = -s; ++q; placed = true; }
+            else { sd.nd.ja[q] = j; sd.nd.val[q] = a.val[k]; ++q; }
+          }
+          if (!placed) { sd.nd.ja[q] = i; sd.nd.val[q] = -s; ++q; }
+        } else {
+          for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k)
+            if (a.ja[k] != i) { sd.nd.ja[q] = a.ja[k]; sd.nd.val[q] = a.val[k]; ++q; }
+        }
+        sd.dv[i] = (dd == 0.0) ? 1.0 : 1.0 / dd;
+      }
+      sd.nnz_nd = sd.nd.nnz();
+      return 0;
+    }
+    case ILU_N: {
+      if (spec.fillin != 0) return E_UNSUPPORTED;  // ILU(k>0) not provided
+      sd.nnz_nd = 0;                               // off-block part: empty on one device
+      int rc = ilu0_factor(a, sd.l, sd.u, sd.dv);
+      if (rc) return rc;
+      level_schedule(sd.l, true, sd.lsched_ptr, sd.lsched_rows);
+      level_schedule(sd.u, false, sd.usched_ptr, sd.usched_rows);
+      return 0;
+    }
+    case DENSE_LU:
+      sd.nnz_nd = 0;
+      return dense_lu(a, sd.lu, sd.piv);
+    default:
+      return E_UNSUPPORTED;  // GS/BWGS/MILU/ILUT and third-party solvers
+  }
+}
+
+int hierarchy_build(Prec& p, int n, const int64_t* irp, const int* ja, const double* val) {
+  if (!p.initialized || p.precv.empty()) return E_NOT_INIT;
+  p.hier_built = p.smoothers_built = false;
+  std::vector<Level>& v = p.precv;
+  Level& l0 = v[0];
+  l0.a.clear();
+  l0.a.nr = l0.a.nc = n;
+  l0.a.irp.assign(irp, irp + n + 1);
+  int64_t nz = l0.a.irp[n];
+  l0.a.ja.assign(ja, ja + nz);
+  l0.a.val.assign(val, val + nz);
+  for (int i = 0; i < n; ++i)
+    for (int64_t k = irp[i] + 1; k < irp[i + 1]; ++k)
+      if (ja[k] <= ja[k - 1]) { p.last_error = "CSR rows must have strictly ascending columns"; return E_BAD_ARG; }
+  if ((int)v.size() == 1) { p.hier_built = true; return 0; }
+  // default coarse size: 40 * int(N^(1/3)) (mld_d_hierarchy_bld.f90:187-197)
+  int casize = p.min_coarse_size;
+  if (casize < 0) {
+    casize = (int)std::pow((double)n, 1.0 / (1.0 * 3.0));
+    casize = std::max(casize, 1) * 40;
+    p.min_coarse_size = casize;
+  }
+  const int iszv0 = (int)v.size();
+  const int nplevs = std::max(2, p.max_levs);
+  // resize to the target number of levels (:199-241): base parms/smoothers
+  // at 1, intermediate ones from level 2, coarse ones at the end
+  if (iszv0 != nplevs) {
+    std::vector<Level> t(nplevs);
+    Level base = v[0], med = v[std::min(1, iszv0 - 1)], crs = v[iszv0 - 1];
+    t[0].parms = base.parms; t[0].sm = base.sm; t[0].sm2a = base.sm2a;
+    for (int i = 2; i <= std::min(iszv0, nplevs) - 1; ++i) {
+      t[i - 1].parms = med.parms; t[i - 1].sm = v[i - 1].sm; t[i - 1].sm2a = v[i - 1].sm2a;
+    }
+    for (int i = iszv0 - 1; i <= nplevs - 1; ++i) {  // as in the reference, may start at level 1
+      t[i - 1].parms = med.parms; t[i - 1].sm = med.sm; t[i - 1].sm2a = med.sm2a;
+    }
+    t[nplevs - 1].parms = crs.parms; t[nplevs - 1].sm = crs.sm; t[nplevs - 1].sm2a = crs.sm2a;
+    t[0].a = std::move(v[0].a);
+    v = std::move(t);
+  }
+  const int iszv = (int)v.size();
+  const LevelParms coarseparms = v[iszv - 1].parms;
+  const SmootherSpec coarse_sm = v[iszv - 1].sm, coarse_sm2a = v[iszv - 1].sm2a;
+  int newsz = 0;
+  for (int i = 2; i <= iszv; ++i) {  // array_build_loop, 1-based level numbers
+    Level& prev = v[i - 2];
+    Level& cur = v[i - 1];
+    std::vector<int> ilaggr;
+    int naggr = 0;
+    if (cur.parms.par_aggr_alg != DEC_AGGR) return E_UNSUPPORTED;
+    int rc = dec_map_bld(cur.parms.aggr_ord, cur.parms.aggr_thresh, prev.a, ilaggr, naggr);
+    if (rc) return rc;
+    double sizeratio = (double)naggr;
+    sizeratio = (i == 2) ? (double)n / sizeratio : (double)prev.naggr / sizeratio;
+    cur.szratio = sizeratio;
+    if (naggr <= casize) newsz = i;
+    if (i > 2) {
+      if (sizeratio < p.min_cr_ratio) newsz = (sizeratio > 1.0) ? i : i - 1;
+      if (naggr == prev.naggr) newsz = i - 1;
+    }
+    if (newsz > 0) {
+      Level& tgt = v[newsz - 1];
+      double athresh = tgt.parms.aggr_thresh, aomega = tgt.parms.aggr_omega_val;
+      tgt.parms = coarseparms;
+      tgt.parms.aggr_thresh = athresh;
+      tgt.parms.aggr_omega_val = aomega;
+      tgt.sm = coarse_sm; tgt.sm2a = coarse_sm2a;
+      if (newsz < i) { ilaggr = tgt.ilaggr; naggr = tgt.naggr; }
+      rc = aggrmat_smth_asb(v[newsz - 2].a, ilaggr, naggr, tgt.parms, tgt.a, tgt.p, tgt.r);
+      if (rc) return rc;
+      tgt.ilaggr = std::move(ilaggr);
+      tgt.naggr = naggr;
+      break;
+    }
+    rc = aggrmat_smth_asb(prev.a, ilaggr, naggr, cur.parms, cur.a, cur.p, cur.r);
+    if (rc) return rc;
+    cur.ilaggr = std::move(ilaggr);
+    cur.naggr = naggr;
+  }
+  if (newsz > 0) v.resize(newsz);
+  for (auto& lv : v)
+    if ((int64_t)lv.a.nnz() >= (int64_t)1 << 31) { p.last_error = "level matrix exceeds 2^31 entries"; return E_UNSUPPORTED; }
+  p.hier_built = true;
+  return 0;
+}
+
+int smoothers_build(Prec& p) {
+  if (!p.initialized) return E_NOT_INIT;
+  if (!p.hier_built) return E_NOT_BUILT;
+  for (auto& lv : p.precv) {
+    if (!lv.sm.allocated) return E_NO_SOLVER;
+    int rc = build_smoother(lv.a, lv.sm, lv.smd);
+    if (rc) return rc;
+    if (lv.sm2a.allocated) {
+      rc = build_smoother(lv.a, lv.sm2a, lv.smd2);
+      if (rc) return rc;
+    }
+  }
+  p.smoothers_built = true;
+  return 0;
+}
+
+double operator_complexity(const Prec& p) {  // mld_d_prec_type.f90:465-497
+  if (p.precv.empty() || p.precv[0].a.nnz() == 0) return 0.0;
+  double s = 0.0;
+  for (auto& lv : p.precv) s += (double)lv.a.nnz();
+  return s / (double)p.precv[0].a.nnz();
+}
+
+}  // namespace mld

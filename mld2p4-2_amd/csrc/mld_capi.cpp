@@ -1,0 +1,352 @@
+// extern "C" layer: declarations in include/mld2p4_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/mld2p4_amd.h"
+#include "mld_device.hpp"
+#include "mld_host.hpp"
+
+struct mldhip_prec {
+  mld::Prec host;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::unique_ptr<mld::DevPrec> dev;
+  std::string err;
+};
+
+using mld::E_BAD_ARG;
+using mld::E_BAD_LEVEL;
+using mld::E_DEVICE;
+using mld::E_NOT_BUILT;
+
+static std::string S(const char* c) { return c ? std::string(c) : std::string(); }
+
+static int fail(mldhip_prec* p, int code, const std::string& msg) {
+  if (p) p->err = msg;
+  return code;
+}
+
+static const mld::HCsr* level_csr(mldhip_prec* p, int lev, int what) {
+  if (!p || lev < 0 || lev >= (int)p->host.precv.size()) return nullptr;
+  const mld::Level& L = p->host.precv[lev];
+  switch (what) {
+    case 0: return &L.a;
+    case 1: return lev > 0 ? &L.p : nullptr;
+    case 2: return lev > 0 ? &L.r : nullptr;
+    case 3: return &L.smd.nd;
+    case 4: return &L.smd.l;
+    case 5: return &L.smd.u;
+    case 13: return L.sm2a.allocated ? &L.smd2.nd : nullptr;
+    case 14: return L.sm2a.allocated ? &L.smd2.l : nullptr;
+    case 15: return L.sm2a.allocated ? &L.smd2.u : nullptr;
+  }
+  return nullptr;
+}
+
+extern "C" {
+
+int mldhip_create(mldhip_prec** pp, int device) {
+  if (!pp) return E_BAD_ARG;
+  auto* p = new mldhip_prec;
+  p->device = device;
+  if (device >= 0) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device >= ndev) {
+      delete p;
+      *pp = nullptr;
+      return E_DEVICE;
+    }
+  }
+  *pp = p;
+  return 0;
+}
+
+int mldhip_destroy(mldhip_prec* p) {
+  if (!p) return 0;
+  if (p->device >= 0) (void)hipSetDevice(p->device);
+  p->dev.reset();
+  if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return 0;
+}
+
+int mldhip_init(mldhip_prec* p, const char* ptype) {
+  if (!p) return E_BAD_ARG;
+  p->dev.reset();
+  int rc = mld::prec_init(p->host, S(ptype));
+  if (rc) return fail(p, rc, p->host.last_error);
+  return 0;
+}
+
+int mldhip_seti(mldhip_prec* p, const char* what, int val, int ilev, int ilmax, const char* pos) {
+  if (!p) return E_BAD_ARG;
+  return mld::prec_set_int(p->host, S(what), val, ilev, ilmax, S(pos));
+}
+int mldhip_setc(mldhip_prec* p, const char* what, const char* val, int ilev, int ilmax, const char* pos) {
+  if (!p) return E_BAD_ARG;
+  return mld::prec_set_str(p->host, S(what), S(val), ilev, ilmax, S(pos));
+}
+int mldhip_setr(mldhip_prec* p, const char* what, double val, int ilev, int ilmax, const char* pos) {
+  if (!p) return E_BAD_ARG;
+  return mld::prec_set_real(p->host, S(what), val, ilev, ilmax, S(pos));
+}
+int mldhip_set_precision(mldhip_prec* p, int kind) {
+  if (!p || (kind != 0 && kind != 1)) return E_BAD_ARG;
+  p->host.value_kind = kind;
+  return 0;
+}
+int mldhip_set_format(mldhip_prec* p, int fmt) {
+  if (!p || fmt < 0 || fmt > 2) return E_BAD_ARG;
+  p->host.dev_format = fmt;
+  return 0;
+}
+
+int mldhip_hierarchy_build(mldhip_prec* p, int n, const int64_t* irp, const int* ja, const double* val) {
+  if (!p || n < 0 || !irp || (n > 0 && (!ja || !val))) return E_BAD_ARG;
+  p->dev.reset();
+  int rc = mld::hierarchy_build(p->host, n, irp, ja, val);
+  if (rc) return fail(p, rc, p->host.last_error.empty() ? "hierarchy_build failed" : p->host.last_error);
+  return 0;
+}
+
+int mldhip_smoothers_build(mldhip_prec* p) {
+  if (!p) return E_BAD_ARG;
+  p->dev.reset();
+  int rc = mld::smoothers_build(p->host);
+  if (rc) return fail(p, rc, "smoothers_build failed");
+  if (p->device < 0) return 0;
+  return mld::dev_guard(
+      [&]() -> int {
+        if (hipSetDevice(p->device) != hipSuccess) return E_DEVICE;
+        if (!p->stream) {
+          if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return E_DEVICE;
+          p->own_stream = true;
+        }
+        p->dev.reset(mld::make_dev_prec(p->host.value_kind, p->device, p->stream));
+        return p->dev->upload(p->host);
+      },
+      p->err);
+}
+
+int mldhip_build(mldhip_prec* p, int n, const int64_t* irp, const int* ja, const double* val) {
+  int rc = mldhip_hierarchy_build(p, n, irp, ja, val);
+  if (rc) return rc;
+  return mldhip_smoothers_build(p);
+}
+
+static int need_dev(mldhip_prec* p) {
+  if (!p) return E_BAD_ARG;
+  if (!p->host.smoothers_built) return fail(p, E_NOT_BUILT, "preconditioner not built");
+  if (!p->dev) return fail(p, E_DEVICE, "no device copy of the preconditioner");
+  return 0;
+}
+
+int mldhip_apply(mldhip_prec* p, const double* x, double* y, char trans) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  if (trans == 'n') trans = 'N';
+  if (trans == 't' || trans == 'C' || trans == 'c') trans = 'T';
+  if (trans != 'N' && trans != 'T') return fail(p, E_BAD_ARG, "invalid trans");
+  return mld::dev_guard([&]() { return p->dev->apply(x, y, trans); }, p->err);
+}
+
+int mldhip_apply_host(mldhip_prec* p, const double* x, double* y, char trans) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  const size_t n = (size_t)p->host.precv[0].a.nr;
+  return mld::dev_guard(
+      [&]() -> int {
+        double *dx = nullptr, *dy = nullptr;
+        if (hipMalloc(&dx, n * sizeof(double) + 8) != hipSuccess) return E_DEVICE;
+        if (hipMalloc(&dy, n * sizeof(double) + 8) != hipSuccess) { (void)hipFree(dx); return E_DEVICE; }
+        int r = 0;
+        if (hipMemcpyAsync(dx, x, n * sizeof(double), hipMemcpyHostToDevice, p->stream) != hipSuccess) r = E_DEVICE;
+        if (!r) r = mldhip_apply(p, dx, dy, trans);
+        if (!r && hipMemcpyAsync(y, dy, n * sizeof(double), hipMemcpyDeviceToHost, p->stream) != hipSuccess) r = E_DEVICE;
+        if (hipStreamSynchronize(p->stream) != hipSuccess) r = E_DEVICE;
+        (void)hipFree(dx);
+        (void)hipFree(dy);
+        return r;
+      },
+      p->err);
+}
+
+static int method_code(const char* m) {
+  std::string s = S(m);
+  for (auto& c : s) c = (char)toupper((unsigned char)c);
+  if (s == "CG") return 0;
+  if (s == "FCG") return 1;
+  if (s == "BICGSTAB") return 2;
+  return -1;
+}
+
+int mldhip_krylov(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,
+                  int istop, int* iter, double* err) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  int mc = method_code(method);
+  if (mc < 0) return fail(p, mld::E_UNSUPPORTED, "Krylov method not provided: " + S(method));
+  if (!iter || !err) return E_BAD_ARG;
+  return mld::dev_guard([&]() { return p->dev->krylov(mc, b, x, eps, itmax, istop, iter, err); }, p->err);
+}
+
+int mldhip_krylov_host(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,
+                       int istop, int* iter, double* err) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  const size_t n = (size_t)p->host.precv[0].a.nr;
+  return mld::dev_guard(
+      [&]() -> int {
+        double *db = nullptr, *dx = nullptr;
+        if (hipMalloc(&db, n * sizeof(double) + 8) != hipSuccess) return E_DEVICE;
+        if (hipMalloc(&dx, n * sizeof(double) + 8) != hipSuccess) { (void)hipFree(db); return E_DEVICE; }
+        int r = 0;
+        if (hipMemcpy(db, b, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) r = E_DEVICE;
+        if (!r && hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) r = E_DEVICE;
+        if (!r) r = mldhip_krylov(p, method, db, dx, eps, itmax, istop, iter, err);
+        if (!r && hipMemcpy(x, dx, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) r = E_DEVICE;
+        (void)hipFree(db);
+        (void)hipFree(dx);
+        return r;
+      },
+      p->err);
+}
+
+int mldhip_set_stream(mldhip_prec* p, void* s) {
+  if (!p) return E_BAD_ARG;
+  if (p->own_stream && p->stream && p->stream != (hipStream_t)s) {
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipStreamDestroy(p->stream);
+  }
+  p->stream = (hipStream_t)s;
+  p->own_stream = false;
+  if (p->dev) p->dev->set_stream(p->stream);
+  return 0;
+}
+
+int mldhip_synchronize(mldhip_prec* p) {
+  if (!p) return E_BAD_ARG;
+  if (!p->dev) return 0;
+  return mld::dev_guard([&]() { return p->dev->sync(); }, p->err);
+}
+
+int mldhip_get_nlev(mldhip_prec* p, int* nlev) {
+  if (!p || !nlev) return E_BAD_ARG;
+  *nlev = (int)p->host.precv.size();
+  return 0;
+}
+
+int mldhip_get_level_info(mldhip_prec* p, int lev, int64_t* info) {
+  if (!p || !info) return E_BAD_ARG;
+  if (lev < 0 || lev >= (int)p->host.precv.size()) return E_BAD_LEVEL;
+  const mld::Level& L = p->host.precv[lev];
+  info[0] = L.a.nr;
+  info[1] = L.a.nnz();
+  info[2] = L.p.nnz();
+  info[3] = L.r.nnz();
+  info[4] = L.smd.nnz_nd;
+  info[5] = L.naggr;
+  info[6] = L.parms.sweeps_pre;
+  info[7] = L.parms.sweeps_post;
+  info[8] = L.sm.allocated ? L.sm.solver : -1;
+  info[9] = L.sm2a.allocated ? L.sm2a.solver : -1;
+  return 0;
+}
+
+int mldhip_get_csr_size(mldhip_prec* p, int lev, int what, int64_t* nr, int64_t* nc, int64_t* nnz) {
+  const mld::HCsr* m = level_csr(p, lev, what);
+  if (!m) return E_BAD_ARG;
+  if (nr) *nr = m->nr;
+  if (nc) *nc = m->nc;
+  if (nnz) *nnz = m->nnz();
+  return 0;
+}
+
+int mldhip_get_level_csr(mldhip_prec* p, int lev, int what, int64_t* irp, int* ja, double* val) {
+  const mld::HCsr* m = level_csr(p, lev, what);
+  if (!m) return E_BAD_ARG;
+  if (irp) {
+    if (m->irp.empty()) for (int i = 0; i <= m->nr; ++i) irp[i] = 0;
+    else std::memcpy(irp, m->irp.data(), m->irp.size() * sizeof(int64_t));
+  }
+  if (ja && !m->ja.empty()) std::memcpy(ja, m->ja.data(), m->ja.size() * sizeof(int));
+  if (val && !m->val.empty()) std::memcpy(val, m->val.data(), m->val.size() * sizeof(double));
+  return 0;
+}
+
+int mldhip_get_aggr(mldhip_prec* p, int lev, int* ilaggr) {
+  if (!p || !ilaggr) return E_BAD_ARG;
+  if (lev < 1 || lev >= (int)p->host.precv.size()) return E_BAD_LEVEL;
+  const auto& v = p->host.precv[lev].ilaggr;
+  std::memcpy(ilaggr, v.data(), v.size() * sizeof(int));
+  return 0;
+}
+
+int mldhip_get_level_dv(mldhip_prec* p, int lev, int which, double* dv) {
+  if (!p || !dv) return E_BAD_ARG;
+  if (lev < 0 || lev >= (int)p->host.precv.size()) return E_BAD_LEVEL;
+  const mld::Level& L = p->host.precv[lev];
+  const auto& v = (which == 1 && L.sm2a.allocated) ? L.smd2.dv : L.smd.dv;
+  if (!v.empty()) std::memcpy(dv, v.data(), v.size() * sizeof(double));
+  return 0;
+}
+
+int mldhip_get_omega(mldhip_prec* p, int lev, double* omega) {
+  if (!p || !omega) return E_BAD_ARG;
+  if (lev < 1 || lev >= (int)p->host.precv.size()) return E_BAD_LEVEL;
+  *omega = p->host.precv[lev].parms.aggr_omega_val;
+  return 0;
+}
+
+int mldhip_get_complexity(mldhip_prec* p, double* c) {
+  if (!p || !c) return E_BAD_ARG;
+  *c = mld::operator_complexity(p->host);
+  return 0;
+}
+
+int mldhip_get_device_bytes(mldhip_prec* p, int64_t* bytes) {
+  if (!p || !bytes) return E_BAD_ARG;
+  *bytes = p->dev ? (int64_t)p->dev->device_bytes() : 0;
+  return 0;
+}
+
+int mldhip_get_error(mldhip_prec* p, char* buf, int len) {
+  if (!p || !buf || len <= 0) return E_BAD_ARG;
+  std::strncpy(buf, p->err.c_str(), (size_t)len - 1);
+  buf[len - 1] = 0;
+  return 0;
+}
+
+int mldhip_level_spmv(mldhip_prec* p, int lev, int what, const double* x, double* y) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  return mld::dev_guard([&]() { return p->dev->spmv_level(lev, what, x, y); }, p->err);
+}
+
+int mldhip_level_smooth(mldhip_prec* p, int lev, int which, const double* x, int beta, double* y, int sweeps,
+                        char trans) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  if (beta != 0 && beta != 1) return E_BAD_ARG;
+  return mld::dev_guard([&]() { return p->dev->smooth_level(lev, x, beta, y, sweeps, trans, which); }, p->err);
+}
+
+int mldhip_time_step(mldhip_prec* p, int lev, int kind, int reps, double* avg_ms) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  if (!avg_ms) return E_BAD_ARG;
+  return mld::dev_guard([&]() { return p->dev->time_step(lev, kind, reps, avg_ms); }, p->err);
+}
+
+int mldhip_get_op_format(mldhip_prec* p, int lev, int what, int* fmt, int* lanes, int* width) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  if (!fmt || !lanes || !width) return E_BAD_ARG;
+  return p->dev->op_format(lev, what, fmt, lanes, width);
+}
+
+}  // extern "C"

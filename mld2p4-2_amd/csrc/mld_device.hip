@@ -1,0 +1,861 @@
+// Device side of the engine: upload of the host hierarchy to HBM (once), the
+// multilevel cycle of mld_dmlprec_aply (mlprec/impl/mld_dmlprec_aply.f90
+// :203-809) as a sequence of fused HIP kernels on one stream, and
+// device-resident CG / FCG / BiCGStab.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mld_device.hpp"
+#include "mld_kernels.hpp"
+
+namespace mld {
+
+using namespace mld::k;
+
+struct DevError : std::runtime_error {
+  explicit DevError(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define HIPCHECK(x)                                                                                 \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess)                                                                           \
+      throw DevError(std::string(#x) + ": " + hipGetErrorString(e_) + " at " + __FILE__ + ":" +     \
+                     std::to_string(__LINE__));                                                     \
+  } while (0)
+
+enum { FMT_CSR = 0, FMT_ELL = 1 };
+
+template <typename V>
+struct DMat {
+  int nr = 0, nc = 0;
+  int64_t nnz = 0;
+  int fmt = FMT_CSR, lanes = 1, w = 0, ld = 0;
+  int* irp = nullptr;
+  int* ja = nullptr;
+  V* val = nullptr;
+};
+
+template <typename V>
+struct DSmoother {
+  int smoother = JAC, solver = DIAG_SCALE;
+  int64_t nnz_nd = 0;
+  DMat<V> nd, ndt;     // ND and its transpose (uploaded for 'T' applies)
+  V* dv = nullptr;
+  // ILU(0): factors and level schedules; for 'T': transposed factors
+  DMat<V> l, u, lt, ut;
+  int* lrows = nullptr; int* urows = nullptr; int* lptr_d = nullptr; int* uptr_d = nullptr;
+  std::vector<int> lptr, uptr;
+  int* ltrows = nullptr; int* utrows = nullptr; int* ltptr_d = nullptr; int* utptr_d = nullptr;
+  std::vector<int> ltptr, utptr;
+  bool has_t = false;
+  // dense LU
+  double* lu = nullptr; int* piv = nullptr;
+};
+
+template <typename V>
+struct DLevel {
+  int n = 0;
+  DMat<V> a, at, p, r;  // at: A^T for 'T' applies
+  DSmoother<V> sm, sm2;
+  bool has_sm2 = false;
+  int sweeps_pre = 1, sweeps_post = 1, ml_cycle = VCYCLE;
+  V *x2l = nullptr, *y2l = nullptr, *tx = nullptr, *ty = nullptr, *t1 = nullptr, *t2 = nullptr;
+  double* wlu = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// Allocation bookkeeping: every device allocation of a handle is freed with it.
+// ---------------------------------------------------------------------------
+struct DevArena {
+  std::vector<void*> ptrs;
+  size_t bytes = 0;
+  template <typename T>
+  T* alloc(size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    HIPCHECK(hipMalloc(&p, n * sizeof(T)));
+    ptrs.push_back(p);
+    bytes += n * sizeof(T);
+    return (T*)p;
+  }
+  template <typename T>
+  T* upload(const T* h, size_t n, hipStream_t s) {
+    T* d = alloc<T>(n);
+    if (n) HIPCHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return d;
+  }
+  ~DevArena() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
+
+static inline int nblk(int64_t n) { return (int)((n + BS - 1) / BS); }
+
+// ---------------------------------------------------------------------------
+// Storage choice.  ELL when the widest row is short and padding is small
+// (the 7-point fine level: 0.3% padding), else CSR with a power-of-two
+// number of lanes per row matched to the mean row length.
+// ---------------------------------------------------------------------------
+struct FmtChoice { int fmt, lanes, w; };
+
+static FmtChoice choose_format(const HCsr& h, int force) {
+  int maxrow = 0;
+  for (int i = 0; i < h.nr; ++i) maxrow = std::max(maxrow, (int)(h.irp[i + 1] - h.irp[i]));
+  const double nnz = (double)h.nnz();
+  const double avg = h.nr ? nnz / h.nr : 0.0;
+  FmtChoice c{FMT_CSR, 1, maxrow};
+  bool ell_ok = maxrow > 0 && maxrow <= 16 && (double)h.nr * maxrow <= 1.10 * nnz;
+  if (force == 2 && maxrow > 0 && maxrow <= 64) ell_ok = true;
+  if (force == 1) ell_ok = false;
+  if (ell_ok) { c.fmt = FMT_ELL; return c; }
+  int lanes = 1;
+  while (lanes < 64 && lanes * 4 <= avg) lanes *= 2;
+  c.lanes = lanes;
+  return c;
+}
+
+template <typename V>
+static DMat<V> upload_mat(const HCsr& h, DevArena& ar, hipStream_t s, int force, int lanes_override = 0) {
+  DMat<V> m;
+  m.nr = h.nr; m.nc = h.nc; m.nnz = h.nnz();
+  FmtChoice c = choose_format(h, force);
+  if (lanes_override > 0) { c.fmt = FMT_CSR; c.lanes = lanes_override; }
+  m.fmt = c.fmt; m.lanes = c.lanes; m.w = c.w;
+  if (c.fmt == FMT_ELL) {
+    m.ld = h.nr;
+    std::vector<int> ja((size_t)m.ld * m.w);
+    std::vector<V> val((size_t)m.ld * m.w);
+    for (int i = 0; i < h.nr; ++i) {
+      int k = 0;
+      for (int64_t q = h.irp[i]; q < h.irp[i + 1]; ++q, ++k) {
+        ja[(size_t)k * m.ld + i] = h.ja[q];
+        val[(size_t)k * m.ld + i] = (V)h.val[q];
+      }
+      for (; k < m.w; ++k) {  // padding: zero value on the row's own column
+        ja[(size_t)k * m.ld + i] = std::min(i, h.nc - 1);
+        val[(size_t)k * m.ld + i] = (V)0;
+      }
+    }
+    m.ja = ar.upload(ja.data(), ja.size(), s);
+    m.val = ar.upload(val.data(), val.size(), s);
+    HIPCHECK(hipStreamSynchronize(s));
+  } else {
+    std::vector<int> irp(h.nr + 1);
+    for (int i = 0; i <= h.nr; ++i) irp[i] = (int)h.irp[i];
+    std::vector<V> val(h.val.size());
+    for (size_t q = 0; q < val.size(); ++q) val[q] = (V)h.val[q];
+    m.irp = ar.upload(irp.data(), irp.size(), s);
+    m.ja = ar.upload(h.ja.data(), h.ja.size(), s);
+    m.val = ar.upload(val.data(), val.size(), s);
+    HIPCHECK(hipStreamSynchronize(s));
+  }
+  return m;
+}
+
+static HCsr host_transpose(const HCsr& a) {
+  HCsr t;
+  t.nr = a.nc; t.nc = a.nr;
+  t.irp.assign((size_t)t.nr + 1, 0);
+  for (int64_t k = 0; k < a.nnz(); ++k) t.irp[a.ja[k] + 1]++;
+  for (int i = 0; i < t.nr; ++i) t.irp[i + 1] += t.irp[i];
+  t.ja.resize(a.nnz()); t.val.resize(a.nnz());
+  std::vector<int64_t> pos(t.irp.begin(), t.irp.end() - 1);
+  for (int i = 0; i < a.nr; ++i)
+    for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
+      int64_t q = pos[a.ja[k]]++;
+      t.ja[q] = i; t.val[q] = a.val[k];
+    }
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Generic sparse launch: dispatch on format / width / lanes.
+// ---------------------------------------------------------------------------
+template <typename V, class G, class E>
+static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s) {
+  if (m.nr == 0) return;
+  if (m.fmt == FMT_ELL) {
+    const int nb = nblk(m.nr);
+    switch (m.w) {
+      case 4: ell_kernel<V, 4><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e); break;
+      case 5: ell_kernel<V, 5><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e); break;
+      case 6: ell_kernel<V, 6><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e); break;
+      case 7: ell_kernel<V, 7><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e); break;
+      default: ell_kernel<V, 0><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e); break;
+    }
+  } else {
+    const int nb = nblk((int64_t)m.nr * m.lanes);
+    switch (m.lanes) {
+      case 1: csr_kernel<V, 1><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
+      case 2: csr_kernel<V, 2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
+      case 4: csr_kernel<V, 4><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
+      case 8: csr_kernel<V, 8><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
+      case 16: csr_kernel<V, 16><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
+      case 32: csr_kernel<V, 32><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
+      default: csr_kernel<V, 64><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
+    }
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+static int dot_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(nblk(n), 2048)); }
+
+// y = A x and the dot(s) u.y (and w.y) in one pass (Krylov SpMV).
+template <typename V, bool W2>
+static void spmv_dot_launch(const DMat<V>& m, const V* x, V* y, const V* u, const V* w2, double* partial,
+                            unsigned* counter, double* result, hipStream_t s) {
+  const int nb = dot_grid((int64_t)m.nr * (m.fmt == FMT_ELL ? 1 : m.lanes));
+  if (m.fmt == FMT_ELL) {
+    switch (m.w) {
+      case 7: ell_dot_kernel<V, 7, W2><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
+      case 6: ell_dot_kernel<V, 6, W2><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
+      default: ell_dot_kernel<V, 0, W2><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
+    }
+  } else {
+    switch (m.lanes) {
+      case 1: csr_dot_kernel<V, 1, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
+      case 2: csr_dot_kernel<V, 2, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
+      case 4: csr_dot_kernel<V, 4, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
+      case 8: csr_dot_kernel<V, 8, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
+      case 16: csr_dot_kernel<V, 16, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
+      case 32: csr_dot_kernel<V, 32, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
+      default: csr_dot_kernel<V, 64, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
+    }
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// The device preconditioner for one value type.
+// ---------------------------------------------------------------------------
+template <typename V>
+class DevPrecT : public DevPrec {
+ public:
+  DevPrecT(int device, hipStream_t stream) : device_(device), s_(stream) {}
+
+  int upload(const Prec& p) override {
+    HIPCHECK(hipSetDevice(device_));
+    const int nlev = (int)p.precv.size();
+    L_.resize(nlev);
+    outer_sweeps_ = p.outer_sweeps;
+    for (int l = 0; l < nlev; ++l) {
+      const Level& H = p.precv[l];
+      DLevel<V>& D = L_[l];
+      D.n = H.a.nr;
+      D.a = upload_mat<V>(H.a, ar_, s_, p.dev_format);
+      if (l > 0) {
+        D.p = upload_mat<V>(H.p, ar_, s_, p.dev_format);
+        D.r = upload_mat<V>(H.r, ar_, s_, p.dev_format);
+      }
+      D.sweeps_pre = H.parms.sweeps_pre;
+      D.sweeps_post = H.parms.sweeps_post;
+      D.ml_cycle = H.parms.ml_cycle;
+      upload_smoother(H.smd, D.sm, p.dev_format);
+      D.has_sm2 = H.sm2a.allocated;
+      if (D.has_sm2) upload_smoother(H.smd2, D.sm2, p.dev_format);
+      const size_t n = (size_t)D.n;
+      D.x2l = ar_.alloc<V>(n); D.y2l = ar_.alloc<V>(n);
+      D.tx = ar_.alloc<V>(n); D.ty = ar_.alloc<V>(n);
+      D.t1 = ar_.alloc<V>(n); D.t2 = ar_.alloc<V>(n);
+      if (H.smd.solver == DENSE_LU || (D.has_sm2 && H.smd2.solver == DENSE_LU)) D.wlu = ar_.alloc<double>(n);
+    }
+    // Krylov: FP64 copy of the fine matrix (shared with level 0 in FP64)
+    if constexpr (std::is_same<V, double>::value) {
+      akry_ = L_[0].a;
+    } else {
+      akry_ = upload_mat<double>(p.precv[0].a, ar_, s_, p.dev_format);
+    }
+    n0_ = L_[0].n;
+    partial_ = ar_.alloc<double>(4 * 2048);
+    counter_ = ar_.alloc<unsigned>(4);
+    scal_ = ar_.alloc<double>(S_NSCAL);
+    HIPCHECK(hipMemsetAsync(counter_, 0, 4 * sizeof(unsigned), s_));
+    HIPCHECK(hipMemsetAsync(scal_, 0, S_NSCAL * sizeof(double), s_));
+    HIPCHECK(hipHostMalloc((void**)&hscal_, S_NSCAL * sizeof(double), hipHostMallocDefault));
+    HIPCHECK(hipStreamSynchronize(s_));
+    hprec_ = &p;
+    return 0;
+  }
+
+  ~DevPrecT() override {
+    if (hscal_) (void)hipHostFree(hscal_);
+  }
+
+  size_t device_bytes() const override { return ar_.bytes; }
+
+  // y = op(M^-1) x (mld_dprecaply2_vect with alpha = 1, beta = 0).
+  int apply(const double* x, double* y, char trans) override {
+    HIPCHECK(hipSetDevice(device_));
+    if (trans != 'N') ensure_transposes();
+    const int nlev = (int)L_.size();
+    DLevel<V>& L0 = L_[0];
+    if (nlev == 1) {
+      const int nsw = std::max(L0.sweeps_pre, L0.sweeps_post);
+      const V* xin = in_vec(x, L0.x2l, L0.n);
+      V* yout = out_vec(y, L0.y2l);
+      int rc = smooth(L0, L0.sm, xin, 0, yout, nsw, trans);
+      if (rc) return rc;
+      finish_out(y, yout, L0.n);
+      return 0;
+    }
+    if (outer_sweeps_ > 1) return apply_outer(x, y, trans);
+    // level-0 input/output alias the caller's vectors in FP64
+    V* x0save = L0.x2l;
+    V* y0save = L0.y2l;
+    L0.x2l = const_cast<V*>(in_vec(x, L0.x2l, L0.n));
+    L0.y2l = out_vec(y, L0.y2l);
+    int rc = inner_mult(0, trans);
+    finish_out(y, L0.y2l, L0.n);
+    L0.x2l = x0save;
+    L0.y2l = y0save;
+    return rc;
+  }
+
+  int spmv_level(int lev, int what, const double* x, double* y) override {
+    if (!std::is_same<V, double>::value) return E_UNSUPPORTED;
+    if (lev < 0 || lev >= (int)L_.size()) return E_BAD_LEVEL;
+    DLevel<V>& D = L_[lev];
+    const DMat<V>* m = nullptr;
+    switch (what) {
+      case 0: m = &D.a; break;
+      case 1: m = lev > 0 ? &D.p : nullptr; break;
+      case 2: m = lev > 0 ? &D.r : nullptr; break;
+      case 3: m = &D.sm.nd; break;
+    }
+    if (!m) return E_BAD_ARG;
+    spmv_launch(*m, GatherX<V, double>{x}, EpiSet<V, double>{y}, s_);
+    return 0;
+  }
+
+  int smooth_level(int lev, const double* x, int beta, double* y, int sweeps, char trans, int which) override {
+    if (!std::is_same<V, double>::value) return E_UNSUPPORTED;
+    if (lev < 0 || lev >= (int)L_.size()) return E_BAD_LEVEL;
+    if (trans != 'N') ensure_transposes();
+    DLevel<V>& D = L_[lev];
+    DSmoother<V>& S = (which == 1 && D.has_sm2) ? D.sm2 : D.sm;
+    return smooth(D, S, (const V*)x, beta, (V*)y, sweeps, trans);
+  }
+
+  int krylov(int method, const double* b, double* x, double eps, int itmax, int istop, int* iter,
+             double* err) override;
+
+  int sync() override {
+    HIPCHECK(hipStreamSynchronize(s_));
+    return 0;
+  }
+
+  int op_format(int lev, int what, int* fmt, int* lanes, int* width) override {
+    if (lev < 0 || lev >= (int)L_.size()) return E_BAD_LEVEL;
+    DLevel<V>& D = L_[lev];
+    const DMat<V>* m = nullptr;
+    switch (what) {
+      case 0: m = &D.a; break;
+      case 1: m = &D.p; break;
+      case 2: m = &D.r; break;
+      case 3: m = &D.sm.nd; break;
+      default: return E_BAD_ARG;
+    }
+    *fmt = m->fmt; *lanes = m->lanes; *width = m->w;
+    return 0;
+  }
+
+  int time_step(int lev, int kind, int reps, double* avg_ms) override {
+    HIPCHECK(hipSetDevice(device_));
+    if (lev < 0 || lev >= (int)L_.size() || reps <= 0) return E_BAD_LEVEL;
+    if ((kind == 2 || kind == 3) && lev == 0) return E_BAD_LEVEL;
+    if (kind == 5 && lev != 0) return E_BAD_LEVEL;
+    DLevel<V>& D = L_[lev];
+    // deterministic, finite operands
+    fill_kernel<V><<<nblk(D.n), BS, 0, s_>>>(D.n, D.x2l, (V)1);
+    fill_kernel<V><<<nblk(D.n), BS, 0, s_>>>(D.n, D.y2l, (V)0.5);
+    fill_kernel<V><<<nblk(D.n), BS, 0, s_>>>(D.n, D.ty, (V)0.25);
+    if (lev > 0) {
+      DLevel<V>& F = L_[lev - 1];
+      fill_kernel<V><<<nblk(F.n), BS, 0, s_>>>(F.n, F.ty, (V)0.25);
+      fill_kernel<V><<<nblk(F.n), BS, 0, s_>>>(F.n, F.y2l, (V)0.5);
+      fill_kernel<V><<<nblk(D.n), BS, 0, s_>>>(D.n, D.y2l, (V)0.5);
+    }
+    double* kx = nullptr;
+    double* kq = nullptr;
+    if (kind == 5) {
+      kx = kvec(2);
+      kq = kvec(3);
+      fill_kernel<double><<<nblk(n0_), BS, 0, s_>>>(n0_, kx, 1.0);
+    }
+    HIPCHECK(hipGetLastError());
+    auto run = [&]() -> int {
+      switch (kind) {
+        case 0: residual(D, D.x2l, D.y2l, D.ty, 'N'); return 0;
+        case 1: return smooth(D, D.sm, D.x2l, 0, D.y2l, D.sweeps_pre, 'N');
+        case 2: {
+          DLevel<V>& F = L_[lev - 1];
+          spmv_launch(D.r, GatherX<V, V>{F.ty}, EpiSet<V, V>{D.x2l}, s_);
+          return 0;
+        }
+        case 3: {
+          DLevel<V>& F = L_[lev - 1];
+          spmv_launch(D.p, GatherX<V, V>{D.y2l}, EpiAdd<V>{F.y2l}, s_);
+          return 0;
+        }
+        case 4: return smooth(D, D.has_sm2 ? D.sm2 : D.sm, D.ty, 1, D.y2l, D.sweeps_post, 'N');
+        case 5:
+          spmv_dot_launch<double, false>(akry_, kx, kq, kx, nullptr, partial_, counter_, scal_ + S_TMP0, s_);
+          return 0;
+      }
+      return E_BAD_ARG;
+    };
+    int rc = run();  // warm-up
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, s_));
+    for (int i = 0; i < reps && !rc; ++i) rc = run();
+    HIPCHECK(hipEventRecord(e1, s_));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / reps;
+    return rc;
+  }
+  void set_stream(hipStream_t s) override { s_ = s; }
+  hipStream_t stream() const override { return s_; }
+
+ private:
+  int device_;
+  hipStream_t s_;
+  DevArena ar_;
+  std::vector<DLevel<V>> L_;
+  int outer_sweeps_ = 1;
+  int n0_ = 0;
+  DMat<double> akry_;
+  double* partial_ = nullptr;
+  unsigned* counter_ = nullptr;
+  double* scal_ = nullptr;
+  double* hscal_ = nullptr;
+  const Prec* hprec_ = nullptr;
+  bool transposes_ = false;
+  std::vector<double*> kvec_;  // Krylov work vectors
+  double* kr_ = nullptr;
+
+  // --- vector plumbing between the FP64 interface and the V-typed cycle
+  const V* in_vec(const double* x, V* buf, int n) {
+    if (std::is_same<V, double>::value) return (const V*)x;
+    convert_kernel<V, double><<<nblk(n), BS, 0, s_>>>(n, x, buf);
+    HIPCHECK(hipGetLastError());
+    return buf;
+  }
+  V* out_vec(double* y, V* buf) {
+    if (std::is_same<V, double>::value) return (V*)y;
+    return buf;
+  }
+  void finish_out(double* y, const V* buf, int n) {
+    if (std::is_same<V, double>::value) return;
+    convert_kernel<double, V><<<nblk(n), BS, 0, s_>>>(n, buf, y);
+    HIPCHECK(hipGetLastError());
+  }
+
+  void upload_smoother(const SmootherData& H, DSmoother<V>& S, int force) {
+    S.smoother = H.smoother;
+    S.solver = H.solver;
+    S.nnz_nd = H.nnz_nd;
+    if (H.smoother == NOPREC) return;
+    if (!H.dv.empty()) {
+      std::vector<V> dv(H.dv.begin(), H.dv.end());
+      S.dv = ar_.upload(dv.data(), dv.size(), s_);
+      HIPCHECK(hipStreamSynchronize(s_));
+    }
+    if (H.nnz_nd > 0) S.nd = upload_mat<V>(H.nd, ar_, s_, force);
+    if (H.solver == ILU_N) {
+      S.l = upload_mat<V>(H.l, ar_, s_, 1, 1);  // CSR, one lane per row (stored order)
+      S.u = upload_mat<V>(H.u, ar_, s_, 1, 1);
+      S.lrows = ar_.upload(H.lsched_rows.data(), H.lsched_rows.size(), s_);
+      S.urows = ar_.upload(H.usched_rows.data(), H.usched_rows.size(), s_);
+      S.lptr = H.lsched_ptr;
+      S.uptr = H.usched_ptr;
+      S.lptr_d = ar_.upload(H.lsched_ptr.data(), H.lsched_ptr.size(), s_);
+      S.uptr_d = ar_.upload(H.usched_ptr.data(), H.usched_ptr.size(), s_);
+      HIPCHECK(hipStreamSynchronize(s_));
+    }
+    if (H.solver == DENSE_LU) {
+      S.lu = ar_.upload(H.lu.data(), H.lu.size(), s_);
+      S.piv = ar_.upload(H.piv.data(), H.piv.size(), s_);
+      HIPCHECK(hipStreamSynchronize(s_));
+    }
+  }
+
+  // Transposed operators for trans = 'T' (A^T, ND^T, U^T and L^T).
+  void ensure_transposes() {
+    if (transposes_) return;
+    const Prec& p = *hprec_;
+    for (size_t l = 0; l < L_.size(); ++l) {
+      const Level& H = p.precv[l];
+      DLevel<V>& D = L_[l];
+      D.at = upload_mat<V>(host_transpose(H.a), ar_, s_, p.dev_format);
+      auto tsm = [&](const SmootherData& hs, DSmoother<V>& ds) {
+        if (hs.nnz_nd > 0) ds.ndt = upload_mat<V>(host_transpose(hs.nd), ar_, s_, p.dev_format);
+        if (hs.solver == ILU_N) {
+          HCsr ut = host_transpose(hs.u), lt = host_transpose(hs.l);  // U^T lower, L^T upper
+          std::vector<int> p1, r1, p2, r2;
+          level_schedule(ut, true, p1, r1);
+          level_schedule(lt, false, p2, r2);
+          ds.ut = upload_mat<V>(ut, ar_, s_, 1, 1);
+          ds.lt = upload_mat<V>(lt, ar_, s_, 1, 1);
+          ds.utrows = ar_.upload(r1.data(), r1.size(), s_);
+          ds.ltrows = ar_.upload(r2.data(), r2.size(), s_);
+          ds.utptr = p1; ds.ltptr = p2;
+          ds.utptr_d = ar_.upload(p1.data(), p1.size(), s_);
+          ds.ltptr_d = ar_.upload(p2.data(), p2.size(), s_);
+          HIPCHECK(hipStreamSynchronize(s_));
+        }
+        ds.has_t = true;
+      };
+      tsm(H.smd, D.sm);
+      if (D.has_sm2) tsm(H.smd2, D.sm2);
+    }
+    transposes_ = true;
+  }
+
+  // ---------------------------------------------------------------- solver
+  // y = S^-1 x (beta 0) or y = y + S^-1 x (beta 1).
+  int solver_apply(DLevel<V>& D, DSmoother<V>& S, const V* x, int beta, V* y, char trans) {
+    const int n = D.n;
+    switch (S.solver) {
+      case F_NONE:
+        if (beta) add_kernel<V, V, true><<<nblk(n), BS, 0, s_>>>(n, x, y);
+        else add_kernel<V, V, false><<<nblk(n), BS, 0, s_>>>(n, x, y);
+        break;
+      case DIAG_SCALE:
+      case L1_DIAG:
+        if (beta) vmul_kernel<V, V, true><<<nblk(n), BS, 0, s_>>>(n, S.dv, x, y);
+        else vmul_kernel<V, V, false><<<nblk(n), BS, 0, s_>>>(n, S.dv, x, y);
+        break;
+      case ILU_N: {
+        const bool t = (trans != 'N');
+        const DMat<V>& F1 = t ? S.ut : S.l;   // first (lower-triangular) solve
+        const DMat<V>& F2 = t ? S.lt : S.u;   // second (upper-triangular) solve
+        const std::vector<int>& p1 = t ? S.utptr : S.lptr;
+        const std::vector<int>& p2 = t ? S.ltptr : S.uptr;
+        const int* r1 = t ? S.utrows : S.lrows;
+        const int* r2 = t ? S.ltrows : S.urows;
+        const int* p1d = t ? S.utptr_d : S.lptr_d;
+        const int* p2d = t ? S.ltptr_d : S.uptr_d;
+        const int nl1 = (int)p1.size() - 1, nl2 = (int)p2.size() - 1;
+        V* t1 = D.tx;
+        V* t2 = beta ? D.t2 : y;
+        if (n <= 8192) {
+          if (beta)
+            ilu_small_kernel<V, 1><<<1, 1024, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d, r1,
+                                                       nl1, p2d, r2, nl2, x, t1, t2, y);
+          else
+            ilu_small_kernel<V, 0><<<1, 1024, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d, r1,
+                                                       nl1, p2d, r2, nl2, x, t1, t2, y);
+          HIPCHECK(hipGetLastError());
+        } else {
+          for (int lv = 0; lv < nl1; ++lv) {
+            const int cnt = p1[lv + 1] - p1[lv];
+            trsv_level_kernel<V, false, 2><<<nblk(cnt), BS, 0, s_>>>(cnt, r1 + p1[lv], F1.irp, F1.ja, F1.val, x,
+                                                                    nullptr, t1, nullptr);
+          }
+          for (int lv = 0; lv < nl2; ++lv) {
+            const int cnt = p2[lv + 1] - p2[lv];
+            if (beta)
+              trsv_level_kernel<V, true, 1><<<nblk(cnt), BS, 0, s_>>>(cnt, r2 + p2[lv], F2.irp, F2.ja, F2.val, t1,
+                                                                     S.dv, t2, y);
+            else
+              trsv_level_kernel<V, true, 2><<<nblk(cnt), BS, 0, s_>>>(cnt, r2 + p2[lv], F2.irp, F2.ja, F2.val, t1,
+                                                                     S.dv, t2, y);
+          }
+          HIPCHECK(hipGetLastError());
+        }
+        break;
+      }
+      case DENSE_LU:
+        if (trans != 'N') return E_UNSUPPORTED;
+        if (beta) dense_lu_small_kernel<V, 1><<<1, 1024, 0, s_>>>(n, S.lu, S.piv, x, D.wlu, y);
+        else dense_lu_small_kernel<V, 0><<<1, 1024, 0, s_>>>(n, S.lu, S.piv, x, D.wlu, y);
+        HIPCHECK(hipGetLastError());
+        break;
+      default:
+        return E_UNSUPPORTED;
+    }
+    return 0;
+  }
+
+  // One Jacobi sweep y_out = D^-1 (b - ND y_in) (FIRST: y_in = D^-1 b).
+  template <bool FIRST, bool ADD>
+  void jac_sweep(DSmoother<V>& S, const V* b, const V* yin, V* yout, char trans) {
+    const DMat<V>& nd = (trans != 'N') ? S.ndt : S.nd;
+    EpiJac<V, V, V, ADD> e{S.dv, b, yout};
+    if (FIRST) spmv_launch(nd, GatherDX<V, V>{S.dv, b}, e, s_);
+    else spmv_launch(nd, GatherX<V, V>{yin}, e, s_);
+  }
+
+  // mld_d_jac_smoother_apply_vect (init 'Z'), alpha = 1, beta in {0,1}.
+  int smooth(DLevel<V>& D, DSmoother<V>& S, const V* x, int beta, V* y, int sweeps, char trans) {
+    const int n = D.n;
+    if (S.smoother == NOPREC) {
+      if (beta) add_kernel<V, V, true><<<nblk(n), BS, 0, s_>>>(n, x, y);
+      else add_kernel<V, V, false><<<nblk(n), BS, 0, s_>>>(n, x, y);
+      return 0;
+    }
+    if (sweeps < 0) return E_BAD_ARG;
+    if (sweeps <= 1 || S.nnz_nd == 0) return solver_apply(D, S, x, beta, y, trans);
+    if (S.solver != DIAG_SCALE && S.solver != L1_DIAG) return E_UNSUPPORTED;
+    if (sweeps == 2) {
+      if (beta) jac_sweep<true, true>(S, x, nullptr, y, trans);
+      else jac_sweep<true, false>(S, x, nullptr, y, trans);
+      return 0;
+    }
+    V* ta = D.tx;
+    V* tb = D.t1;
+    jac_sweep<true, false>(S, x, nullptr, ta, trans);
+    for (int i = 2; i <= sweeps - 2; ++i) {
+      jac_sweep<false, false>(S, x, ta, tb, trans);
+      std::swap(ta, tb);
+    }
+    if (beta) jac_sweep<false, true>(S, x, ta, y, trans);
+    else jac_sweep<false, false>(S, x, ta, y, trans);
+    return 0;
+  }
+
+  void residual(DLevel<V>& D, const V* b, const V* yv, V* r, char trans) {
+    const DMat<V>& a = (trans != 'N') ? D.at : D.a;
+    spmv_launch(a, GatherX<V, V>{yv}, EpiRes<V, V>{b, r}, s_);
+  }
+
+  // mld_d_inner_mult (mld_dmlprec_aply.f90:590-809)
+  int inner_mult(int level, char trans) {
+    const int nlev = (int)L_.size();
+    DLevel<V>& D = L_[level];
+    const bool tn = (trans == 'N');
+    const bool pre = (D.sweeps_pre > 0 && tn) || (D.sweeps_post > 0 && !tn);
+    const bool post = (D.sweeps_post > 0 && tn) || (D.sweeps_pre > 0 && !tn);
+    DSmoother<V>& Spre = (tn || !D.has_sm2) ? D.sm : D.sm2;
+    DSmoother<V>& Spost = (tn && D.has_sm2) ? D.sm2 : D.sm;
+    int rc;
+    if (level < nlev - 1) {
+      DLevel<V>& C = L_[level + 1];
+      if (pre) {
+        rc = smooth(D, Spre, D.x2l, 0, D.y2l, tn ? D.sweeps_pre : D.sweeps_post, trans);
+        if (rc) return rc;
+        residual(D, D.x2l, D.y2l, D.ty, trans);
+        spmv_launch(C.r, GatherX<V, V>{D.ty}, EpiSet<V, V>{C.x2l}, s_);
+      } else {
+        fill_kernel<V><<<nblk(D.n), BS, 0, s_>>>(D.n, D.y2l, (V)0);
+        spmv_launch(C.r, GatherX<V, V>{D.x2l}, EpiSet<V, V>{C.x2l}, s_);
+      }
+      rc = inner_mult(level + 1, trans);
+      if (rc) return rc;
+      spmv_launch(C.p, GatherX<V, V>{C.y2l}, EpiAdd<V>{D.y2l}, s_);
+      if (D.ml_cycle == WCYCLE) {
+        residual(D, D.x2l, D.y2l, D.ty, trans);
+        spmv_launch(C.r, GatherX<V, V>{D.ty}, EpiSet<V, V>{C.x2l}, s_);
+        rc = inner_mult(level + 1, trans);
+        if (rc) return rc;
+        spmv_launch(C.p, GatherX<V, V>{C.y2l}, EpiAdd<V>{D.y2l}, s_);
+      }
+      if (post) {
+        residual(D, D.x2l, D.y2l, D.ty, trans);
+        rc = smooth(D, Spost, D.ty, 1, D.y2l, tn ? D.sweeps_post : D.sweeps_pre, trans);
+        if (rc) return rc;
+      }
+      return 0;
+    }
+    return smooth(D, D.sm, D.x2l, 0, D.y2l, D.sweeps_pre, trans);
+  }
+
+  // outer_sweeps > 1 (mld_dmlprec_aply.f90:285-327), with internal buffers.
+  int apply_outer(const double* x, double* y, char trans) {
+    DLevel<V>& L0 = L_[0];
+    const int n = L0.n;
+    V* xin = L0.x2l;  // holds x converted to V for the whole call
+    if (!xres_) xres_ = ar_.alloc<V>(n);
+    convert_kernel<V, double><<<nblk(n), BS, 0, s_>>>(n, x, xres_);
+    V* ysum = L0.t2;
+    for (int isw = 1; isw <= outer_sweeps_; ++isw) {
+      if (isw == 1) {
+        add_kernel<V, V, false><<<nblk(n), BS, 0, s_>>>(n, xres_, xin);
+      } else {
+        // x2l = x - A y
+        spmv_launch(L0.a, GatherX<V, V>{ysum}, EpiRes<V, V>{xres_, xin}, s_);
+      }
+      int rc = inner_mult(0, trans);
+      if (rc) return rc;
+      if (isw == 1) add_kernel<V, V, false><<<nblk(n), BS, 0, s_>>>(n, L0.y2l, ysum);
+      else add_kernel<V, V, true><<<nblk(n), BS, 0, s_>>>(n, L0.y2l, ysum);
+    }
+    convert_kernel<double, V><<<nblk(n), BS, 0, s_>>>(n, ysum, y);
+    HIPCHECK(hipGetLastError());
+    return 0;
+  }
+  V* xres_ = nullptr;
+
+  // --------------------------------------------------------------- Krylov
+  double* kvec(int i) {
+    while ((int)kvec_.size() <= i) kvec_.push_back(ar_.alloc<double>(n0_));
+    return kvec_[i];
+  }
+  double read_scal(int slot) {
+    HIPCHECK(hipMemcpyAsync(hscal_, scal_, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, s_));
+    HIPCHECK(hipStreamSynchronize(s_));
+    return hscal_[slot];
+  }
+  void dot1(const double* a, const double* b, int slot) {
+    dot_kernel<double, 1><<<dot_grid(n0_), BS, 0, s_>>>(n0_, a, b, nullptr, nullptr, partial_, counter_, scal_ + slot);
+    HIPCHECK(hipGetLastError());
+  }
+  void axpby(double a, const double* x, double b, double* y) {
+    axpby_kernel<double><<<nblk(n0_), BS, 0, s_>>>(n0_, a, x, b, y);
+    HIPCHECK(hipGetLastError());
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Krylov solvers, all vectors device-resident in FP64.  The preconditioner
+// is applied through the same apply() the C-ABI exposes (the psb_krylov ->
+// prec%apply call site).  Convergence test (istop = 2): ||r||_2/||b||_2 <= eps
+// after each iteration; one 8-byte device-to-host read per iteration.
+// ---------------------------------------------------------------------------
+template <typename V>
+int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int itmax, int istop, int* iter,
+                        double* err) {
+  HIPCHECK(hipSetDevice(device_));
+  if (istop != 2) return E_UNSUPPORTED;
+  const int n = n0_;
+  double* r = kvec(0);
+  double* z = kvec(1);
+  double* p = kvec(2);
+  double* q = kvec(3);
+  // bn2 and r = b - A x
+  dot1(b, b, S_TMP0);
+  spmv_launch(akry_, GatherX<double, double>{x}, EpiRes<double, double>{b, r}, s_);
+  dot1(r, r, S_RR);
+  HIPCHECK(hipMemcpyAsync(hscal_, scal_, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, s_));
+  HIPCHECK(hipStreamSynchronize(s_));
+  double bn2 = std::sqrt(hscal_[S_TMP0]);
+  if (bn2 == 0.0) bn2 = 1.0;
+  *err = std::sqrt(hscal_[S_RR]) / bn2;
+  int it = 0;
+  if (*err <= eps) { *iter = 0; return 0; }
+  const int nbd = dot_grid(n);
+  if (method == 0) {  // CG
+    while (it < itmax) {
+      ++it;
+      int rc = apply(r, z, 'N');
+      if (rc) return rc;
+      dot1(r, z, (it & 1) ? S_RHO1 : S_RHO0);
+      cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, z, p, scal_, it);
+      spmv_dot_launch<double, false>(akry_, p, q, p, nullptr, partial_, counter_, scal_ + S_SIGMA, s_);
+      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, (it & 1) ? S_RHO1 : S_RHO0, S_SIGMA, partial_, counter_,
+                                   scal_ + S_RR);
+      HIPCHECK(hipGetLastError());
+      read_scal(S_RR);
+      *err = std::sqrt(hscal_[S_RR]) / bn2;
+      const double rho_new = hscal_[(it & 1) ? S_RHO1 : S_RHO0], rho_old = hscal_[(it & 1) ? S_RHO0 : S_RHO1];
+      if (*err <= eps) break;
+      if (hscal_[S_SIGMA] == 0.0 || (it > 1 && rho_old == 0.0) || rho_new == 0.0) break;
+    }
+  } else if (method == 1) {  // FCG(1)
+    while (it < itmax) {
+      ++it;
+      int rc = apply(r, z, 'N');
+      if (rc) return rc;
+      if (it > 1) dot1(z, q, S_GAMMA);
+      fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z, p, r, scal_, it, partial_, counter_, scal_ + S_PR);
+      spmv_dot_launch<double, false>(akry_, p, q, p, nullptr, partial_, counter_, scal_ + S_DELTA, s_);
+      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, S_PR, S_DELTA, partial_, counter_, scal_ + S_RR);
+      HIPCHECK(hipGetLastError());
+      read_scal(S_RR);
+      *err = std::sqrt(hscal_[S_RR]) / bn2;
+      if (*err <= eps) break;
+      if (hscal_[S_DELTA] == 0.0) break;
+    }
+  } else if (method == 2) {  // BiCGStab (right preconditioned)
+    double* rt = kvec(4);
+    double* s = kvec(5);
+    double* t = kvec(6);
+    double* ph = kvec(7);
+    double* sh = kvec(8);
+    double* v = q;
+    HIPCHECK(hipMemcpyAsync(rt, r, n * sizeof(double), hipMemcpyDeviceToDevice, s_));
+    double rho_old = 1.0, alpha = 1.0, omega = 1.0;
+    while (it < itmax) {
+      ++it;
+      dot1(rt, r, S_TMP0);
+      const double rho = read_scal(S_TMP0);
+      if (rho == 0.0) break;
+      if (it == 1) {
+        HIPCHECK(hipMemcpyAsync(p, r, n * sizeof(double), hipMemcpyDeviceToDevice, s_));
+      } else {
+        const double beta = (rho / rho_old) * (alpha / omega);
+        axpby(-omega, v, 1.0, p);
+        axpby(1.0, r, beta, p);
+      }
+      int rc = apply(p, ph, 'N');
+      if (rc) return rc;
+      spmv_dot_launch<double, false>(akry_, ph, v, rt, nullptr, partial_, counter_, scal_ + S_TMP1, s_);
+      const double rtv = read_scal(S_TMP1);
+      if (rtv == 0.0) break;
+      alpha = rho / rtv;
+      HIPCHECK(hipMemcpyAsync(s, r, n * sizeof(double), hipMemcpyDeviceToDevice, s_));
+      axpby(-alpha, v, 1.0, s);
+      dot1(s, s, S_TMP0);
+      const double sn = std::sqrt(read_scal(S_TMP0)) / bn2;
+      if (sn <= eps) {
+        axpby(alpha, ph, 1.0, x);
+        HIPCHECK(hipMemcpyAsync(r, s, n * sizeof(double), hipMemcpyDeviceToDevice, s_));
+        *err = sn;
+        break;
+      }
+      rc = apply(s, sh, 'N');
+      if (rc) return rc;
+      spmv_dot_launch<double, true>(akry_, sh, t, nullptr, s, partial_, counter_, scal_ + S_TMP0, s_);  // t.t, s.t
+      read_scal(S_TMP0);
+      const double tt = hscal_[S_TMP0], ts = hscal_[S_TMP1];
+      omega = (tt == 0.0) ? 0.0 : ts / tt;
+      axpby(alpha, ph, 1.0, x);
+      axpby(omega, sh, 1.0, x);
+      HIPCHECK(hipMemcpyAsync(r, s, n * sizeof(double), hipMemcpyDeviceToDevice, s_));
+      axpby(-omega, t, 1.0, r);
+      rho_old = rho;
+      dot1(r, r, S_RR);
+      *err = std::sqrt(read_scal(S_RR)) / bn2;
+      if (*err <= eps) break;
+      if (omega == 0.0) break;
+    }
+  } else {
+    return E_BAD_ARG;
+  }
+  HIPCHECK(hipStreamSynchronize(s_));
+  *iter = it;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+DevPrec* make_dev_prec(int value_kind, int device, hipStream_t stream) {
+  if (value_kind == 1) return new DevPrecT<float>(device, stream);
+  return new DevPrecT<double>(device, stream);
+}
+
+int dev_guard(const std::function<int()>& f, std::string& err) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    err = e.what();
+    return E_DEVICE;
+  }
+}
+
+}  // namespace mld

@@ -1,0 +1,38 @@
+// Interface between the C-ABI layer and the device engine (one value type
+// per instance: FP64, or FP32 for the mixed-precision form).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <string>
+
+#include "mld_host.hpp"
+
+namespace mld {
+
+class DevPrec {
+ public:
+  virtual ~DevPrec() {}
+  virtual int upload(const Prec& p) = 0;
+  virtual int apply(const double* x, double* y, char trans) = 0;
+  virtual int spmv_level(int lev, int what, const double* x, double* y) = 0;
+  virtual int smooth_level(int lev, const double* x, int beta, double* y, int sweeps, char trans, int which) = 0;
+  virtual int krylov(int method, const double* b, double* x, double eps, int itmax, int istop, int* iter,
+                     double* err) = 0;
+  virtual int sync() = 0;
+  virtual void set_stream(hipStream_t s) = 0;
+  virtual hipStream_t stream() const = 0;
+  virtual size_t device_bytes() const = 0;
+  // Average duration (ms) of one launch sequence of a single cycle step at
+  // level lev, timed with HIP events on the handle's stream over reps runs.
+  // kind: 0 residual r = x - A y, 1 pre smoother, 2 restriction R r,
+  //       3 prolongation y += P yc, 4 post smoother, 5 Krylov SpMV + dot.
+  virtual int time_step(int lev, int kind, int reps, double* avg_ms) = 0;
+  // Storage chosen for a level operator: fmt (0 CSR, 1 ELL), lanes, width.
+  virtual int op_format(int lev, int what, int* fmt, int* lanes, int* width) = 0;
+};
+
+DevPrec* make_dev_prec(int value_kind, int device, hipStream_t stream);
+int dev_guard(const std::function<int()>& f, std::string& err);
+
+}  // namespace mld

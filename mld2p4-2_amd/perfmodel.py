@@ -1,0 +1,66 @@
+"""Algorithmic (compulsory) HBM bytes of the cycle steps.
+
+Every array an operation must touch is counted once: matrix values and
+column indices (s_v + 4 bytes per stored entry of the unpadded operator),
+row pointers for operators stored as CSR, and each vector read or written
+once.  ELL padding and cache re-reads are not algorithmic and are not
+counted, so achieved GB/s = bytes / kernel time is a lower bound on the
+bandwidth the kernel actually moved.
+"""
+from __future__ import annotations
+
+DIAG_SCALE, ILU_N, L1_DIAG, DENSE_LU, F_NONE = 5, 7, 101, 102, 4
+
+
+def spmv_bytes(nr, nc, nnz, sv, sx, beta_read=False, csr=True):
+    b = nnz * (sv + 4) + nc * sx + nr * sx * (2 if beta_read else 1)
+    if csr:
+        b += (nr + 1) * 4
+    return b
+
+
+def residual_bytes(n, nnz, sv, sx, csr=False):
+    # r = b - A y : A, y (gathered, once), b, r
+    return nnz * (sv + 4) + 3 * n * sx + ((n + 1) * 4 if csr else 0)
+
+
+def smoother_bytes(n, solver, sweeps, nnz_nd, nnz_lu, sv, sx, beta, nd_csr=False):
+    """One sm%apply(1, x, beta, y, sweeps, init='Z')."""
+    yread = n * sx if beta else 0
+    if solver in (DIAG_SCALE, L1_DIAG):
+        if sweeps <= 1 or nnz_nd == 0:
+            return 2 * n * sx + n * sv + yread  # y = d*x
+        rp = (n + 1) * 4 if nd_csr else 0
+        first = nnz_nd * (sv + 4) + rp + n * (2 * sx + sv)  # D^-1(x - ND D^-1 x)
+        more = (sweeps - 2) * (nnz_nd * (sv + 4) + rp + n * (3 * sx + sv))
+        return first + more + yread
+    if solver == ILU_N:
+        return nnz_lu * (sv + 4) + 2 * (n + 1) * 4 + n * sv + 4 * n * sx + yread
+    if solver == DENSE_LU:
+        return n * n * 8 + n * 4 + 2 * n * sx + yread
+    return 2 * n * sx + yread
+
+
+def vcycle_bytes(levels, sv, sx):
+    """levels: list of dicts with n, nnz_a, nnz_p, nnz_r, nnz_nd, nnz_lu,
+    solver, sweeps_pre, sweeps_post, csr_a (bool).  P/R at level l map
+    level l-1 <-> l.  Returns (total, per-step list)."""
+    steps = []
+    nlev = len(levels)
+    for l, L in enumerate(levels):
+        n = L["n"]
+        if l < nlev - 1:
+            C = levels[l + 1]
+            nc = C["n"]
+            steps.append((l, "pre", smoother_bytes(n, L["solver"], L["sweeps_pre"], L["nnz_nd"], L.get("nnz_lu", 0),
+                                                   sv, sx, 0, L.get("csr_nd", False))))
+            steps.append((l, "residual", residual_bytes(n, L["nnz_a"], sv, sx, L.get("csr_a", False))))
+            steps.append((l + 1, "restrict", spmv_bytes(nc, n, C["nnz_r"], sv, sx)))
+            steps.append((l + 1, "prolong", spmv_bytes(n, nc, C["nnz_p"], sv, sx, beta_read=True)))
+            steps.append((l, "residual", residual_bytes(n, L["nnz_a"], sv, sx, L.get("csr_a", False))))
+            steps.append((l, "post", smoother_bytes(n, L["solver"], L["sweeps_post"], L["nnz_nd"], L.get("nnz_lu", 0),
+                                                    sv, sx, 1, L.get("csr_nd", False))))
+        else:
+            steps.append((l, "coarse", smoother_bytes(n, L["solver"], L["sweeps_pre"], L["nnz_nd"],
+                                                      L.get("nnz_lu", 0), sv, sx, 0)))
+    return sum(s[2] for s in steps), steps
