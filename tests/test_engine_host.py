@@ -1,0 +1,172 @@
+"""CPU tests of the engine library through its C-ABI (no GPU): exported
+symbols, the configuration API's routing, the host hierarchy build and
+smoother data against the oracle (bit-exact: integers and values), error
+codes, and the problem generator / byte model used by the benchmark."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def test_capi_exports_every_declared_symbol(eng):
+    hdr = open(os.path.join(ROOT, "include", "mld2p4_amd.h")).read()
+    names = set(re.findall(r"\bint\s+(mldhip_\w+)\s*\(", hdr))
+    assert len(names) >= 30
+    lib = eng.lib()
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def config1(eng, smoother="JACOBI", sweeps=1, coarse="BJAC", csweeps=4, mcs=200):
+    p = eng.MldPrec(device=-1).init("ML")
+    p.set("SMOOTHER_TYPE", smoother)
+    p.set("SMOOTHER_SWEEPS", sweeps)
+    p.set("AGGR_THRESH", 0.0)
+    p.set("AGGR_PROL", "SMOOTHED")
+    p.set("MIN_COARSE_SIZE", mcs)
+    p.set("COARSE_SOLVE", coarse)
+    if coarse == "BJAC":
+        p.set("COARSE_SUBSOLVE", "ILU")
+    p.set("COARSE_SWEEPS", csweeps)
+    return p
+
+
+def same(m1, m2):
+    return (m1.shape == m2.shape and (m1.indptr == m2.indptr).all() and (m1.indices == m2.indices).all()
+            and (m1.data == m2.data).all())
+
+
+SMC = {"JACOBI": 1, "L1-JACOBI": 100, "BJAC": 2}
+
+
+@pytest.mark.parametrize("idim", [7, 20, 40])
+@pytest.mark.parametrize("smoother", ["JACOBI", "L1-JACOBI", "BJAC"])
+def test_host_build_bitexact_vs_oracle(eng, orc, idim, smoother):
+    a = orc.gen_pde3d(idim)
+    p = config1(eng, smoother, 2).build(a.to_scipy())
+    po = orc.Prec(orc.config1_params(smoother=SMC[smoother], sweeps_pre=2, sweeps_post=2)).build(a)
+    assert p.nlev == po.nlev
+    for l in range(p.nlev):
+        assert same(p.level_csr(l, "A"), po.level_csr(l, "A").to_scipy())
+        if l > 0:
+            assert (p.aggr(l) == po.ilaggr(l)).all()
+            assert same(p.level_csr(l, "P"), po.level_csr(l, "P").to_scipy())
+            assert same(p.level_csr(l, "R"), po.level_csr(l, "R").to_scipy())
+            assert p.omega(l) == po.omega(l)
+        info = p.level_info(l)
+        if info["nnz_nd"] > 0:
+            assert same(p.level_csr(l, "ND"), po.level_csr(l, "ND").to_scipy())
+        assert (p.dv(l) == po.dv(l)).all()
+        if info["solver"] == 7:
+            assert same(p.level_csr(l, "L"), po.level_csr(l, "L").to_scipy())
+            assert same(p.level_csr(l, "U"), po.level_csr(l, "U").to_scipy())
+
+
+def test_host_build_golden_matrix(eng, orc):
+    A = orc.read_mtx(os.path.join(GOLD, "mld_mat.mtx"))
+    for theta in (0.0, 0.05):
+        p = config1(eng)
+        p.set("AGGR_THRESH", theta)
+        p.build(A)
+        po = orc.Prec(orc.config1_params(theta=theta)).build(orc.Csr.from_scipy(A))
+        assert p.nlev == po.nlev
+        for l in range(1, p.nlev):
+            assert (p.aggr(l) == po.ilaggr(l)).all()
+            assert same(p.level_csr(l, "A"), po.level_csr(l, "A").to_scipy())
+
+
+def test_config_routing(eng):
+    p = eng.MldPrec(device=-1).init("ML")
+    # ML defaults (mld_dprecinit.F90:188-207): FBGS smoother, coarse ILU
+    p.set("SMOOTHER_TYPE", "L1-JACOBI")
+    p.set("SMOOTHER_SWEEPS", 3, pos="post")
+    p.set("COARSE_SOLVE", "BJAC")
+    p.set("COARSE_SWEEPS", 4)
+    p.set("MIN_COARSE_SIZE", 50)
+    from mld2p4_amd import problems
+
+    irp, ja, val = problems.pde3d_poisson(12)
+    p.hierarchy_build((irp, ja, val)).smoothers_build()
+    nl = p.nlev
+    assert nl >= 2
+    for l in range(nl - 1):
+        i = p.level_info(l)
+        assert (i["solver"], i["sweeps_pre"], i["sweeps_post"]) == (101, 1, 3)
+    c = p.level_info(nl - 1)
+    assert (c["solver"], c["sweeps_pre"], c["sweeps_post"]) == (7, 4, 4)
+    assert c["n"] <= 50
+
+
+def test_error_codes(eng):
+    E = eng.MldError
+    p = eng.MldPrec(device=-1)
+    with pytest.raises(E) as ex:
+        p.hierarchy_build(((0, 1), (0,), (1.0,)))
+    assert ex.value.info == 3111
+    with pytest.raises(E) as ex:
+        p.init("NOT-A-PREC")
+    assert ex.value.info == 2
+    p.init("ML")  # default smoother FBGS: not provided by this engine
+    from mld2p4_amd import problems
+
+    A = problems.pde3d_poisson(6)
+    p.hierarchy_build(A)
+    with pytest.raises(E) as ex:
+        p.smoothers_build()
+    assert ex.value.info == -20
+    # unsorted CSR rows are rejected
+    q = eng.MldPrec(device=-1).init("JACOBI")
+    with pytest.raises(E) as ex:
+        q.hierarchy_build((np.array([0, 2, 3]), np.array([1, 0, 1]), np.array([1.0, 2.0, 3.0])))
+    assert ex.value.info == -1
+    # apply before build
+    r = eng.MldPrec(device=-1).init("JACOBI")
+    with pytest.raises(E) as ex:
+        r.apply(np.ones(3))
+    assert ex.value.info == 3112
+    # zero pivot in ILU(0)
+    s = eng.MldPrec(device=-1).init("BJAC")
+    s.hierarchy_build((np.array([0, 2, 4]), np.array([0, 1, 0, 1]), np.array([1.0, 1.0, 1.0, 1.0])))
+    with pytest.raises(E) as ex:
+        s.smoothers_build()
+    assert ex.value.info == 2
+    # built on a host-only handle: no device copy
+    t = eng.MldPrec(device=-1).init("JACOBI")
+    t.build(A).descr()
+    with pytest.raises(E) as ex:
+        t.apply(np.ones(216))
+    assert ex.value.info == -30
+
+
+def test_one_level_jacobi_and_zero_diagonal(eng):
+    # mld_d_diag_solver_bld.f90:86-92: a zero diagonal is replaced by 1
+    p = eng.MldPrec(device=-1).init("JACOBI")
+    p.build((np.array([0, 2, 3]), np.array([0, 1, 0]), np.array([4.0, 1.0, 2.0])))
+    assert (p.dv(0) == np.array([0.25, 1.0])).all()
+
+
+def test_problem_generator_matches_oracle(eng, orc):
+    from mld2p4_amd import problems
+
+    for idim in (3, 11):
+        irp, ja, val = problems.pde3d_poisson(idim)
+        io, jo, vo, _ = orc.gen_pde3d(idim).arrays()
+        assert (irp == io).all() and (ja == jo).all() and (val == vo).all()
+
+
+def test_byte_model(eng):
+    from mld2p4_amd import perfmodel
+
+    # fine-level residual at 256^3 (survey figure without row pointers: ELL)
+    n, nnz = 256 ** 3, 117047296
+    assert perfmodel.residual_bytes(n, nnz, 8, 8, csr=True) == 1874329604
+    # two fused l1-Jacobi sweeps from a zero guess: ND + x + d + y
+    assert perfmodel.smoother_bytes(10, 101, 2, 70, 0, 8, 8, 0) == 70 * 12 + 10 * 24
+    tot, steps = perfmodel.vcycle_bytes(
+        [dict(n=10, nnz_a=70, nnz_p=0, nnz_r=0, nnz_nd=70, solver=101, sweeps_pre=2, sweeps_post=2),
+         dict(n=3, nnz_a=9, nnz_p=20, nnz_r=20, nnz_nd=0, solver=7, sweeps_pre=4, sweeps_post=4, nnz_lu=6)], 8, 8)
+    assert len(steps) == 7 and tot == sum(s[2] for s in steps)

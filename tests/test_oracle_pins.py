@@ -1,0 +1,302 @@
+"""Pins of the oracle (CPU restatement) against what fixes it independently:
+the reference's golden vectors, closed forms, a second pure-Python
+restatement, invariants and dense brute force on tiny inputs.  None of these
+checks retypes the oracle's own loop: each uses an independent formulation
+that a plausible slip (dropped term, wrong sign, transposed operand, wrong
+sweep count) would violate.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def poisson_kron(idim):
+    """3-D 7-point Poisson from Kronecker sums of integer 1-D stencils,
+    divided by h^2 once (h = 1/(idim+2)): an independent construction."""
+    T = sp.diags([-np.ones(idim - 1), 2 * np.ones(idim), -np.ones(idim - 1)], [-1, 0, 1], dtype=np.int64)
+    I = sp.identity(idim, dtype=np.int64)
+    K = sp.kron(sp.kron(T, I), I) + sp.kron(sp.kron(I, T), I) + sp.kron(sp.kron(I, I), T)
+    h = 1.0 / (idim + 2)
+    K = sp.csr_matrix(K)
+    K.sum_duplicates()
+    K.eliminate_zeros()
+    K.sort_indices()
+    return sp.csr_matrix((K.data.astype(np.float64) / (h * h), K.indices, K.indptr), shape=K.shape)
+
+
+@pytest.mark.parametrize("idim", [2, 3, 5])
+def test_gen_pde3d_closed_form(orc, idim):
+    A = orc.gen_pde3d(idim).to_scipy()
+    K = poisson_kron(idim)
+    assert (A.indptr == K.indptr).all() and (A.indices == K.indices).all()
+    assert (A.data == K.data).all()
+
+
+def test_spmv_against_scipy_and_loop(orc):
+    a = orc.gen_pde3d(6)
+    A = a.to_scipy()
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(A.shape[0])
+    y = a.spmv(x)
+    assert np.allclose(y, A @ x, rtol=1e-14, atol=1e-10)
+    # stored-order sequential sum, row by row
+    ref = np.array([sum(A.data[k] * x[A.indices[k]] for k in range(A.indptr[i], A.indptr[i + 1]))
+                    for i in range(A.shape[0])])
+    assert (y == ref).all()
+    y0 = rng.standard_normal(A.shape[0])
+    assert np.allclose(a.spmv(x, alpha=-1.0, beta=1.0, y=y0.copy()), y0 - A @ x, rtol=1e-14, atol=1e-9)
+    assert np.allclose(a.spmv(x, trans=True), A.T @ x, rtol=1e-14, atol=1e-10)
+
+
+# ---------------------------------------------------------------- aggregation
+def py_dec_map_bld(A, theta):
+    """Second, independent restatement of the greedy decoupled aggregation
+    (mld_d_dec_map_bld.f90:148-278), plain Python on small matrices."""
+    n = A.shape[0]
+    d = A.diagonal()
+    rows = [(A.indices[A.indptr[i]:A.indptr[i + 1]], A.data[A.indptr[i]:A.indptr[i + 1]]) for i in range(n)]
+
+    def strong(i):
+        cols, vals = rows[i]
+        return [(j, v) for j, v in zip(cols, vals) if abs(v) > theta * np.sqrt(abs(d[i] * d[j]))]
+
+    agg = [None] * n
+    na = 0
+    for i in range(n):
+        if agg[i] is not None:
+            continue
+        nb = [j for j, _ in strong(i)]
+        if all(agg[j] is None for j in nb):
+            na += 1
+            for j in nb + [i]:
+                agg[j] = na
+    snap = list(agg)
+    for i in range(n):
+        if agg[i] is not None:
+            continue
+        best, pick = 0.0, None
+        for j, v in strong(i):
+            if snap[j] is not None and abs(v) > best:
+                best, pick = abs(v), j
+        if pick is not None:
+            agg[i] = agg[pick]
+    for i in range(n):
+        if agg[i] is not None:
+            continue
+        na += 1
+        agg[i] = na
+        for j, _ in strong(i):
+            if agg[j] is None:
+                agg[j] = na
+    return np.array(agg) - 1, na
+
+
+@pytest.mark.parametrize("idim,theta", [(2, 0.0), (3, 0.0), (4, 0.0), (5, 0.0), (4, 0.2)])
+def test_aggregation_two_restatements(orc, idim, theta):
+    a = orc.gen_pde3d(idim)
+    il, na = orc.dec_map_bld(a, theta=theta)
+    il2, na2 = py_dec_map_bld(a.to_scipy(), theta)
+    assert na == na2 and (il == il2).all()
+
+
+def test_aggregation_golden_matrix_and_invariants(orc):
+    A = orc.read_mtx(os.path.join(GOLD, "mld_mat.mtx"))
+    a = orc.Csr.from_scipy(A)
+    for theta in (0.0, 0.01, 0.3):
+        il, na = orc.dec_map_bld(a, theta=theta)
+        il2, na2 = py_dec_map_bld(A, theta)
+        assert na == na2 and (il == il2).all()
+        # a partition with contiguous ids 0..na-1
+        assert il.min() == 0 and il.max() == na - 1 and len(np.unique(il)) == na
+
+
+def test_aggregation_step1_plus_stencils(orc):
+    # theta=0 on the 7-point stencil: node 0 starts aggregate 0 = {0, 1, idim, idim^2}
+    idim = 6
+    a = orc.gen_pde3d(idim)
+    il, na = orc.dec_map_bld(a, theta=0.0)
+    assert set(np.flatnonzero(il == 0)) >= {0, 1, idim, idim * idim}
+
+
+# ------------------------------------------------------ smoothed prolongator
+@pytest.mark.parametrize("idim", [3, 5])
+def test_galerkin_and_prolongator(orc, idim):
+    a = orc.gen_pde3d(idim)
+    A = a.to_scipy().toarray()
+    p = orc.Prec(orc.config1_params(min_coarse_size=1, max_levs=2)).build(a)
+    assert p.nlev == 2
+    P = p.level_csr(1, "P").to_scipy().toarray()
+    R = p.level_csr(1, "R").to_scipy().toarray()
+    Ac = p.level_csr(1, "A").to_scipy().toarray()
+    assert (R == P.T).all()
+    ref = P.T @ A @ P
+    assert np.abs(Ac - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.allclose(Ac, Ac.T, rtol=0, atol=1e-12 * np.abs(Ac).max())
+    # omega = 4/(3*||D^-1 A||_inf) = 2/3 on the 7-point Poisson (interior row sum 2)
+    assert abs(p.omega(1) - 2.0 / 3.0) <= 4e-16
+    # P = (I - omega D^-1 A) Ptilde with Ptilde from the aggregation map
+    il = p.ilaggr(1)
+    Pt = np.zeros((A.shape[0], p.naggr(1)))
+    Pt[np.arange(A.shape[0]), il] = 1.0
+    D = np.diag(1.0 / np.diag(A))
+    P2 = (np.eye(A.shape[0]) - p.omega(1) * D @ A) @ Pt
+    assert np.abs(P - P2).max() <= 1e-14
+
+
+def test_default_coarse_size_truncates_cube_root(orc):
+    # mld_d_hierarchy_bld.f90:187-197: int((N)**(1/3))*40 with the
+    # floating-point cube root (64000**(1/3) = 39.99999999999999 -> 39)
+    for idim, expect in [(10, 9 * 40), (20, 19 * 40), (40, 39 * 40)]:
+        p = orc.Prec(orc.config1_params(min_coarse_size=-1)).build(orc.gen_pde3d(idim))
+        assert p.casize == max(int(float(idim ** 3) ** (1.0 / 3.0)), 1) * 40 == expect
+
+
+def test_hierarchy_stops_at_min_coarse_size(orc):
+    p = orc.Prec(orc.config1_params()).build(orc.gen_pde3d(40))
+    sizes = [p.level_csr(l, "A").shape[0] for l in range(p.nlev)]
+    assert sizes[-1] <= 200 and all(s > 200 for s in sizes[:-1])
+    assert sizes == [64000, 7809, 200]
+
+
+# ------------------------------------------------------------------ ILU(0)
+def ilu_check(orc, A):
+    a = orc.Csr.from_scipy(A)
+    L, U, d, rc = orc.ilu0(a)
+    assert rc == 0
+    n = A.shape[0]
+    Ld = L.to_scipy().toarray() + np.eye(n)
+    Ud = U.to_scipy().toarray() + np.eye(n)
+    M = Ld @ np.diag(1.0 / d) @ Ud
+    Ad = A.toarray()
+    mask = Ad != 0
+    return np.abs((M - Ad)[mask]).max() / np.abs(Ad).max(), L, U, d
+
+
+def test_ilu0_defining_property(orc):
+    for A in (orc.gen_pde3d(4).to_scipy(), orc.read_mtx(os.path.join(GOLD, "mld_mat.mtx"))[:400, :400].tocsr()):
+        err, *_ = ilu_check(orc, A)
+        assert err <= 1e-14
+
+
+def test_ilu0_exact_on_tridiagonal(orc):
+    n = 50
+    rng = np.random.default_rng(1)
+    A = sp.diags([-rng.random(n - 1), 4 + rng.random(n), -rng.random(n - 1)], [-1, 0, 1]).tocsr()
+    err, L, U, d = ilu_check(orc, A)
+    x = rng.standard_normal(n)
+    t = orc.trsv(L, x, lower=True)
+    y = orc.trsv(U, d * t, lower=False)
+    assert np.allclose(A @ y, x, rtol=1e-12, atol=1e-12)
+    # transposed solves give A^T y = x
+    t = orc.trsv(U, x, lower=False, trans=True)
+    y = orc.trsv(L, d * t, lower=True, trans=True)
+    assert np.allclose(A.T @ y, x, rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------------- smoothers
+def test_jacobi_closed_forms(orc):
+    a = orc.gen_pde3d(5)
+    A = a.to_scipy()
+    p = orc.Prec(orc.config1_params(min_coarse_size=1, max_levs=2)).build(a)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(A.shape[0])
+    d = A.diagonal()
+    assert (p.smooth(0, x, sweeps=1) == (1.0 / d) * x).all()
+    y1 = x / d
+    y2 = (x - (A @ y1 - d * y1)) / d
+    assert np.allclose(p.smooth(0, x, sweeps=2), y2, rtol=1e-14, atol=0)
+    y0 = rng.standard_normal(A.shape[0])
+    assert np.allclose(p.smooth(0, x, beta=1.0, y=y0, sweeps=2), y0 + y2, rtol=1e-14, atol=1e-16)
+    # many sweeps converge to A^-1 x (Jacobi converges on the Poisson matrix)
+    y = p.smooth(0, x, sweeps=600)
+    assert np.allclose(y, spla.spsolve(A.tocsc(), x), rtol=1e-8)
+
+
+def test_l1_jacobi_converges_and_diagonal(orc):
+    a = orc.gen_pde3d(5)
+    A = a.to_scipy()
+    p = orc.Prec(orc.config1_params(smoother=orc.SM_L1JAC, min_coarse_size=1, max_levs=2)).build(a)
+    d_l1 = A.diagonal() + (abs(A).sum(axis=1).A1 - abs(A.diagonal()))
+    assert np.allclose(p.dv(0), 1.0 / d_l1, rtol=1e-15)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(A.shape[0])
+    # l1-Jacobi: y_{k+1} = y_k + D_l1^-1 (x - A y_k), from y_0 = 0
+    yk = np.zeros_like(x)
+    for _ in range(3):
+        yk = yk + (x - A @ yk) / d_l1
+    assert np.allclose(p.smooth(0, x, sweeps=3), yk, rtol=1e-12)
+    y = p.smooth(0, x, sweeps=1500)
+    assert np.allclose(y, spla.spsolve(A.tocsc(), x), rtol=1e-7)
+
+
+# --------------------------------------------------------------- V-cycle
+def test_two_level_exact_coarse_identity(orc):
+    a = orc.gen_pde3d(4)
+    A = a.to_scipy().toarray()
+    prm = orc.config1_params(min_coarse_size=1, max_levs=2, sweeps_pre=0, sweeps_post=0,
+                             c_smoother=orc.SM_DENSELU, c_sweeps=1)
+    p = orc.Prec(prm).build(a)
+    P = p.level_csr(1, "P").to_scipy().toarray()
+    M = P @ np.linalg.solve(P.T @ A @ P, P.T)
+    x = np.random.default_rng(4).standard_normal(A.shape[0])
+    assert np.allclose(p.apply(x), M @ x, rtol=1e-11, atol=1e-14)
+
+
+def test_two_level_error_propagation(orc):
+    # V(1,1) with Jacobi and an exact coarse solve:
+    # I - M A = (I - D^-1 A)(I - P Ac^-1 P^T A)(I - D^-1 A)
+    a = orc.gen_pde3d(4)
+    A = a.to_scipy().toarray()
+    prm = orc.config1_params(min_coarse_size=1, max_levs=2, c_smoother=orc.SM_DENSELU, c_sweeps=1)
+    p = orc.Prec(prm).build(a)
+    P = p.level_csr(1, "P").to_scipy().toarray()
+    n = A.shape[0]
+    S = np.eye(n) - np.diag(1.0 / np.diag(A)) @ A
+    C = np.eye(n) - P @ np.linalg.solve(P.T @ A @ P, P.T @ A)
+    E = S @ C @ S
+    M = (np.eye(n) - E) @ np.linalg.inv(A)
+    x = np.random.default_rng(5).standard_normal(n)
+    assert np.allclose(p.apply(x), M @ x, rtol=1e-9, atol=1e-14 * np.abs(M @ x).max())
+
+
+@pytest.mark.parametrize("sm", ["JAC", "L1JAC"])
+def test_vcycle_linear_symmetric_transpose(orc, sm):
+    a = orc.gen_pde3d(12)
+    code = {"JAC": orc.SM_JAC, "L1JAC": orc.SM_L1JAC}[sm]
+    p = orc.Prec(orc.config1_params(smoother=code, sweeps_pre=2, sweeps_post=2)).build(a)
+    rng = np.random.default_rng(6)
+    u, v = rng.standard_normal((2, a.shape[0]))
+    Mu, Mv = p.apply(u), p.apply(v)
+    assert np.allclose(p.apply(2.0 * u - 3.0 * v), 2.0 * Mu - 3.0 * Mv, rtol=1e-12, atol=1e-12 * np.abs(Mu).max())
+    assert abs(u @ Mv - v @ Mu) <= 1e-12 * abs(u @ Mv)
+    assert np.allclose(p.apply(u, trans="T"), Mu, rtol=1e-11, atol=1e-12 * np.abs(Mu).max())
+
+
+# ----------------------------------------------------------------- Krylov
+@pytest.mark.parametrize("method", ["CG", "FCG", "BICGSTAB"])
+def test_krylov_small_exact(orc, method):
+    a = orc.gen_pde3d(3)
+    A = a.to_scipy()
+    p = orc.Prec(orc.config1_params(min_coarse_size=5)).build(a)
+    b = np.random.default_rng(7).standard_normal(A.shape[0])
+    x, it, err = p.krylov(method, b, eps=1e-13, itmax=A.shape[0] + 5)
+    assert err <= 1e-13 and it <= A.shape[0]
+    assert np.allclose(x, np.linalg.solve(A.toarray(), b), rtol=1e-10)
+
+
+@pytest.mark.parametrize("method", ["CG", "FCG", "BICGSTAB"])
+def test_golden_solution(orc, method):
+    """tests/fileread/runs/mld_{mat,rhs,sol}.mtx of the reference: solving to
+    1e-12 reproduces the shipped solution to kappa*1e-10."""
+    A = orc.read_mtx(os.path.join(GOLD, "mld_mat.mtx"))
+    b = orc.read_mtx(os.path.join(GOLD, "mld_rhs.mtx"))
+    xr = orc.read_mtx(os.path.join(GOLD, "mld_sol.mtx"))
+    p = orc.Prec(orc.config1_params()).build(orc.Csr.from_scipy(A))
+    x, it, err = p.krylov(method, b, eps=1e-12, itmax=500)
+    assert err <= 1e-12
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 2e-7
