@@ -178,8 +178,14 @@ def main():
     log(f"hierarchy_build {t_h:.1f}s smoothers_build+upload {t_s:.1f}s nlev={p.nlev} "
         f"complexity={p.complexity():.3f}")
     levels = level_table(eng, p)
+    def fmt_str(l, what):
+        f = p.op_format(l, what)
+        return f["fmt"] + (f"x{f['lanes']}" if f["fmt"] == "CSR" else "")
+
     for l, L in enumerate(levels):
-        log(f"  level {l}: n={L['n']} nnz={L['nnz_a']} nnzP={L['nnz_p']} A:{p.op_format(l, 'A')}")
+        ops = " ".join(f"{w}:{fmt_str(l, w)}" for w in (["A", "ND"] if l == 0 else ["A", "ND", "P", "R"])
+                       if not (w == "ND" and L["nnz_nd"] == 0))
+        log(f"  level {l}: n={L['n']} nnz={L['nnz_a']} nnzP={L['nnz_p']} {ops}")
     b = torch.ones(n, dtype=torch.float64, device="cuda:0")
     log("FP64 CG solves")
     x, times, its, errs = solve_timed(eng, torch, None, p, "CG", b, 1e-8, args.steps, args.warmup)

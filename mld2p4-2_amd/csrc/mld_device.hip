@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -32,14 +33,17 @@ struct DevError : std::runtime_error {
                      std::to_string(__LINE__));                                                     \
   } while (0)
 
-enum { FMT_CSR = 0, FMT_ELL = 1 };
+enum { FMT_CSR = 0, FMT_ELL = 1, FMT_SELL = 2 };
 
 template <typename V>
 struct DMat {
   int nr = 0, nc = 0;
   int64_t nnz = 0;
   int fmt = FMT_CSR, lanes = 1, w = 0, ld = 0;
-  int* irp = nullptr;
+  int iters = 0;        // CSR: entries per lane loaded up front (0, 2, 4, 8)
+  int* irp = nullptr;   // CSR row pointers
+  int* sptr = nullptr;  // SELL slice offsets (nslices + 1)
+  int* perm = nullptr;  // SELL position -> row
   int* ja = nullptr;
   V* val = nullptr;
 };
@@ -101,9 +105,11 @@ struct DevArena {
 static inline int nblk(int64_t n) { return (int)((n + BS - 1) / BS); }
 
 // ---------------------------------------------------------------------------
-// Storage choice.  ELL when the widest row is short and padding is small
-// (the 7-point fine level: 0.3% padding), else CSR with a power-of-two
-// number of lanes per row matched to the mean row length.
+// Storage choice (force: 0 automatic, 1 CSR, 2 ELL, 3 SELL).  Automatic:
+// ELL when the widest row is short and padding small (the 7-point fine
+// level: 0.3 % padding); SELL for other matrices with enough rows to fill
+// slices; CSR with a power-of-two number of lanes per row (matched to the
+// mean row length) for small matrices.
 // ---------------------------------------------------------------------------
 struct FmtChoice { int fmt, lanes, w; };
 
@@ -113,15 +119,23 @@ static FmtChoice choose_format(const HCsr& h, int force) {
   const double nnz = (double)h.nnz();
   const double avg = h.nr ? nnz / h.nr : 0.0;
   FmtChoice c{FMT_CSR, 1, maxrow};
-  bool ell_ok = maxrow > 0 && maxrow <= 16 && (double)h.nr * maxrow <= 1.10 * nnz;
-  if (force == 2 && maxrow > 0 && maxrow <= 64) ell_ok = true;
-  if (force == 1) ell_ok = false;
-  if (ell_ok) { c.fmt = FMT_ELL; return c; }
   int lanes = 1;
   while (lanes < 64 && lanes * 4 <= avg) lanes *= 2;
   c.lanes = lanes;
+  if (force == 1 || maxrow == 0) return c;
+  if (force == 2 && maxrow <= 64) { c.fmt = FMT_ELL; return c; }
+  if (force >= 3 && force <= 5) { c.fmt = FMT_SELL; return c; }
+  if (maxrow <= 16 && (double)h.nr * maxrow <= 1.10 * nnz) { c.fmt = FMT_ELL; return c; }
+  if (h.nr >= 4096) { c.fmt = FMT_SELL; return c; }
   return c;
 }
+
+// Rows are sorted by decreasing length inside windows of sigma rows (stable,
+// so equal-length rows keep their order); slices of 64 positions.  Large
+// windows cut padding; small windows keep consecutive rows (and so the
+// columns they gather) together in a wavefront.  force 3: sigma 4096,
+// force 4: sigma 256, force 5: no sorting.
+static int sell_sigma(int force) { return force == 4 ? 256 : (force == 5 ? 1 : 4096); }
 
 template <typename V>
 static DMat<V> upload_mat(const HCsr& h, DevArena& ar, hipStream_t s, int force, int lanes_override = 0) {
@@ -130,6 +144,10 @@ static DMat<V> upload_mat(const HCsr& h, DevArena& ar, hipStream_t s, int force,
   FmtChoice c = choose_format(h, force);
   if (lanes_override > 0) { c.fmt = FMT_CSR; c.lanes = lanes_override; }
   m.fmt = c.fmt; m.lanes = c.lanes; m.w = c.w;
+  {
+    const int per_lane = (c.w + c.lanes - 1) / std::max(1, c.lanes);
+    m.iters = per_lane <= 2 ? 2 : (per_lane <= 4 ? 4 : 8);
+  }
   if (c.fmt == FMT_ELL) {
     m.ld = h.nr;
     std::vector<int> ja((size_t)m.ld * m.w);
@@ -140,11 +158,58 @@ static DMat<V> upload_mat(const HCsr& h, DevArena& ar, hipStream_t s, int force,
         ja[(size_t)k * m.ld + i] = h.ja[q];
         val[(size_t)k * m.ld + i] = (V)h.val[q];
       }
-      for (; k < m.w; ++k) {  // padding: zero value on the row's own column
+      for (; k < m.w; ++k) {  // padding: zero value on a valid column
         ja[(size_t)k * m.ld + i] = std::min(i, h.nc - 1);
         val[(size_t)k * m.ld + i] = (V)0;
       }
     }
+    m.ja = ar.upload(ja.data(), ja.size(), s);
+    m.val = ar.upload(val.data(), val.size(), s);
+    HIPCHECK(hipStreamSynchronize(s));
+  } else if (c.fmt == FMT_SELL) {
+    const int nr = h.nr;
+    const int nsl = (nr + SELL_C - 1) / SELL_C;
+    std::vector<int> perm(nr);
+    for (int i = 0; i < nr; ++i) perm[i] = i;
+    auto len = [&](int i) { return (int)(h.irp[i + 1] - h.irp[i]); };
+    const int sigma = sell_sigma(force);
+    for (int w0 = 0; w0 < nr && sigma > 1; w0 += sigma) {
+      const int w1 = std::min(nr, w0 + sigma);
+      std::stable_sort(perm.begin() + w0, perm.begin() + w1, [&](int a, int b) { return len(a) > len(b); });
+    }
+    std::vector<int> sptr(nsl + 1, 0);
+    for (int sl = 0; sl < nsl; ++sl) {
+      int wmax = 0;
+      for (int l = 0; l < SELL_C; ++l) {
+        const int p = sl * SELL_C + l;
+        if (p < nr) wmax = std::max(wmax, len(perm[p]));
+      }
+      const int64_t next = (int64_t)sptr[sl] + (int64_t)wmax * SELL_C;
+      if (next >= ((int64_t)1 << 31)) throw DevError("SELL storage exceeds 2^31 entries");
+      sptr[sl + 1] = (int)next;
+    }
+    const size_t tot = (size_t)sptr[nsl];
+    std::vector<int> ja(tot);
+    std::vector<V> val(tot);
+    for (int sl = 0; sl < nsl; ++sl) {
+      const int w = (sptr[sl + 1] - sptr[sl]) / SELL_C;
+      for (int l = 0; l < SELL_C; ++l) {
+        const int p = sl * SELL_C + l;
+        const int row = (p < nr) ? perm[p] : -1;
+        int k = 0;
+        if (row >= 0)
+          for (int64_t q = h.irp[row]; q < h.irp[row + 1]; ++q, ++k) {
+            ja[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = h.ja[q];
+            val[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = (V)h.val[q];
+          }
+        for (; k < w; ++k) {  // padding: zero value on a valid column
+          ja[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = (row >= 0) ? std::min(row, h.nc - 1) : 0;
+          val[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = (V)0;
+        }
+      }
+    }
+    m.sptr = ar.upload(sptr.data(), sptr.size(), s);
+    m.perm = ar.upload(perm.data(), perm.size(), s);
     m.ja = ar.upload(ja.data(), ja.size(), s);
     m.val = ar.upload(val.data(), val.size(), s);
     HIPCHECK(hipStreamSynchronize(s));
@@ -177,61 +242,120 @@ static HCsr host_transpose(const HCsr& a) {
   return t;
 }
 
+// Blocks of the grid of a sparse launch (row- or lane-per-thread).
+template <typename V>
+static int spmv_blocks(const DMat<V>& m) {
+  if (m.fmt == FMT_CSR) return nblk((int64_t)m.nr * m.lanes);
+  return nblk(m.nr);
+}
+
+template <int LANES, int K, typename V, class G, class E>
+static void csr_launch_l(const DMat<V>& m, int nb, G g, E e, hipStream_t s, Red red) {
+  switch (m.iters) {
+    case 2: csr_kernel<V, LANES, 2, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
+    case 4: csr_kernel<V, LANES, 4, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
+    case 8: csr_kernel<V, LANES, 8, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
+    default: csr_kernel<V, LANES, 0, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Generic sparse launch: dispatch on format / width / lanes.
+// Generic sparse launch: dispatch on format / width / lanes.  K > 0: the
+// epilogue's dot contributions are reduced into red.result[0..K).
 // ---------------------------------------------------------------------------
-template <typename V, class G, class E>
-static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s) {
+template <int K = 0, typename V, class G, class E>
+static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red = Red{nullptr, nullptr}) {
   if (m.nr == 0) return;
+  const int nb = spmv_blocks(m);
   if (m.fmt == FMT_ELL) {
-    const int nb = nblk(m.nr);
     switch (m.w) {
-      case 4: ell_kernel<V, 4><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e); break;
-      case 5: ell_kernel<V, 5><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e); break;
-      case 6: ell_kernel<V, 6><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e); break;
-      case 7: ell_kernel<V, 7><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e); break;
-      default: ell_kernel<V, 0><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e); break;
+      case 4: ell_kernel<V, 4, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
+      case 5: ell_kernel<V, 5, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
+      case 6: ell_kernel<V, 6, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
+      case 7: ell_kernel<V, 7, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
+      default: ell_kernel<V, 0, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
     }
+  } else if (m.fmt == FMT_SELL) {
+    sell_kernel<V, K><<<nb, BS, 0, s>>>(m.nr, m.sptr, m.perm, m.ja, m.val, g, e, red);
   } else {
-    const int nb = nblk((int64_t)m.nr * m.lanes);
     switch (m.lanes) {
-      case 1: csr_kernel<V, 1><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
-      case 2: csr_kernel<V, 2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
-      case 4: csr_kernel<V, 4><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
-      case 8: csr_kernel<V, 8><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
-      case 16: csr_kernel<V, 16><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
-      case 32: csr_kernel<V, 32><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
-      default: csr_kernel<V, 64><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e); break;
+      case 1: csr_launch_l<1, K>(m, nb, g, e, s, red); break;
+      case 2: csr_launch_l<2, K>(m, nb, g, e, s, red); break;
+      case 4: csr_launch_l<4, K>(m, nb, g, e, s, red); break;
+      case 8: csr_launch_l<8, K>(m, nb, g, e, s, red); break;
+      case 16: csr_launch_l<16, K>(m, nb, g, e, s, red); break;
+      case 32: csr_launch_l<32, K>(m, nb, g, e, s, red); break;
+      default: csr_launch_l<64, K>(m, nb, g, e, s, red); break;
     }
   }
+  if constexpr (K > 0) finalize_kernel<K><<<1, BS, 0, s>>>(nb, red.partial, red.result);
   HIPCHECK(hipGetLastError());
 }
 
-static int dot_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(nblk(n), 2048)); }
+// ---------------------------------------------------------------------------
+// Autotuner: storage of each large operator picked by timing y = M x in every
+// candidate layout (ELL when eligible, SELL, CSR with 1..16 lanes per row up
+// to the mean row length) on the device.  Layouts that keep each row's stored
+// order (ELL, SELL, one-lane CSR) win ties within 3 %.  Disabled by
+// MLD_AUTOTUNE=0 (then the static rule of choose_format applies).
+// ---------------------------------------------------------------------------
+constexpr int TUNE_MIN_ROWS = 32768;
 
-// y = A x and the dot(s) u.y (and w.y) in one pass (Krylov SpMV).
-template <typename V, bool W2>
-static void spmv_dot_launch(const DMat<V>& m, const V* x, V* y, const V* u, const V* w2, double* partial,
-                            unsigned* counter, double* result, hipStream_t s) {
-  const int nb = dot_grid((int64_t)m.nr * (m.fmt == FMT_ELL ? 1 : m.lanes));
-  if (m.fmt == FMT_ELL) {
-    switch (m.w) {
-      case 7: ell_dot_kernel<V, 7, W2><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
-      case 6: ell_dot_kernel<V, 6, W2><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
-      default: ell_dot_kernel<V, 0, W2><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
+static bool autotune_enabled() {
+  const char* e = std::getenv("MLD_AUTOTUNE");
+  return !(e && e[0] == '0');
+}
+
+template <typename V>
+static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int force) {
+  if (force != 0 || h.nr < TUNE_MIN_ROWS || !autotune_enabled()) return upload_mat<V>(h, ar, s, force);
+  int maxrow = 0;
+  for (int i = 0; i < h.nr; ++i) maxrow = std::max(maxrow, (int)(h.irp[i + 1] - h.irp[i]));
+  const double avg = (double)h.nnz() / std::max(1, h.nr);
+  // an almost padding-free ELL is not worth a search
+  if (maxrow <= 16 && (double)h.nr * maxrow <= 1.02 * (double)h.nnz()) return upload_mat<V>(h, ar, s, 2);
+  struct Cand { int force, lanes; bool ordered; };
+  std::vector<Cand> cands;
+  if (maxrow <= 64 && (double)h.nr * maxrow <= 1.25 * (double)h.nnz()) cands.push_back({2, 0, true});
+  cands.push_back({3, 0, true});
+  cands.push_back({4, 0, true});
+  cands.push_back({5, 0, true});
+  for (int l = 1; l <= 64; l *= 2)
+    if (l == 1 || l <= avg) cands.push_back({1, l, l == 1});
+  V* x = nullptr;
+  V* y = nullptr;
+  HIPCHECK(hipMalloc(&x, sizeof(V) * (size_t)std::max(1, h.nc)));
+  HIPCHECK(hipMalloc(&y, sizeof(V) * (size_t)std::max(1, h.nr)));
+  fill_kernel<V><<<nblk(h.nc), BS, 0, s>>>(h.nc, x, (V)1);
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  double best = 1e300;
+  int bi = 0;
+  std::vector<double> tm(cands.size());
+  for (size_t c = 0; c < cands.size(); ++c) {
+    DevArena tmp;
+    DMat<V> m = upload_mat<V>(h, tmp, s, cands[c].force, cands[c].lanes);
+    spmv_launch(m, GatherX<V, V>{x}, EpiSet<V, V>{y}, s);  // warm-up
+    float msmin = 1e30f;
+    const int reps = 5;
+    for (int trial = 0; trial < 3; ++trial) {  // best of three batches
+      HIPCHECK(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; ++r) spmv_launch(m, GatherX<V, V>{x}, EpiSet<V, V>{y}, s);
+      HIPCHECK(hipEventRecord(e1, s));
+      HIPCHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+      msmin = std::min(msmin, ms);
     }
-  } else {
-    switch (m.lanes) {
-      case 1: csr_dot_kernel<V, 1, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
-      case 2: csr_dot_kernel<V, 2, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
-      case 4: csr_dot_kernel<V, 4, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
-      case 8: csr_dot_kernel<V, 8, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
-      case 16: csr_dot_kernel<V, 16, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
-      case 32: csr_dot_kernel<V, 32, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
-      default: csr_dot_kernel<V, 64, W2><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, x, y, u, w2, partial, counter, result); break;
-    }
+    tm[c] = msmin / reps * (cands[c].ordered ? 1.0 : 1.03);
+    if (tm[c] < best) { best = tm[c]; bi = (int)c; }
   }
-  HIPCHECK(hipGetLastError());
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(x);
+  (void)hipFree(y);
+  return upload_mat<V>(h, ar, s, cands[bi].force, cands[bi].lanes);
 }
 
 // ---------------------------------------------------------------------------
@@ -251,10 +375,10 @@ class DevPrecT : public DevPrec {
       const Level& H = p.precv[l];
       DLevel<V>& D = L_[l];
       D.n = H.a.nr;
-      D.a = upload_mat<V>(H.a, ar_, s_, p.dev_format);
+      D.a = upload_tuned<V>(H.a, ar_, s_, p.dev_format);
       if (l > 0) {
-        D.p = upload_mat<V>(H.p, ar_, s_, p.dev_format);
-        D.r = upload_mat<V>(H.r, ar_, s_, p.dev_format);
+        D.p = upload_tuned<V>(H.p, ar_, s_, p.dev_format);
+        D.r = upload_tuned<V>(H.r, ar_, s_, p.dev_format);
       }
       D.sweeps_pre = H.parms.sweeps_pre;
       D.sweeps_post = H.parms.sweeps_post;
@@ -275,10 +399,8 @@ class DevPrecT : public DevPrec {
       akry_ = upload_mat<double>(p.precv[0].a, ar_, s_, p.dev_format);
     }
     n0_ = L_[0].n;
-    partial_ = ar_.alloc<double>(4 * 2048);
-    counter_ = ar_.alloc<unsigned>(4);
+    partial_ = ar_.alloc<double>(2 * (size_t)std::max(spmv_blocks(akry_), red_blocks(n0_)) + 64);
     scal_ = ar_.alloc<double>(S_NSCAL);
-    HIPCHECK(hipMemsetAsync(counter_, 0, 4 * sizeof(unsigned), s_));
     HIPCHECK(hipMemsetAsync(scal_, 0, S_NSCAL * sizeof(double), s_));
     HIPCHECK(hipHostMalloc((void**)&hscal_, S_NSCAL * sizeof(double), hipHostMallocDefault));
     HIPCHECK(hipStreamSynchronize(s_));
@@ -408,7 +530,7 @@ class DevPrecT : public DevPrec {
         }
         case 4: return smooth(D, D.has_sm2 ? D.sm2 : D.sm, D.ty, 1, D.y2l, D.sweeps_post, 'N');
         case 5:
-          spmv_dot_launch<double, false>(akry_, kx, kq, kx, nullptr, partial_, counter_, scal_ + S_TMP0, s_);
+          spmv_launch<1>(akry_, GatherX<double, double>{kx}, EpiSetDot<double>{kq, kx, nullptr}, s_, red(S_TMP0));
           return 0;
       }
       return E_BAD_ARG;
@@ -441,7 +563,14 @@ class DevPrecT : public DevPrec {
   int n0_ = 0;
   DMat<double> akry_;
   double* partial_ = nullptr;
-  unsigned* counter_ = nullptr;
+  Red red(int slot) { return Red{partial_, scal_ + slot}; }
+  // one-block sum of the partials of a streaming reduction over n0_ entries
+  void finalize(int K, int slot) {
+    const int nb = red_blocks(n0_);
+    if (K == 1) finalize_kernel<1><<<1, BS, 0, s_>>>(nb, partial_, scal_ + slot);
+    else finalize_kernel<2><<<1, BS, 0, s_>>>(nb, partial_, scal_ + slot);
+    HIPCHECK(hipGetLastError());
+  }
   double* scal_ = nullptr;
   double* hscal_ = nullptr;
   const Prec* hprec_ = nullptr;
@@ -476,7 +605,7 @@ class DevPrecT : public DevPrec {
       S.dv = ar_.upload(dv.data(), dv.size(), s_);
       HIPCHECK(hipStreamSynchronize(s_));
     }
-    if (H.nnz_nd > 0) S.nd = upload_mat<V>(H.nd, ar_, s_, force);
+    if (H.nnz_nd > 0) S.nd = upload_tuned<V>(H.nd, ar_, s_, force);
     if (H.solver == ILU_N) {
       S.l = upload_mat<V>(H.l, ar_, s_, 1, 1);  // CSR, one lane per row (stored order)
       S.u = upload_mat<V>(H.u, ar_, s_, 1, 1);
@@ -555,12 +684,14 @@ class DevPrecT : public DevPrec {
         V* t1 = D.tx;
         V* t2 = beta ? D.t2 : y;
         if (n <= 8192) {
+          // one workgroup; a single wavefront when the matrix is tiny
+          const int nt = std::min(1024, ((n + 63) / 64) * 64);
           if (beta)
-            ilu_small_kernel<V, 1><<<1, 1024, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d, r1,
-                                                       nl1, p2d, r2, nl2, x, t1, t2, y);
+            ilu_small_kernel<V, 1><<<1, nt, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d, r1,
+                                                     nl1, p2d, r2, nl2, x, t1, t2, y);
           else
-            ilu_small_kernel<V, 0><<<1, 1024, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d, r1,
-                                                       nl1, p2d, r2, nl2, x, t1, t2, y);
+            ilu_small_kernel<V, 0><<<1, nt, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d, r1,
+                                                     nl1, p2d, r2, nl2, x, t1, t2, y);
           HIPCHECK(hipGetLastError());
         } else {
           for (int lv = 0; lv < nl1; ++lv) {
@@ -594,12 +725,21 @@ class DevPrecT : public DevPrec {
   }
 
   // One Jacobi sweep y_out = D^-1 (b - ND y_in) (FIRST: y_in = D^-1 b).
+  // FIRST on an ELL operator gathers d_j*x_j directly; on SELL/CSR, whose
+  // gathers are the costly part, d.*x is formed first in one streaming pass
+  // (same products, so the same result) and gathered once.
   template <bool FIRST, bool ADD>
-  void jac_sweep(DSmoother<V>& S, const V* b, const V* yin, V* yout, char trans) {
+  void jac_sweep(DLevel<V>& D, DSmoother<V>& S, const V* b, const V* yin, V* yout, char trans) {
     const DMat<V>& nd = (trans != 'N') ? S.ndt : S.nd;
     EpiJac<V, V, V, ADD> e{S.dv, b, yout};
-    if (FIRST) spmv_launch(nd, GatherDX<V, V>{S.dv, b}, e, s_);
-    else spmv_launch(nd, GatherX<V, V>{yin}, e, s_);
+    if (FIRST && nd.fmt == FMT_ELL) {
+      spmv_launch(nd, GatherDX<V, V>{S.dv, b}, e, s_);
+    } else if (FIRST) {
+      vmul_kernel<V, V, false><<<nblk(D.n), BS, 0, s_>>>(D.n, S.dv, b, D.t2);
+      spmv_launch(nd, GatherX<V, V>{D.t2}, e, s_);
+    } else {
+      spmv_launch(nd, GatherX<V, V>{yin}, e, s_);
+    }
   }
 
   // mld_d_jac_smoother_apply_vect (init 'Z'), alpha = 1, beta in {0,1}.
@@ -614,19 +754,19 @@ class DevPrecT : public DevPrec {
     if (sweeps <= 1 || S.nnz_nd == 0) return solver_apply(D, S, x, beta, y, trans);
     if (S.solver != DIAG_SCALE && S.solver != L1_DIAG) return E_UNSUPPORTED;
     if (sweeps == 2) {
-      if (beta) jac_sweep<true, true>(S, x, nullptr, y, trans);
-      else jac_sweep<true, false>(S, x, nullptr, y, trans);
+      if (beta) jac_sweep<true, true>(D, S, x, nullptr, y, trans);
+      else jac_sweep<true, false>(D, S, x, nullptr, y, trans);
       return 0;
     }
     V* ta = D.tx;
     V* tb = D.t1;
-    jac_sweep<true, false>(S, x, nullptr, ta, trans);
+    jac_sweep<true, false>(D, S, x, nullptr, ta, trans);
     for (int i = 2; i <= sweeps - 2; ++i) {
-      jac_sweep<false, false>(S, x, ta, tb, trans);
+      jac_sweep<false, false>(D, S, x, ta, tb, trans);
       std::swap(ta, tb);
     }
-    if (beta) jac_sweep<false, true>(S, x, ta, y, trans);
-    else jac_sweep<false, false>(S, x, ta, y, trans);
+    if (beta) jac_sweep<false, true>(D, S, x, ta, y, trans);
+    else jac_sweep<false, false>(D, S, x, ta, y, trans);
     return 0;
   }
 
@@ -713,7 +853,8 @@ class DevPrecT : public DevPrec {
     return hscal_[slot];
   }
   void dot1(const double* a, const double* b, int slot) {
-    dot_kernel<double, 1><<<dot_grid(n0_), BS, 0, s_>>>(n0_, a, b, nullptr, nullptr, partial_, counter_, scal_ + slot);
+    dot_kernel<double, 1><<<red_blocks(n0_), BS, 0, s_>>>(n0_, a, b, nullptr, nullptr, red(slot));
+    finalize(1, slot);
     HIPCHECK(hipGetLastError());
   }
   void axpby(double a, const double* x, double b, double* y) {
@@ -749,7 +890,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
   *err = std::sqrt(hscal_[S_RR]) / bn2;
   int it = 0;
   if (*err <= eps) { *iter = 0; return 0; }
-  const int nbd = dot_grid(n);
+  const int nbd = red_blocks(n);
   if (method == 0) {  // CG
     while (it < itmax) {
       ++it;
@@ -757,9 +898,9 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       if (rc) return rc;
       dot1(r, z, (it & 1) ? S_RHO1 : S_RHO0);
       cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, z, p, scal_, it);
-      spmv_dot_launch<double, false>(akry_, p, q, p, nullptr, partial_, counter_, scal_ + S_SIGMA, s_);
-      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, (it & 1) ? S_RHO1 : S_RHO0, S_SIGMA, partial_, counter_,
-                                   scal_ + S_RR);
+      spmv_launch<1>(akry_, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, s_, red(S_SIGMA));
+      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, (it & 1) ? S_RHO1 : S_RHO0, S_SIGMA, red(S_RR));
+      finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
       read_scal(S_RR);
       *err = std::sqrt(hscal_[S_RR]) / bn2;
@@ -773,9 +914,11 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       int rc = apply(r, z, 'N');
       if (rc) return rc;
       if (it > 1) dot1(z, q, S_GAMMA);
-      fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z, p, r, scal_, it, partial_, counter_, scal_ + S_PR);
-      spmv_dot_launch<double, false>(akry_, p, q, p, nullptr, partial_, counter_, scal_ + S_DELTA, s_);
-      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, S_PR, S_DELTA, partial_, counter_, scal_ + S_RR);
+      fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z, p, r, scal_, it, red(S_PR));
+      finalize(1, S_PR);
+      spmv_launch<1>(akry_, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, s_, red(S_DELTA));
+      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, S_PR, S_DELTA, red(S_RR));
+      finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
       read_scal(S_RR);
       *err = std::sqrt(hscal_[S_RR]) / bn2;
@@ -805,7 +948,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       }
       int rc = apply(p, ph, 'N');
       if (rc) return rc;
-      spmv_dot_launch<double, false>(akry_, ph, v, rt, nullptr, partial_, counter_, scal_ + S_TMP1, s_);
+      spmv_launch<1>(akry_, GatherX<double, double>{ph}, EpiSetDot<double>{v, rt, nullptr}, s_, red(S_TMP1));
       const double rtv = read_scal(S_TMP1);
       if (rtv == 0.0) break;
       alpha = rho / rtv;
@@ -821,7 +964,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       }
       rc = apply(s, sh, 'N');
       if (rc) return rc;
-      spmv_dot_launch<double, true>(akry_, sh, t, nullptr, s, partial_, counter_, scal_ + S_TMP0, s_);  // t.t, s.t
+      spmv_launch<2>(akry_, GatherX<double, double>{sh}, EpiSetDot<double>{t, nullptr, s}, s_, red(S_TMP0));  // t.t, s.t
       read_scal(S_TMP0);
       const double tt = hscal_[S_TMP0], ts = hscal_[S_TMP1];
       omega = (tt == 0.0) ? 0.0 : ts / tt;

@@ -2,17 +2,19 @@
 //
 // Every sparse product is "one row -> one value" followed by an epilogue
 // that consumes the row value in registers (residual, Jacobi update,
-// prolongation add, Krylov dot).  Two storage formats:
+// prolongation add, Krylov dot).  Three storage formats:
 //   ELL  : column-major (entry k of row i at k*ld+i), one lane per row; each
-//          load instruction of a wavefront reads 64 consecutive values, and
-//          the row sum follows the stored order exactly.
+//          load instruction of a wavefront reads 64 consecutive values.
+//   SELL : sliced ELL, slices of 64 rows (one wavefront) stored column-major,
+//          rows length-sorted inside windows so that a slice is as wide as
+//          its longest row; one lane per row.
 //   CSR  : LANES lanes per row (1..64) reading the row in coalesced strides,
-//          reduced with cross-lane shuffles (DPP/permute); LANES == 1 keeps
-//          the stored order.
-// Reductions are deterministic: fixed per-block partials, then the last
-// block to finish (device-scope atomic ticket) adds them in index order.
-// Built with -ffp-contract=off: products and sums are rounded separately, as
-// in the CPU restatement the results are checked against.
+//          reduced with cross-lane shuffles.
+// ELL, SELL and one-lane CSR sum each row in its stored order, exactly as
+// the sequential CPU loop does.  Epilogues may also return up to two dot
+// contributions; the kernel writes per-block partial sums and a one-block
+// kernel adds them in block order (deterministic, no atomics).
+// Built with -ffp-contract=off: products and sums are rounded separately.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -22,6 +24,15 @@ namespace mld {
 namespace k {
 
 constexpr int BS = 256;  // threads per block (4 wavefronts)
+constexpr int SELL_C = 64;
+
+// Matrix values and column indices are read exactly once per product:
+// non-temporal loads keep them from evicting the gathered vectors from the
+// caches.
+template <typename T>
+__device__ __forceinline__ T ldnt(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
 
 // ---------------------------------------------------------------- gathers
 template <typename V, typename XT>
@@ -38,31 +49,45 @@ struct GatherDX {
 };
 
 // -------------------------------------------------------------- epilogues
+// operator()(i, acc, dots): consume the row value acc of row i.
 template <typename V, typename YT>
 struct EpiSet {  // y = A x
   YT* __restrict__ y;
-  __device__ __forceinline__ void operator()(int i, V acc) const { y[i] = (YT)acc; }
+  __device__ __forceinline__ void operator()(int i, V acc, double*) const { y[i] = (YT)acc; }
 };
 template <typename V>
 struct EpiAdd {  // y = y + A x      (prolongation, psb_map_Y2X with beta = 1)
   V* __restrict__ y;
-  __device__ __forceinline__ void operator()(int i, V acc) const { y[i] = y[i] + acc; }
+  __device__ __forceinline__ void operator()(int i, V acc, double*) const { y[i] = y[i] + acc; }
 };
 template <typename V, typename BT>
 struct EpiRes {  // r = b - A x     (psb_geaxpby + psb_spmm(-1,A,x,1,r))
   const BT* __restrict__ b;
   V* __restrict__ r;
-  __device__ __forceinline__ void operator()(int i, V acc) const { r[i] = (V)b[i] - acc; }
+  __device__ __forceinline__ void operator()(int i, V acc, double*) const { r[i] = (V)b[i] - acc; }
 };
 template <typename V, typename BT, typename YT, bool ADD>
 struct EpiJac {  // t = d*(b - ND y);  y = t  or  y = y + t
   const V* __restrict__ d;
   const BT* __restrict__ b;
   YT* __restrict__ y;
-  __device__ __forceinline__ void operator()(int i, V acc) const {
+  __device__ __forceinline__ void operator()(int i, V acc, double*) const {
     V t = d[i] * ((V)b[i] - acc);
     if (ADD) y[i] = (YT)((V)y[i] + t);
     else y[i] = (YT)t;
+  }
+};
+// Krylov SpMV: y = A x and dots u.y (u == nullptr: y.y) and w.y
+template <typename V>
+struct EpiSetDot {
+  V* y;
+  const V* u;
+  const V* w;
+  __device__ __forceinline__ void operator()(int i, V acc, double* d) const {
+    y[i] = acc;
+    const double a = (double)acc;
+    d[0] = (u ? (double)u[i] : a) * a;
+    if (w) d[1] = (double)w[i] * a;
   }
 };
 
@@ -88,147 +113,152 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   return s;
 }
 
-// K partial sums per block -> partial[K*nblocks]; the last block adds them in
-// block order and writes result[0..K).  counter is returned to zero.
+// K partial sums per block -> partial[q*nblocks + block].  Every thread of
+// the block must call it.  The partials are added in block order by
+// finalize_kernel, launched right after on the same stream (no atomics).
 template <int K>
-__device__ __forceinline__ void grid_sum(const double (&v)[K], double* partial, unsigned* counter,
-                                         double* result) {
+__device__ __forceinline__ void block_partials(const double* v, double* partial, int slot) {
   __shared__ double lds[BS / 64];
-  __shared__ int is_last;
   const int nb = gridDim.x;
-  double bs[K];
 #pragma unroll
-  for (int q = 0; q < K; ++q) bs[q] = block_sum(v[q], lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < K; ++q) partial[(size_t)q * nb + blockIdx.x] = bs[q];
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    is_last = (t == (unsigned)nb - 1);
+  for (int q = 0; q < K; ++q) {
+    const double s = block_sum(v[q], lds);
+    if (threadIdx.x == 0) partial[(size_t)q * nb + slot] = s;
   }
-  __syncthreads();
-  if (is_last) {
-    __threadfence();
+}
+template <int K>
+__device__ __forceinline__ void block_partials(const double* v, double* partial) {
+  block_partials<K>(v, partial, blockIdx.x);
+}
+
+template <int K>
+__global__ __launch_bounds__(BS) void finalize_kernel(int nb, const double* __restrict__ partial,
+                                                       double* __restrict__ result) {
+  __shared__ double lds[BS / 64];
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-      double s = 0.0;
-      for (int i = threadIdx.x; i < nb; i += BS) s += __builtin_nontemporal_load(&partial[(size_t)q * nb + i]);
-      s = block_sum(s, lds);
-      if (threadIdx.x == 0) result[q] = s;
-    }
-    if (threadIdx.x == 0) *counter = 0u;
+  for (int q = 0; q < K; ++q) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += BS) s += partial[(size_t)q * nb + i];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) result[q] = s;
   }
 }
 
+struct Red {  // reduction buffers of a kernel with dot outputs
+  double* partial;
+  double* result;
+};
+
+// Block -> row-block order.  Workgroups are dispatched to the 8 XCDs
+// round-robin in blockIdx order.  Giving each XCD one contiguous run of row
+// blocks (x*q + min(x,r) + i for block 8i+x) was measured slower on the
+// fine-level stencil (residual 326 -> 358 us at 256^3) than the identity,
+// which spreads neighbouring row blocks over all XCDs; the identity is kept.
+__device__ __forceinline__ int xcd_block() { return blockIdx.x; }
+
 // ----------------------------------------------------------- SpMV kernels
-template <typename V, int LANES, class G, class E>
+// ITERS > 0: the first ITERS strided entries of each lane are loaded up front
+// (predicated), so that all index/value loads and then all gathers of a lane
+// are in flight together; rows longer than ITERS*LANES finish in a loop.
+// The lane's summation order is the same as the plain strided loop.
+template <typename V, int LANES, int ITERS, int K, class G, class E>
 __global__ __launch_bounds__(BS) void csr_kernel(int nr, const int* __restrict__ irp,
                                                   const int* __restrict__ ja, const V* __restrict__ val,
-                                                  G g, E e) {
-  const int gid = blockIdx.x * BS + threadIdx.x;
+                                                  G g, E e, Red red) {
+  const int lb = xcd_block();
+  const int gid = lb * BS + threadIdx.x;
   const int row = gid / LANES;
   const int lane = gid & (LANES - 1);
   V acc = 0;
   if (row < nr) {
     const int b = irp[row], en = irp[row + 1];
-    for (int k = b + lane; k < en; k += LANES) acc = acc + val[k] * g(ja[k]);
+    int k0 = b + lane;
+    if constexpr (ITERS > 0) {
+      int jj[ITERS];
+      V vv[ITERS];
+#pragma unroll
+      for (int it = 0; it < ITERS; ++it) {
+        const int k = k0 + it * LANES;
+        const bool in = k < en;
+        jj[it] = in ? ldnt(ja + k) : -1;
+        vv[it] = in ? ldnt(val + k) : (V)0;
+      }
+      V xx[ITERS];
+#pragma unroll
+      for (int it = 0; it < ITERS; ++it) xx[it] = (jj[it] >= 0) ? g(jj[it]) : (V)0;
+#pragma unroll
+      for (int it = 0; it < ITERS; ++it)
+        if (jj[it] >= 0) acc = acc + vv[it] * xx[it];
+      k0 += ITERS * LANES;
+    }
+    for (int k = k0; k < en; k += LANES) acc = acc + ldnt(val + k) * g(ldnt(ja + k));
   }
   if (LANES > 1) {
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, LANES);
   }
-  if (lane == 0 && row < nr) e(row, acc);
+  double d[2] = {0.0, 0.0};
+  if (lane == 0 && row < nr) e(row, acc, d);
+  if constexpr (K > 0) block_partials<K>(d, red.partial, lb);
 }
 
-template <typename V, int W, class G, class E>
+template <typename V, int W, int K, class G, class E>
 __global__ __launch_bounds__(BS) void ell_kernel(int nr, int w, int ld, const int* __restrict__ ja,
-                                                  const V* __restrict__ val, G g, E e) {
-  const int row = blockIdx.x * BS + threadIdx.x;
-  if (row >= nr) return;
-  V acc = 0;
-  if constexpr (W > 0) {
-#pragma unroll
-    for (int k = 0; k < W; ++k) {
-      const int idx = k * ld + row;
-      acc = acc + val[idx] * g(ja[idx]);
-    }
-  } else {
-    for (int k = 0; k < w; ++k) {
-      const int idx = k * ld + row;
-      acc = acc + val[idx] * g(ja[idx]);
-    }
-  }
-  e(row, acc);
-}
-
-// y = A x with the Krylov dot(s) folded in: D1 = sum_i u_i*y_i (and D2 =
-// sum_i w_i*y_i when W2) over rows, deterministic grid sum.
-template <typename V, int LANES, bool W2>
-__global__ __launch_bounds__(BS) void csr_dot_kernel(int nr, const int* __restrict__ irp,
-                                                      const int* __restrict__ ja, const V* __restrict__ val,
-                                                      const V* __restrict__ x, V* __restrict__ y,
-                                                      const V* __restrict__ u, const V* __restrict__ w2,
-                                                      double* partial, unsigned* counter, double* result) {
-  const int nrows_blk = BS / LANES;
-  double d1 = 0.0, d2 = 0.0;
-  for (int base = blockIdx.x * nrows_blk; base < nr; base += gridDim.x * nrows_blk) {
-    const int row = base + threadIdx.x / LANES;
-    const int lane = threadIdx.x & (LANES - 1);
-    V acc = 0;
-    if (row < nr) {
-      const int b = irp[row], en = irp[row + 1];
-      for (int k = b + lane; k < en; k += LANES) acc = acc + val[k] * x[ja[k]];
-    }
-    if (LANES > 1) {
-#pragma unroll
-      for (int off = LANES / 2; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, LANES);
-    }
-    if (lane == 0 && row < nr) {
-      y[row] = acc;
-      d1 += (u ? (double)u[row] : (double)acc) * (double)acc;  // u == nullptr: y.y
-      if (W2) d2 += (double)w2[row] * (double)acc;
-    }
-  }
-  double v[2] = {d1, d2};
-  if (W2) grid_sum<2>(v, partial, counter, result);
-  else {
-    double v1[1] = {d1};
-    grid_sum<1>(v1, partial, counter, result);
-  }
-}
-
-template <typename V, int W, bool W2>
-__global__ __launch_bounds__(BS) void ell_dot_kernel(int nr, int w, int ld, const int* __restrict__ ja,
-                                                      const V* __restrict__ val, const V* __restrict__ x,
-                                                      V* __restrict__ y, const V* __restrict__ u,
-                                                      const V* __restrict__ w2, double* partial,
-                                                      unsigned* counter, double* result) {
-  double d1 = 0.0, d2 = 0.0;
-  for (int row = blockIdx.x * BS + threadIdx.x; row < nr; row += gridDim.x * BS) {
+                                                  const V* __restrict__ val, G g, E e, Red red) {
+  const int lb = xcd_block();
+  const int row = lb * BS + threadIdx.x;
+  double d[2] = {0.0, 0.0};
+  if (row < nr) {
     V acc = 0;
     if constexpr (W > 0) {
 #pragma unroll
       for (int k = 0; k < W; ++k) {
         const int idx = k * ld + row;
-        acc = acc + val[idx] * x[ja[idx]];
+        acc = acc + ldnt(val + idx) * g(ldnt(ja + idx));
       }
     } else {
       for (int k = 0; k < w; ++k) {
         const int idx = k * ld + row;
-        acc = acc + val[idx] * x[ja[idx]];
+        acc = acc + ldnt(val + idx) * g(ldnt(ja + idx));
       }
     }
-    y[row] = acc;
-    d1 += (u ? (double)u[row] : (double)acc) * (double)acc;  // u == nullptr: y.y
-    if (W2) d2 += (double)w2[row] * (double)acc;
+    e(row, acc, d);
   }
-  if (W2) {
-    double v[2] = {d1, d2};
-    grid_sum<2>(v, partial, counter, result);
-  } else {
-    double v1[1] = {d1};
-    grid_sum<1>(v1, partial, counter, result);
+  if constexpr (K > 0) block_partials<K>(d, red.partial, lb);
+}
+
+// SELL: position p = 64*slice + lane holds row perm[p]; slice s occupies
+// [sptr[s], sptr[s+1]) with entry k of its lane at sptr[s] + 64*k + lane.
+template <typename V, int K, class G, class E>
+__global__ __launch_bounds__(BS) void sell_kernel(int nr, const int* __restrict__ sptr,
+                                                   const int* __restrict__ perm, const int* __restrict__ ja,
+                                                   const V* __restrict__ val, G g, E e, Red red) {
+  const int lb = xcd_block();
+  const int p = lb * BS + threadIdx.x;
+  double d[2] = {0.0, 0.0};
+  if (p < nr) {
+    const int s = p >> 6, l = p & 63;
+    const int base = sptr[s] + l;
+    const int w = (sptr[s + 1] - sptr[s]) >> 6;
+    V acc = 0;
+    int k = 0;
+    for (; k + 4 <= w; k += 4) {
+      const int i0 = base + 64 * k;
+      const int j0 = ldnt(ja + i0), j1 = ldnt(ja + i0 + 64), j2 = ldnt(ja + i0 + 128), j3 = ldnt(ja + i0 + 192);
+      const V v0 = ldnt(val + i0), v1 = ldnt(val + i0 + 64), v2 = ldnt(val + i0 + 128), v3 = ldnt(val + i0 + 192);
+      const V x0 = g(j0), x1 = g(j1), x2 = g(j2), x3 = g(j3);
+      acc = acc + v0 * x0;
+      acc = acc + v1 * x1;
+      acc = acc + v2 * x2;
+      acc = acc + v3 * x3;
+    }
+    for (; k < w; ++k) {
+      const int i0 = base + 64 * k;
+      acc = acc + ldnt(val + i0) * g(ldnt(ja + i0));
+    }
+    e(perm[p], acc, d);
   }
+  if constexpr (K > 0) block_partials<K>(d, red.partial, lb);
 }
 
 // --------------------------------------------------------- vector kernels
@@ -272,23 +302,26 @@ __global__ __launch_bounds__(BS) void axpby_kernel(int n, V a, const V* __restri
   else y[i] = a * x[i] + b * y[i];
 }
 
+// Elements per thread of the streaming reduction kernels (each is followed
+// by finalize_kernel over its red_blocks(n) partials).
+constexpr int RED_EPT = 2;
+__host__ __device__ constexpr int red_blocks(int n) { return (n + BS * RED_EPT - 1) / (BS * RED_EPT); }
+
 // dot products: K pairs (x_q, y_q) in one pass
 template <typename V, int K>
 __global__ __launch_bounds__(BS) void dot_kernel(int n, const V* __restrict__ x0, const V* __restrict__ y0,
-                                                  const V* __restrict__ x1, const V* __restrict__ y1,
-                                                  double* partial, unsigned* counter, double* result) {
-  double s0 = 0.0, s1 = 0.0;
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-    s0 += (double)x0[i] * (double)y0[i];
-    if (K > 1) s1 += (double)x1[i] * (double)y1[i];
+                                                  const V* __restrict__ x1, const V* __restrict__ y1, Red red) {
+  double s[2] = {0.0, 0.0};
+  const int b = blockIdx.x * BS * RED_EPT + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < RED_EPT; ++e) {
+    const int i = b + e * BS;
+    if (i < n) {
+      s[0] += (double)x0[i] * (double)y0[i];
+      if (K > 1) s[1] += (double)x1[i] * (double)y1[i];
+    }
   }
-  if (K > 1) {
-    double v[2] = {s0, s1};
-    grid_sum<2>(v, partial, counter, result);
-  } else {
-    double v[1] = {s0};
-    grid_sum<1>(v, partial, counter, result);
-  }
+  block_partials<K>(s, red.partial);
 }
 
 // ------------------------------------------------------------ Krylov steps
@@ -315,40 +348,48 @@ __global__ __launch_bounds__(BS) void cg_pupdate(int n, const double* __restrict
 __global__ __launch_bounds__(BS) void xr_update(int n, double* __restrict__ x, double* __restrict__ r,
                                                  const double* __restrict__ p, const double* __restrict__ q,
                                                  const double* __restrict__ scal, int num_slot, int den_slot,
-                                                 double* partial, unsigned* counter, double* result) {
+                                                 Red red) {
   const double den = scal[den_slot];
   const double alpha = (den != 0.0) ? scal[num_slot] / den : 0.0;
-  double s = 0.0;
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-    x[i] = alpha * p[i] + x[i];
-    const double ri = (-alpha) * q[i] + r[i];
-    r[i] = ri;
-    s += ri * ri;
+  double s[2] = {0.0, 0.0};
+  const int b = blockIdx.x * BS * RED_EPT + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < RED_EPT; ++e) {
+    const int i = b + e * BS;
+    if (i < n) {
+      x[i] = alpha * p[i] + x[i];
+      const double ri = (-alpha) * q[i] + r[i];
+      r[i] = ri;
+      s[0] += ri * ri;
+    }
   }
-  double v[1] = {s};
-  grid_sum<1>(v, partial, counter, result);
+  block_partials<1>(s, red.partial);
 }
 
 // FCG(1): p = z - (gamma/delta) p, and sum_i p_i r_i for the step length.
 __global__ __launch_bounds__(BS) void fcg_pupdate(int n, const double* __restrict__ z, double* __restrict__ p,
                                                    const double* __restrict__ r, const double* __restrict__ scal,
-                                                   int it, double* partial, unsigned* counter, double* result) {
+                                                   int it, Red red) {
   const double c = (it == 1) ? 0.0 : -(scal[S_GAMMA] / scal[S_DELTA]);
-  double s = 0.0;
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-    const double pi = (it == 1) ? z[i] : z[i] + c * p[i];
-    p[i] = pi;
-    s += pi * r[i];
+  double s[2] = {0.0, 0.0};
+  const int b = blockIdx.x * BS * RED_EPT + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < RED_EPT; ++e) {
+    const int i = b + e * BS;
+    if (i < n) {
+      const double pi = (it == 1) ? z[i] : z[i] + c * p[i];
+      p[i] = pi;
+      s[0] += pi * r[i];
+    }
   }
-  double v[1] = {s};
-  grid_sum<1>(v, partial, counter, result);
+  block_partials<1>(s, red.partial);
 }
 
 // ---------------------------------------------------- triangular solves
 // Unit lower / upper solve of the rows of one level set, one lane per row,
-// row sum in stored order:  t_i = b_i - sum_j l_ij t_j  (b_i = x_i, or for
-// the upper solve of ILU b_i = d_i * t1_i, the D scaling of psb_spsm).
-// Upper solve can also finish y = t (beta 0) or y = y + t (beta 1).
+// row sum in stored order:  t_i = b_i - sum_j l_ij t_j  (for the upper solve
+// of ILU b_i = d_i * t1_i, the D scaling of psb_spsm).  BETA: 0 also writes
+// y = t, 1 writes y = y + t, 2 writes t only.
 template <typename V, bool UPPER, int BETA>
 __global__ __launch_bounds__(BS) void trsv_level_kernel(int cnt, const int* __restrict__ rows,
                                                          const int* __restrict__ irp, const int* __restrict__ ja,
@@ -417,8 +458,7 @@ __global__ __launch_bounds__(1024) void dense_lu_small_kernel(int n, const doubl
       if (p != k) { double tmp = w[k]; w[k] = w[p]; w[p] = tmp; }
     }
   __syncthreads();
-  // forward (unit lower)
-  for (int i = 0; i < n; ++i) {
+  for (int i = 0; i < n; ++i) {  // forward (unit lower)
     double s = 0.0;
     for (int j = threadIdx.x; j < i; j += blockDim.x) s += lu[(size_t)i * n + j] * w[j];
     s = wave_sum(s);
@@ -431,7 +471,7 @@ __global__ __launch_bounds__(1024) void dense_lu_small_kernel(int n, const doubl
     }
     __syncthreads();
   }
-  for (int i = n - 1; i >= 0; --i) {
+  for (int i = n - 1; i >= 0; --i) {  // backward
     double s = 0.0;
     for (int j = i + 1 + threadIdx.x; j < n; j += blockDim.x) s += lu[(size_t)i * n + j] * w[j];
     s = wave_sum(s);

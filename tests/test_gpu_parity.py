@@ -71,7 +71,7 @@ def test_apply_fp64(eng, orc, idim, smoother, sweeps):
         assert relmax(y, yo) <= 1e-12
 
 
-@pytest.mark.parametrize("fmt", ["csr", "ell"])
+@pytest.mark.parametrize("fmt", ["auto", "csr", "ell", "sell"])
 def test_level_spmv_bitexact(eng, orc, fmt):
     _say(f"[level spmv] fmt={fmt}")
     a = orc.gen_pde3d(20)
@@ -80,7 +80,7 @@ def test_level_spmv_bitexact(eng, orc, fmt):
     po = oracle_prec(orc, a)
     rng = np.random.default_rng(3)
     for lev in range(p.nlev):
-        for what in (["A", "ND"] if lev == 0 else ["A", "P", "R"]):
+        for what in (["A", "ND"] if lev == 0 else ["A", "P", "R", "ND"]):
             if what == "ND" and p.level_info(lev)["nnz_nd"] == 0:
                 continue
             m = po.level_csr(lev, what)
@@ -90,11 +90,13 @@ def test_level_spmv_bitexact(eng, orc, fmt):
             p.level_spmv(lev, what, dev(x), y)
             yo = m.spmv(x)
             yg = y.cpu().numpy()
-            info = p.level_info(lev)
-            # ELL and single-lane CSR keep the stored order: bit-exact
-            if fmt == "ell" or what == "P":
-                pass
-            assert relmax(yg, yo) <= 1e-14, (lev, what)
+            f = p.op_format(lev, what)
+            if f["fmt"] in ("ELL", "SELL") or f["lanes"] == 1:
+                # one lane per row, stored order, no fused multiply-add:
+                # identical to the sequential CSR loop
+                assert (yg == yo).all(), (lev, what, f)
+            else:
+                assert relmax(yg, yo) <= 1e-14, (lev, what, f)
 
 
 @pytest.mark.parametrize("smoother,sweeps,beta", [("JACOBI", 1, 0), ("JACOBI", 2, 0), ("JACOBI", 3, 1),
