@@ -399,7 +399,15 @@ class DevPrecT : public DevPrec {
       akry_ = upload_mat<double>(p.precv[0].a, ar_, s_, p.dev_format);
     }
     n0_ = L_[0].n;
-    partial_ = ar_.alloc<double>(2 * (size_t)std::max(spmv_blocks(akry_), red_blocks(n0_)) + 64);
+    {
+      // partial sums of every reducing launch: Krylov SpMV, vector reductions,
+      // fused output of the fine level's post smoother
+      int nbmax = std::max(spmv_blocks(akry_), red_blocks(n0_));
+      nbmax = std::max(nbmax, spmv_blocks(L_[0].sm.nd));
+      if (L_[0].has_sm2) nbmax = std::max(nbmax, spmv_blocks(L_[0].sm2.nd));
+      nbmax = std::max(nbmax, nblk(n0_));
+      partial_ = ar_.alloc<double>(2 * (size_t)nbmax + 64);
+    }
     scal_ = ar_.alloc<double>(S_NSCAL);
     HIPCHECK(hipMemsetAsync(scal_, 0, S_NSCAL * sizeof(double), s_));
     HIPCHECK(hipHostMalloc((void**)&hscal_, S_NSCAL * sizeof(double), hipHostMallocDefault));
@@ -415,9 +423,27 @@ class DevPrecT : public DevPrec {
   size_t device_bytes() const override { return ar_.bytes; }
 
   // y = op(M^-1) x (mld_dprecaply2_vect with alpha = 1, beta = 0).
-  int apply(const double* x, double* y, char trans) override {
+  int apply(const double* x, double* y, char trans) override { return apply_impl(x, y, trans, nullptr, -1); }
+
+  // y = op(M^-1) x; when dot_u is given, also scal_[dot_slot] = dot_u . y.
+  // The fine level's last post-smoothing sweep writes y (FP64) and the dot
+  // directly when it can (fused output), else a conversion / dot follows.
+  int apply_impl(const double* x, double* y, char trans, const double* dot_u, int dot_slot) {
     HIPCHECK(hipSetDevice(device_));
     if (trans != 'N') ensure_transposes();
+    out64_ = y;
+    dot_u_ = dot_u;
+    dot_slot_ = dot_slot;
+    out_done_ = false;
+    int rc = apply_cycle(x, y, trans);
+    out64_ = nullptr;
+    dot_u_ = nullptr;
+    if (rc) return rc;
+    if (dot_u && !out_done_) dot1(dot_u, y, dot_slot);
+    return 0;
+  }
+
+  int apply_cycle(const double* x, double* y, char trans) {
     const int nlev = (int)L_.size();
     DLevel<V>& L0 = L_[0];
     if (nlev == 1) {
@@ -429,18 +455,27 @@ class DevPrecT : public DevPrec {
       finish_out(y, yout, L0.n);
       return 0;
     }
-    if (outer_sweeps_ > 1) return apply_outer(x, y, trans);
+    if (outer_sweeps_ > 1) {
+      out64_ = nullptr;  // intermediate cycles must update y2l itself
+      return apply_outer(x, y, trans);
+    }
     // level-0 input/output alias the caller's vectors in FP64
     V* x0save = L0.x2l;
     V* y0save = L0.y2l;
     L0.x2l = const_cast<V*>(in_vec(x, L0.x2l, L0.n));
     L0.y2l = out_vec(y, L0.y2l);
     int rc = inner_mult(0, trans);
-    finish_out(y, L0.y2l, L0.n);
+    if (!out_done_) finish_out(y, L0.y2l, L0.n);
     L0.x2l = x0save;
     L0.y2l = y0save;
     return rc;
   }
+
+  // fused-output state of the current apply_impl call
+  double* out64_ = nullptr;
+  const double* dot_u_ = nullptr;
+  int dot_slot_ = -1;
+  bool out_done_ = false;
 
   int spmv_level(int lev, int what, const double* x, double* y) override {
     if (!std::is_same<V, double>::value) return E_UNSUPPORTED;
@@ -728,18 +763,33 @@ class DevPrecT : public DevPrec {
   // FIRST on an ELL operator gathers d_j*x_j directly; on SELL/CSR, whose
   // gathers are the costly part, d.*x is formed first in one streaming pass
   // (same products, so the same result) and gathered once.
-  template <bool FIRST, bool ADD>
-  void jac_sweep(DLevel<V>& D, DSmoother<V>& S, const V* b, const V* yin, V* yout, char trans) {
+  template <bool FIRST, int K, class E>
+  void jac_launch(DLevel<V>& D, DSmoother<V>& S, const V* b, const V* yin, E e, char trans, Red red) {
     const DMat<V>& nd = (trans != 'N') ? S.ndt : S.nd;
-    EpiJac<V, V, V, ADD> e{S.dv, b, yout};
     if (FIRST && nd.fmt == FMT_ELL) {
-      spmv_launch(nd, GatherDX<V, V>{S.dv, b}, e, s_);
+      spmv_launch<K>(nd, GatherDX<V, V>{S.dv, b}, e, s_, red);
     } else if (FIRST) {
       vmul_kernel<V, V, false><<<nblk(D.n), BS, 0, s_>>>(D.n, S.dv, b, D.t2);
-      spmv_launch(nd, GatherX<V, V>{D.t2}, e, s_);
+      spmv_launch<K>(nd, GatherX<V, V>{D.t2}, e, s_, red);
     } else {
-      spmv_launch(nd, GatherX<V, V>{yin}, e, s_);
+      spmv_launch<K>(nd, GatherX<V, V>{yin}, e, s_, red);
     }
+  }
+  template <bool FIRST, bool ADD>
+  void jac_sweep(DLevel<V>& D, DSmoother<V>& S, const V* b, const V* yin, V* yout, char trans) {
+    jac_launch<FIRST, 0>(D, S, b, yin, EpiJac<V, V, V, ADD>{S.dv, b, yout}, trans, Red{nullptr, nullptr});
+  }
+  // Last post-smoothing sweep of the fine level (beta = 1): writes the
+  // FP64 output of the cycle and the requested dot in the same pass.
+  template <bool FIRST>
+  void jac_sweep_out(DLevel<V>& D, DSmoother<V>& S, const V* b, const V* yin, V* y2l, char trans) {
+    EpiJacOut<V, V> e{S.dv, b, y2l, out64_, dot_u_};
+    if (dot_u_) jac_launch<FIRST, 1>(D, S, b, yin, e, trans, red(dot_slot_));
+    else jac_launch<FIRST, 0>(D, S, b, yin, e, trans, Red{nullptr, nullptr});
+    out_done_ = true;
+  }
+  bool fuse_out(DLevel<V>& D, int beta, V* y) const {
+    return out64_ != nullptr && &D == &L_[0] && beta == 1 && y == D.y2l;
   }
 
   // mld_d_jac_smoother_apply_vect (init 'Z'), alpha = 1, beta in {0,1}.
@@ -751,10 +801,27 @@ class DevPrecT : public DevPrec {
       return 0;
     }
     if (sweeps < 0) return E_BAD_ARG;
-    if (sweeps <= 1 || S.nnz_nd == 0) return solver_apply(D, S, x, beta, y, trans);
-    if (S.solver != DIAG_SCALE && S.solver != L1_DIAG) return E_UNSUPPORTED;
+    const bool jac = (S.solver == DIAG_SCALE || S.solver == L1_DIAG);
+    const bool fo = jac && fuse_out(D, beta, y);
+    if (sweeps <= 1 || S.nnz_nd == 0) {
+      if (fo) {  // y = y + d*x, written as the FP64 output
+        const int nb = nblk(n);
+        if (dot_u_) {
+          vmul_out_kernel<V, 1><<<nb, BS, 0, s_>>>(n, S.dv, x, y, out64_, dot_u_, red(dot_slot_));
+          finalize_kernel<1><<<1, BS, 0, s_>>>(nb, partial_, scal_ + dot_slot_);
+        } else {
+          vmul_out_kernel<V, 0><<<nb, BS, 0, s_>>>(n, S.dv, x, y, out64_, nullptr, Red{nullptr, nullptr});
+        }
+        HIPCHECK(hipGetLastError());
+        out_done_ = true;
+        return 0;
+      }
+      return solver_apply(D, S, x, beta, y, trans);
+    }
+    if (!jac) return E_UNSUPPORTED;
     if (sweeps == 2) {
-      if (beta) jac_sweep<true, true>(D, S, x, nullptr, y, trans);
+      if (fo) jac_sweep_out<true>(D, S, x, nullptr, y, trans);
+      else if (beta) jac_sweep<true, true>(D, S, x, nullptr, y, trans);
       else jac_sweep<true, false>(D, S, x, nullptr, y, trans);
       return 0;
     }
@@ -765,7 +832,8 @@ class DevPrecT : public DevPrec {
       jac_sweep<false, false>(D, S, x, ta, tb, trans);
       std::swap(ta, tb);
     }
-    if (beta) jac_sweep<false, true>(D, S, x, ta, y, trans);
+    if (fo) jac_sweep_out<false>(D, S, x, ta, y, trans);
+    else if (beta) jac_sweep<false, true>(D, S, x, ta, y, trans);
     else jac_sweep<false, false>(D, S, x, ta, y, trans);
     return 0;
   }
@@ -894,9 +962,8 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
   if (method == 0) {  // CG
     while (it < itmax) {
       ++it;
-      int rc = apply(r, z, 'N');
+      int rc = apply_impl(r, z, 'N', r, (it & 1) ? S_RHO1 : S_RHO0);  // z = M^-1 r, rho = r.z
       if (rc) return rc;
-      dot1(r, z, (it & 1) ? S_RHO1 : S_RHO0);
       cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, z, p, scal_, it);
       spmv_launch<1>(akry_, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, s_, red(S_SIGMA));
       xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, (it & 1) ? S_RHO1 : S_RHO0, S_SIGMA, red(S_RR));
@@ -911,9 +978,9 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
   } else if (method == 1) {  // FCG(1)
     while (it < itmax) {
       ++it;
-      int rc = apply(r, z, 'N');
+      // z = M^-1 r and, from the second iteration, gamma = q.z in the same pass
+      int rc = apply_impl(r, z, 'N', it > 1 ? q : nullptr, S_GAMMA);
       if (rc) return rc;
-      if (it > 1) dot1(z, q, S_GAMMA);
       fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z, p, r, scal_, it, red(S_PR));
       finalize(1, S_PR);
       spmv_launch<1>(akry_, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, s_, red(S_DELTA));

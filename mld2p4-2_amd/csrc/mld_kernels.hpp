@@ -77,6 +77,23 @@ struct EpiJac {  // t = d*(b - ND y);  y = t  or  y = y + t
     else y[i] = (YT)t;
   }
 };
+// Last post-smoothing sweep of the fine level, writing the cycle's output:
+// yn = yin + d*(b - ND y);  yout = yn (FP64);  dot u.yout when u != nullptr.
+// yin and yout may be the same array (FP64 cycle).
+template <typename V, typename BT>
+struct EpiJacOut {
+  const V* d;
+  const BT* b;
+  const V* yin;
+  double* yout;
+  const double* u;
+  __device__ __forceinline__ void operator()(int i, V acc, double* dd) const {
+    const V t = d[i] * ((V)b[i] - acc);
+    const V yn = yin[i] + t;
+    yout[i] = (double)yn;
+    if (u) dd[0] = u[i] * (double)yn;
+  }
+};
 // Krylov SpMV: y = A x and dots u.y (u == nullptr: y.y) and w.y
 template <typename V>
 struct EpiSetDot {
@@ -270,6 +287,22 @@ __global__ __launch_bounds__(BS) void vmul_kernel(int n, const V* __restrict__ d
   V t = d[i] * (V)x[i];
   if (ADD) y[i] = y[i] + t;
   else y[i] = t;
+}
+
+// One-sweep post-smoothing of the fine level writing the cycle's output:
+// yout = yin + d*x (FP64), with the dot u.yout when u != nullptr.
+template <typename V, int K>
+__global__ __launch_bounds__(BS) void vmul_out_kernel(int n, const V* __restrict__ d, const V* __restrict__ x,
+                                                       const V* yin, double* yout, const double* __restrict__ u,
+                                                       Red red) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  double dd[2] = {0.0, 0.0};
+  if (i < n) {
+    const V yn = yin[i] + d[i] * x[i];
+    yout[i] = (double)yn;
+    if (K > 0) dd[0] = u[i] * (double)yn;
+  }
+  if constexpr (K > 0) block_partials<K>(dd, red.partial);
 }
 
 template <typename TO, typename TI>
