@@ -87,6 +87,22 @@ int mldhip_krylov(mldhip_prec* p, const char* method, const double* b, double* x
 int mldhip_krylov_host(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,
                        int istop, int* iter, double* err);
 
+/* ---- differentiation (no reference counterpart: the reference computes no
+ * gradients) --------------------------------------------------------------- */
+/* Adjoint solve A^T x = b with the same hierarchy applied transposed
+ * (prec%apply with trans = 'T', mld_dmlprec_aply.f90:203-357 'T' branch) and
+ * the transposed fine matrix; arguments as mldhip_krylov.  Transposed
+ * operators are uploaded on the first call and kept. */
+int mldhip_krylov_adjoint(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,
+                          int istop, int* iter, double* err);
+/* Edge-product gradient of the 7-point finite-volume diffusion operator
+ * A(theta) on the unit cube (idim^3 cells, h = 1/idim, k = exp(theta),
+ * harmonic face coefficients, zero Dirichlet faces contributing k_i):
+ * grad[c] = -lambda^T (dA/dtheta_c) x.  theta, x, lam, grad: device, length
+ * idim^3; enqueued on `stream` (a hipStream_t, NULL = default stream). */
+int mldhip_fv7_grad_theta(int idim, const double* theta, const double* x, const double* lam, double* grad,
+                          void* stream);
+
 /* ---- streams ------------------------------------------------------------ */
 int mldhip_set_stream(mldhip_prec* p, void* hip_stream);
 int mldhip_synchronize(mldhip_prec* p);

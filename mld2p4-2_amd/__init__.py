@@ -80,6 +80,8 @@ def lib():
             "mldhip_level_smooth": [vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, C.c_char],
             "mldhip_time_step": [vp, C.c_int, C.c_int, C.c_int, dp],
             "mldhip_get_op_format": [vp, C.c_int, C.c_int, ip, ip, ip],
+            "mldhip_krylov_adjoint": [vp, cp, vp, vp, C.c_double, C.c_int, C.c_int, ip, dp],
+            "mldhip_fv7_grad_theta": [C.c_int, vp, vp, vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -332,3 +334,36 @@ def psb_krylov(method, a, prec: MldPrec, b, x=None, eps=1e-6, itmax=500, istop=2
                                  C.byref(err))
     prec._check(rc, f"krylov({method})")
     return x, it.value, err.value
+
+
+def psb_krylov_adjoint(method, prec: MldPrec, b, x=None, eps=1e-6, itmax=500, istop=2):
+    """Adjoint solve A^T x = b with the hierarchy of `prec` applied
+    transposed (prec%apply(..., trans='T')).  b, x: float64 CUDA tensors.
+    Returns (x, iter, err)."""
+    import torch
+
+    it = C.c_int(0)
+    err = C.c_double(0.0)
+    x = torch.zeros_like(b) if x is None else x
+    prec.use_torch_stream()
+    rc = lib().mldhip_krylov_adjoint(prec.h, method.encode(), _ptr(b), _ptr(x), eps, itmax, istop, C.byref(it),
+                                     C.byref(err))
+    prec._check(rc, f"krylov_adjoint({method})")
+    return x, it.value, err.value
+
+
+def fv7_grad_theta(idim, theta, x, lam, grad=None):
+    """grad[c] = -lam^T (dA/dtheta_c) x for the 7-point finite-volume
+    diffusion operator with k = exp(theta) (device kernel, torch's current
+    stream).  All tensors float64 on the GPU, length idim^3."""
+    import torch
+
+    for t in (theta, x, lam):
+        if t.dtype != torch.float64 or not t.is_cuda or t.numel() != idim ** 3 or not t.is_contiguous():
+            raise MldError(-1, "fv7_grad_theta: float64 contiguous CUDA tensors of length idim^3 expected")
+    grad = torch.empty_like(theta) if grad is None else grad
+    s = torch.cuda.current_stream(theta.device).cuda_stream
+    rc = lib().mldhip_fv7_grad_theta(idim, _ptr(theta), _ptr(x), _ptr(lam), _ptr(grad), C.c_void_p(s))
+    if rc:
+        raise MldError(rc, "fv7_grad_theta")
+    return grad

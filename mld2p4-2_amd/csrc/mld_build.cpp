@@ -417,9 +417,15 @@ static int build_smoother(const HCsr& a, const SmootherSpec& spec, SmootherData&
       level_schedule(sd.u, false, sd.usched_ptr, sd.usched_rows);
       return 0;
     }
-    case DENSE_LU:
+    case DENSE_LU: {
+      // dense row-major copy; factorized and inverted on the device
       sd.nnz_nd = 0;
-      return dense_lu(a, sd.lu, sd.piv);
+      if (n > 16384) return E_UNSUPPORTED;
+      sd.lu.assign((size_t)n * n, 0.0);
+      for (int i = 0; i < n; ++i)
+        for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) sd.lu[(size_t)i * n + a.ja[k]] += a.val[k];
+      return 0;
+    }
     default:
       return E_UNSUPPORTED;  // GS/BWGS/MILU/ILUT and third-party solvers
   }

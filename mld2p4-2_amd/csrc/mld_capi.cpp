@@ -191,7 +191,17 @@ int mldhip_krylov(mldhip_prec* p, const char* method, const double* b, double* x
   int mc = method_code(method);
   if (mc < 0) return fail(p, mld::E_UNSUPPORTED, "Krylov method not provided: " + S(method));
   if (!iter || !err) return E_BAD_ARG;
-  return mld::dev_guard([&]() { return p->dev->krylov(mc, b, x, eps, itmax, istop, iter, err); }, p->err);
+  return mld::dev_guard([&]() { return p->dev->krylov(mc, b, x, eps, itmax, istop, iter, err, 'N'); }, p->err);
+}
+
+int mldhip_krylov_adjoint(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,
+                          int istop, int* iter, double* err) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  int mc = method_code(method);
+  if (mc < 0) return fail(p, mld::E_UNSUPPORTED, "Krylov method not provided: " + S(method));
+  if (!iter || !err) return E_BAD_ARG;
+  return mld::dev_guard([&]() { return p->dev->krylov(mc, b, x, eps, itmax, istop, iter, err, 'T'); }, p->err);
 }
 
 int mldhip_krylov_host(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,

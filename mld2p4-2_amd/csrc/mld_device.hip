@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "mld_device.hpp"
+#include "mld_dense.hpp"
 #include "mld_kernels.hpp"
 
 namespace mld {
@@ -63,6 +64,8 @@ struct DSmoother {
   bool has_t = false;
   // dense LU
   double* lu = nullptr; int* piv = nullptr;
+  double* dinv = nullptr;  // DENSE_LU: explicit inverse (row-major)
+  int ndense = 0;
 };
 
 template <typename V>
@@ -503,7 +506,7 @@ class DevPrecT : public DevPrec {
   }
 
   int krylov(int method, const double* b, double* x, double eps, int itmax, int istop, int* iter,
-             double* err) override;
+             double* err, char trans) override;
 
   int sync() override {
     HIPCHECK(hipStreamSynchronize(s_));
@@ -597,6 +600,8 @@ class DevPrecT : public DevPrec {
   int outer_sweeps_ = 1;
   int n0_ = 0;
   DMat<double> akry_;
+  DMat<double> akry_t_;  // FP64 A^T for adjoint solves, uploaded on first use
+  bool akry_t_set_ = false;
   double* partial_ = nullptr;
   Red red(int slot) { return Red{partial_, scal_ + slot}; }
   // one-block sum of the partials of a streaming reduction over n0_ entries
@@ -653,10 +658,52 @@ class DevPrecT : public DevPrec {
       HIPCHECK(hipStreamSynchronize(s_));
     }
     if (H.solver == DENSE_LU) {
-      S.lu = ar_.upload(H.lu.data(), H.lu.size(), s_);
-      S.piv = ar_.upload(H.piv.data(), H.piv.size(), s_);
-      HIPCHECK(hipStreamSynchronize(s_));
+      const int n = (int)std::lround(std::sqrt((double)H.lu.size()));
+      S.lu = ar_.upload(H.lu.data(), H.lu.size(), s_);  // factorized in place
+      S.piv = ar_.alloc<int>(n);
+      S.dinv = ar_.alloc<double>((size_t)n * n);
+      S.ndense = n;
+      dense_factor_invert(n, S.lu, S.piv, S.dinv);
     }
+  }
+
+  // Blocked LU with partial pivoting (MFMA trailing updates) and the explicit
+  // inverse X = U^-1 L^-1 P of the n x n row-major matrix a (overwritten by
+  // its factors).
+  void dense_factor_invert(int n, double* a, int* piv, double* x) {
+    const int nb = 32;
+    int* info = ar_.alloc<int>(1);
+    HIPCHECK(hipMemsetAsync(info, 0, sizeof(int), s_));
+    auto gemm = [&](int m, int nc, int kd, const double* A, const double* B, double* C) {
+      if (m <= 0 || nc <= 0 || kd <= 0) return;
+      const int tiles = ((m + 15) / 16) * ((nc + 15) / 16);
+      gemm_sub_mfma_kernel<<<(tiles + 3) / 4, 256, 0, s_>>>(m, nc, kd, A, n, B, n, C, n);
+    };
+    for (int k0 = 0; k0 < n; k0 += nb) {
+      const int kend = std::min(n, k0 + nb);
+      panel_factor_kernel<<<1, 1024, 0, s_>>>(n, a, k0, nb, piv, info);
+      if (kend < n) {
+        trsm_cols_kernel<true><<<nblk(n - kend), 256, 0, s_>>>(n, a, k0, nb, a, n, kend, n);
+        gemm(n - kend, n - kend, kend - k0, a + (size_t)kend * n + k0, a + (size_t)k0 * n + kend,
+             a + (size_t)kend * n + kend);
+      }
+    }
+    perm_identity_kernel<<<nblk(n), 256, 0, s_>>>(n, piv, x);
+    for (int k0 = 0; k0 < n; k0 += nb) {  // L Y = P X
+      const int kend = std::min(n, k0 + nb);
+      trsm_cols_kernel<true><<<nblk(n), 256, 0, s_>>>(n, a, k0, nb, x, n, 0, n);
+      if (kend < n) gemm(n - kend, n, kend - k0, a + (size_t)kend * n + k0, x + (size_t)k0 * n, x + (size_t)kend * n);
+    }
+    for (int k0 = ((n - 1) / nb) * nb; k0 >= 0; k0 -= nb) {  // U X = Y
+      const int kend = std::min(n, k0 + nb);
+      trsm_cols_kernel<false><<<nblk(n), 256, 0, s_>>>(n, a, k0, nb, x, n, 0, n);
+      if (k0 > 0) gemm(k0, n, kend - k0, a + k0, x + (size_t)k0 * n, x);
+    }
+    HIPCHECK(hipGetLastError());
+    int hinfo = 0;
+    HIPCHECK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s_));
+    HIPCHECK(hipStreamSynchronize(s_));
+    if (hinfo != 0) throw DevError("dense LU: zero pivot in column " + std::to_string(hinfo));
   }
 
   // Transposed operators for trans = 'T' (A^T, ND^T, U^T and L^T).
@@ -747,10 +794,10 @@ class DevPrecT : public DevPrec {
         }
         break;
       }
-      case DENSE_LU:
+      case DENSE_LU:  // y = A^-1 x with the inverse formed at upload (FP64)
         if (trans != 'N') return E_UNSUPPORTED;
-        if (beta) dense_lu_small_kernel<V, 1><<<1, 1024, 0, s_>>>(n, S.lu, S.piv, x, D.wlu, y);
-        else dense_lu_small_kernel<V, 0><<<1, 1024, 0, s_>>>(n, S.lu, S.piv, x, D.wlu, y);
+        if (beta) dense_gemv_kernel<V, 1><<<(n + 3) / 4, 256, 0, s_>>>(n, S.dinv, x, y);
+        else dense_gemv_kernel<V, 0><<<(n + 3) / 4, 256, 0, s_>>>(n, S.dinv, x, y);
         HIPCHECK(hipGetLastError());
         break;
       default:
@@ -939,17 +986,32 @@ class DevPrecT : public DevPrec {
 // ---------------------------------------------------------------------------
 template <typename V>
 int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int itmax, int istop, int* iter,
-                        double* err) {
+                        double* err, char trans) {
   HIPCHECK(hipSetDevice(device_));
   if (istop != 2) return E_UNSUPPORTED;
+  if (trans != 'N' && trans != 'T') return E_BAD_ARG;
   const int n = n0_;
+  if (trans == 'T') {
+    // adjoint system A^T x = b with M^T: transposed operators of every level
+    // (uploaded once, reused by later adjoint solves)
+    ensure_transposes();
+    if (!akry_t_set_) {
+      if constexpr (std::is_same<V, double>::value) {
+        akry_t_ = L_[0].at;
+      } else {
+        akry_t_ = upload_mat<double>(host_transpose(hprec_->precv[0].a), ar_, s_, hprec_->dev_format);
+      }
+      akry_t_set_ = true;
+    }
+  }
+  const DMat<double>& akry = (trans == 'T') ? akry_t_ : akry_;
   double* r = kvec(0);
   double* z = kvec(1);
   double* p = kvec(2);
   double* q = kvec(3);
   // bn2 and r = b - A x
   dot1(b, b, S_TMP0);
-  spmv_launch(akry_, GatherX<double, double>{x}, EpiRes<double, double>{b, r}, s_);
+  spmv_launch(akry, GatherX<double, double>{x}, EpiRes<double, double>{b, r}, s_);
   dot1(r, r, S_RR);
   HIPCHECK(hipMemcpyAsync(hscal_, scal_, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, s_));
   HIPCHECK(hipStreamSynchronize(s_));
@@ -962,10 +1024,10 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
   if (method == 0) {  // CG
     while (it < itmax) {
       ++it;
-      int rc = apply_impl(r, z, 'N', r, (it & 1) ? S_RHO1 : S_RHO0);  // z = M^-1 r, rho = r.z
+      int rc = apply_impl(r, z, trans, r, (it & 1) ? S_RHO1 : S_RHO0);  // z = M^-1 r, rho = r.z
       if (rc) return rc;
       cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, z, p, scal_, it);
-      spmv_launch<1>(akry_, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, s_, red(S_SIGMA));
+      spmv_launch<1>(akry, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, s_, red(S_SIGMA));
       xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, (it & 1) ? S_RHO1 : S_RHO0, S_SIGMA, red(S_RR));
       finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
@@ -979,11 +1041,11 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
     while (it < itmax) {
       ++it;
       // z = M^-1 r and, from the second iteration, gamma = q.z in the same pass
-      int rc = apply_impl(r, z, 'N', it > 1 ? q : nullptr, S_GAMMA);
+      int rc = apply_impl(r, z, trans, it > 1 ? q : nullptr, S_GAMMA);
       if (rc) return rc;
       fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z, p, r, scal_, it, red(S_PR));
       finalize(1, S_PR);
-      spmv_launch<1>(akry_, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, s_, red(S_DELTA));
+      spmv_launch<1>(akry, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, s_, red(S_DELTA));
       xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, S_PR, S_DELTA, red(S_RR));
       finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
@@ -1013,9 +1075,9 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
         axpby(-omega, v, 1.0, p);
         axpby(1.0, r, beta, p);
       }
-      int rc = apply(p, ph, 'N');
+      int rc = apply(p, ph, trans);
       if (rc) return rc;
-      spmv_launch<1>(akry_, GatherX<double, double>{ph}, EpiSetDot<double>{v, rt, nullptr}, s_, red(S_TMP1));
+      spmv_launch<1>(akry, GatherX<double, double>{ph}, EpiSetDot<double>{v, rt, nullptr}, s_, red(S_TMP1));
       const double rtv = read_scal(S_TMP1);
       if (rtv == 0.0) break;
       alpha = rho / rtv;
@@ -1029,9 +1091,9 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
         *err = sn;
         break;
       }
-      rc = apply(s, sh, 'N');
+      rc = apply(s, sh, trans);
       if (rc) return rc;
-      spmv_launch<2>(akry_, GatherX<double, double>{sh}, EpiSetDot<double>{t, nullptr, s}, s_, red(S_TMP0));  // t.t, s.t
+      spmv_launch<2>(akry, GatherX<double, double>{sh}, EpiSetDot<double>{t, nullptr, s}, s_, red(S_TMP0));  // t.t, s.t
       read_scal(S_TMP0);
       const double tt = hscal_[S_TMP0], ts = hscal_[S_TMP1];
       omega = (tt == 0.0) ? 0.0 : ts / tt;

@@ -17,8 +17,10 @@ class DevPrec {
   virtual int apply(const double* x, double* y, char trans) = 0;
   virtual int spmv_level(int lev, int what, const double* x, double* y) = 0;
   virtual int smooth_level(int lev, const double* x, int beta, double* y, int sweeps, char trans, int which) = 0;
+  // trans 'N': solve A x = b preconditioned by M; 'T': the adjoint system
+  // A^T x = b preconditioned by M^T (same hierarchy, transposed operators).
   virtual int krylov(int method, const double* b, double* x, double eps, int itmax, int istop, int* iter,
-                     double* err) = 0;
+                     double* err, char trans) = 0;
   virtual int sync() = 0;
   virtual void set_stream(hipStream_t s) = 0;
   virtual hipStream_t stream() const = 0;

@@ -86,7 +86,7 @@ struct SmootherData {
   HCsr l, u;                  // ILU(0) strict factors
   // level schedules of the triangular solves (rows grouped by level)
   std::vector<int> lsched_ptr, lsched_rows, usched_ptr, usched_rows;
-  std::vector<double> lu;     // dense LU (row-major), DENSE_LU solver
+  std::vector<double> lu;     // DENSE_LU: the dense matrix (row-major), factorized on the device
   std::vector<int> piv;
   bool iterative() const { return false; }
 };

@@ -25,6 +25,7 @@ def load_engine():
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu)")
+    load_engine()  # importable as mld2p4_amd (the library itself loads lazily)
 
 
 @pytest.fixture(scope="session")

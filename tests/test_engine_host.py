@@ -170,3 +170,36 @@ def test_byte_model(eng):
         [dict(n=10, nnz_a=70, nnz_p=0, nnz_r=0, nnz_nd=70, solver=101, sweeps_pre=2, sweeps_post=2),
          dict(n=3, nnz_a=9, nnz_p=20, nnz_r=20, nnz_nd=0, solver=7, sweeps_pre=4, sweeps_post=4, nnz_lu=6)], 8, 8)
     assert len(steps) == 7 and tot == sum(s[2] for s in steps)
+
+
+def test_diffusion_generator_properties(eng):
+    """Heterogeneous FV diffusion: symmetric, zero row sums on interior rows
+    (flux conservation), positive boundary row sums (Dirichlet), harmonic
+    face coefficients, seeded and reproducible."""
+    import scipy.sparse as sp
+    from mld2p4_amd import problems
+
+    idim = 9
+    irp, ja, val, b = problems.diffusion3d(idim, seed=11)
+    irp2, ja2, val2, b2 = problems.diffusion3d(idim, seed=11)
+    assert (val == val2).all() and (b == b2).all()
+    n = idim ** 3
+    A = sp.csr_matrix((val, ja, irp), shape=(n, n))
+    assert abs(A - A.T).max() == 0.0
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    g = np.arange(n)
+    x, y, z = g // idim ** 2, (g // idim) % idim, g % idim
+    inner = (x > 0) & (x < idim - 1) & (y > 0) & (y < idim - 1) & (z > 0) & (z < idim - 1)
+    assert np.abs(rs[inner]).max() <= 1e-12 * A.diagonal().max()
+    assert rs[~inner].min() > 0
+    # one face by hand: k from the smoothed field, harmonic mean, / h^2
+    rng = np.random.Generator(np.random.Philox(11))
+    f = rng.standard_normal((idim, idim, idim))
+    for _ in range(3):
+        f = problems.box3(f)
+    k = np.exp(2.0 * f).reshape(n)
+    i = (4 * idim + 4) * idim + 4
+    assert np.isclose(A[i, i + 1], -2 * k[i] * k[i + 1] / (k[i] + k[i + 1]) * idim ** 2, rtol=1e-14)
+    assert np.isclose(A[i, i + idim * idim], -2 * k[i] * k[i + idim * idim] / (k[i] + k[i + idim * idim]) * idim ** 2,
+                      rtol=1e-14)
+    assert b.min() >= -1.0 and b.max() <= 1.0

@@ -158,3 +158,55 @@ def test_apply_transpose_symmetric(eng, orc):
     yt = p.apply(dev(x), trans="T").cpu().numpy()
     yto = po.apply(x, trans="T")
     assert relmax(yt, yto) <= 1e-12
+
+
+def test_dense_lu_coarse_apply(eng, orc):
+    """DENSE_LU coarsest solve: device blocked LU (MFMA updates) + inverse."""
+    _say("[dense LU coarse] idim=14")
+    a = orc.gen_pde3d(14)
+    A = a.to_scipy()
+    p = engine_prec(eng, A, "JACOBI", 2, coarse="DENSE_LU", csweeps=1, mcs=600)
+    po = oracle_prec(orc, a, "JACOBI", 2, coarse="DENSE_LU", csweeps=1, mcs=600)
+    assert p.nlev == po.nlev and p.level_info(p.nlev - 1)["solver"] == 102
+    assert p.level_info(p.nlev - 1)["n"] > 100
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(A.shape[0])
+    assert relmax(p.apply(dev(x)).cpu().numpy(), po.apply(x)) <= 1e-12
+
+
+def test_dense_lu_two_level_identity(eng, orc):
+    """No smoothing and an exact coarse solve: M = P (P^T A P)^-1 P^T."""
+    _say("[dense LU identity] idim=10")
+    a = orc.gen_pde3d(10)
+    A = a.to_scipy()
+    p = eng.MldPrec(device=0).init("ML")
+    for k, v in [("SMOOTHER_TYPE", "JACOBI"), ("SMOOTHER_SWEEPS", 0), ("AGGR_THRESH", 0.0), ("MAX_LEVS", 2),
+                 ("MIN_COARSE_SIZE", 5), ("COARSE_SOLVE", "DENSE_LU")]:
+        p.set(k, v)
+    p.build(A)
+    P = p.level_csr(1, "P").toarray()
+    Ad = A.toarray()
+    M = P @ np.linalg.solve(P.T @ Ad @ P, P.T)
+    x = np.random.default_rng(4).standard_normal(A.shape[0])
+    y = p.apply(dev(x)).cpu().numpy()
+    assert relmax(y, M @ x) <= 1e-10
+
+
+def test_diffusion_mixed_fcg(eng, orc):
+    """Heterogeneous diffusion, FP32 hierarchy inside FP64 FCG, dense-LU coarse:
+    converges to 1e-8 and matches the oracle's FP64 solution."""
+    _say("[diffusion mixed] idim=32")
+    from mld2p4_amd import problems
+
+    irp, ja, val, b = problems.diffusion3d(32, seed=3)
+    a = orc.Csr.from_arrays(irp, ja, val)
+    A = a.to_scipy()
+    p = engine_prec(eng, A, "L1-JACOBI", 2, coarse="DENSE_LU", csweeps=1, mcs=4096, precision="mixed")
+    x, it, err = eng.psb_krylov("FCG", A, p, dev(b), eps=1e-8)
+    assert err <= 1e-8
+    po = oracle_prec(orc, a, "L1-JACOBI", 2, coarse="DENSE_LU", csweeps=1, mcs=4096)
+    xo, ito, erro = po.krylov("CG", b, eps=1e-12, itmax=300)
+    xg = x.cpu().numpy()
+    rel = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    print(f"mixed FCG iters {it} (oracle FP64 CG to 1e-12: {ito}); solution error vs oracle {rel:.2e}")
+    assert rel <= 1e-6
