@@ -103,6 +103,53 @@ int mldhip_krylov_adjoint(mldhip_prec* p, const char* method, const double* b, d
 int mldhip_fv7_grad_theta(int idim, const double* theta, const double* x, const double* lam, double* grad,
                           void* stream);
 
+/* ---- batched forms (no reference counterpart: the reference applies its
+ * preconditioner to one vector of one system at a time) ----------------------
+ * Multi-vectors are column-major with a leading dimension (device pointers).
+ *
+ * Block V-cycle: y = M^-1 x for nrhs <= 32 columns at once, every operator of
+ * the hierarchy applied as a sparse-times-dense product (one pass over each
+ * matrix for all columns); the same cycle as mldhip_apply per column, with
+ * the coarsest solve applied as its explicit operator.  The block engine is
+ * uploaded on first use.  Point / l1-Jacobi smoothers above the coarsest
+ * level; coarsest ILU(0), dense LU or Jacobi. */
+int mldhip_apply_block(mldhip_prec* p, int nrhs, const double* x, int64_t ldx, double* y, int64_t ldy);
+/* psb_krylov per column of b (method "CG" or "FCG", istop = 2), all columns
+ * advanced together with per-column scalars; iters/errs: host arrays of
+ * nrhs entries.  x holds the initial guesses. */
+int mldhip_krylov_block(mldhip_prec* p, const char* method, int nrhs, const double* b, int64_t ldb, double* x,
+                        int64_t ldx, double eps, int itmax, int istop, int* iters, double* errs);
+
+/* Groups of independent systems, each built in its own handle (hierarchy
+ * and smoothers; the handles must stay alive until mldhip_group_build
+ * returns).  The group runs one V-cycle over the block-diagonal unions of
+ * the systems' levels and captures the CG/FCG iterations of all systems
+ * into one hipGraph, replayed until every system has converged.  Vectors
+ * are the concatenation of the systems' vectors in the order of
+ * mldhip_group_add (offsets from mldhip_group_info), nrhs columns of them
+ * (nb = 1: one column; nb = 32: up to 32). */
+typedef struct mldhip_group mldhip_group;
+int mldhip_group_create(mldhip_group** g, int device, int nb);
+int mldhip_group_destroy(mldhip_group* g);
+int mldhip_group_add(mldhip_group* g, mldhip_prec* p);
+/* precision: 0 FP64, 1 FP32 preconditioner (Krylov vectors stay FP64). */
+int mldhip_group_build(mldhip_group* g, int precision);
+/* nsys, total rows, offsets[nsys+1] (any may be NULL). */
+int mldhip_group_info(mldhip_group* g, int* nsys, int64_t* nrows, int64_t* offsets);
+/* Stream the group's calls are ordered after (NULL: the null stream). */
+int mldhip_group_set_stream(mldhip_group* g, void* hip_stream);
+int mldhip_group_apply(mldhip_group* g, int nrhs, const double* x, int64_t ldx, double* y, int64_t ldy);
+/* iters/errs: host arrays of nsys*nrhs entries (index j*nsys + s);
+ * launches: graph launches used (may be NULL).  Synchronous. */
+int mldhip_group_krylov(mldhip_group* g, const char* method, int nrhs, const double* b, int64_t ldb, double* x,
+                        int64_t ldx, double eps, int itmax, int istop, int* iters, double* errs, int* launches);
+/* Average time of one group preconditioner application (HIP events). */
+int mldhip_group_time_apply(mldhip_group* g, int reps, double* avg_ms);
+/* Union level info[10]: n, continuing rows, nnz A, P, R, ND(+ND2), coarse
+ * operator, sweeps pre, post, storage (format or columns per lane). */
+int mldhip_group_level_info(mldhip_group* g, int lev, int64_t* info);
+int mldhip_group_get_error(mldhip_group* g, char* buf, int len);
+
 /* ---- streams ------------------------------------------------------------ */
 int mldhip_set_stream(mldhip_prec* p, void* hip_stream);
 int mldhip_synchronize(mldhip_prec* p);

@@ -82,6 +82,21 @@ def lib():
             "mldhip_get_op_format": [vp, C.c_int, C.c_int, ip, ip, ip],
             "mldhip_krylov_adjoint": [vp, cp, vp, vp, C.c_double, C.c_int, C.c_int, ip, dp],
             "mldhip_fv7_grad_theta": [C.c_int, vp, vp, vp, vp, vp],
+            "mldhip_apply_block": [vp, C.c_int, vp, C.c_int64, vp, C.c_int64],
+            "mldhip_krylov_block": [vp, cp, C.c_int, vp, C.c_int64, vp, C.c_int64, C.c_double, C.c_int, C.c_int, ip,
+                                    dp],
+            "mldhip_group_create": [C.POINTER(vp), C.c_int, C.c_int],
+            "mldhip_group_destroy": [vp],
+            "mldhip_group_add": [vp, vp],
+            "mldhip_group_build": [vp, C.c_int],
+            "mldhip_group_info": [vp, ip, i64p, i64p],
+            "mldhip_group_set_stream": [vp, vp],
+            "mldhip_group_apply": [vp, C.c_int, vp, C.c_int64, vp, C.c_int64],
+            "mldhip_group_krylov": [vp, cp, C.c_int, vp, C.c_int64, vp, C.c_int64, C.c_double, C.c_int, C.c_int, ip,
+                                    dp, ip],
+            "mldhip_group_time_apply": [vp, C.c_int, dp],
+            "mldhip_group_level_info": [vp, C.c_int, i64p],
+            "mldhip_group_get_error": [vp, C.c_char_p, C.c_int],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -211,6 +226,34 @@ class MldPrec:
         self.use_torch_stream()
         self._check(lib().mldhip_apply(self.h, _ptr(x), _ptr(y), trans.encode()), "apply")
         return y
+
+    def apply_block(self, X, Y=None):
+        """Y = M^-1 X for up to 32 columns at once (block V-cycle).  X: float64
+        CUDA tensor of shape (nrhs, n), one column of the multi-vector per
+        row (column-major storage, leading dimension n)."""
+        import torch
+
+        if X.dim() != 2 or not X.is_contiguous():
+            raise MldError(-1, "apply_block: contiguous (nrhs, n) tensor expected")
+        Y = torch.empty_like(X) if Y is None else Y
+        self.use_torch_stream()
+        k, n = X.shape
+        self._check(lib().mldhip_apply_block(self.h, k, _ptr(X), n, _ptr(Y), n), "apply_block")
+        return Y
+
+    def krylov_block(self, method, B, X=None, eps=1e-8, itmax=1000, istop=2):
+        """psb_krylov on every column of B (shape (nrhs, n)) at once.
+        Returns (X, iters, errs) with per-column numpy arrays."""
+        import torch
+
+        k, n = B.shape
+        X = torch.zeros_like(B) if X is None else X
+        it = np.zeros(k, dtype=np.int32)
+        er = np.zeros(k, dtype=np.float64)
+        self.use_torch_stream()
+        self._check(lib().mldhip_krylov_block(self.h, method.encode(), k, _ptr(B), n, _ptr(X), n, eps, itmax, istop,
+                                              it.ctypes.data_as(C.POINTER(C.c_int)), _dp(er)), "krylov_block")
+        return X, it, er
 
     def use_torch_stream(self):
         import torch
@@ -367,3 +410,98 @@ def fv7_grad_theta(idim, theta, x, lam, grad=None):
     if rc:
         raise MldError(rc, "fv7_grad_theta")
     return grad
+
+
+class MldGroup:
+    """Independent systems solved together (grouped V-cycle + one hipGraph
+    for the Krylov iterations of all systems).  nb = 1: one column per
+    system; nb = 32: up to 32 columns each."""
+
+    def __init__(self, device: int = 0, nb: int = 1):
+        self.h = C.c_void_p()
+        self.device = device
+        self._sys = []
+        rc = lib().mldhip_group_create(C.byref(self.h), device, nb)
+        if rc:
+            raise MldError(rc, "group_create")
+
+    def _check(self, rc, what):
+        if rc:
+            buf = C.create_string_buffer(512)
+            lib().mldhip_group_get_error(self.h, buf, 512)
+            raise MldError(rc, f"{what}: {buf.value.decode(errors='replace')}")
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().mldhip_group_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+    def add(self, prec: MldPrec):
+        self._check(lib().mldhip_group_add(self.h, prec.h), "group_add")
+        self._sys.append(prec)
+        return self
+
+    def build(self, precision: str = "double"):
+        self._check(lib().mldhip_group_build(self.h, {"double": 0, "mixed": 1}[precision]), "group_build")
+        return self
+
+    def info(self):
+        ns = C.c_int(0)
+        nr = C.c_int64(0)
+        self._check(lib().mldhip_group_info(self.h, C.byref(ns), C.byref(nr), None), "group_info")
+        off = np.zeros(ns.value + 1, dtype=np.int64)
+        self._check(lib().mldhip_group_info(self.h, None, None, off.ctypes.data_as(C.POINTER(C.c_int64))),
+                    "group_info")
+        return ns.value, nr.value, off
+
+    def _stream(self):
+        import torch
+
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        lib().mldhip_group_set_stream(self.h, C.c_void_p(s))
+
+    def apply(self, X, Y=None):
+        """X: float64 CUDA tensor (nrows,) or (nrhs, nrows)."""
+        import torch
+
+        X2 = X if X.dim() == 2 else X.view(1, -1)
+        Y = torch.empty_like(X) if Y is None else Y
+        Y2 = Y if Y.dim() == 2 else Y.view(1, -1)
+        k, n = X2.shape
+        self._stream()
+        self._check(lib().mldhip_group_apply(self.h, k, _ptr(X2), n, _ptr(Y2), n), "group_apply")
+        return Y
+
+    def krylov(self, method, B, X=None, eps=1e-8, itmax=1000, istop=2):
+        """Returns (X, iters, errs, graph_launches); iters/errs of shape
+        (nrhs, nsys)."""
+        import torch
+
+        B2 = B if B.dim() == 2 else B.view(1, -1)
+        X = torch.zeros_like(B) if X is None else X
+        X2 = X if X.dim() == 2 else X.view(1, -1)
+        k, n = B2.shape
+        ns = len(self._sys)
+        it = np.zeros(k * ns, dtype=np.int32)
+        er = np.zeros(k * ns, dtype=np.float64)
+        nl = C.c_int(0)
+        self._stream()
+        self._check(lib().mldhip_group_krylov(self.h, method.encode(), k, _ptr(B2), n, _ptr(X2), n, eps, itmax,
+                                              istop, it.ctypes.data_as(C.POINTER(C.c_int)), _dp(er), C.byref(nl)),
+                    "group_krylov")
+        return X, it.reshape(k, ns), er.reshape(k, ns), nl.value
+
+    def time_apply(self, reps=10):
+        ms = C.c_double(0.0)
+        self._check(lib().mldhip_group_time_apply(self.h, reps, C.byref(ms)), "group_time_apply")
+        return ms.value
+
+    def level_info(self, lev):
+        info = np.zeros(10, dtype=np.int64)
+        self._check(lib().mldhip_group_level_info(self.h, lev, info.ctypes.data_as(C.POINTER(C.c_int64))),
+                    "group_level_info")
+        keys = ["n", "ni", "nnz_a", "nnz_p", "nnz_r", "nnz_nd", "nnz_coarse", "sweeps_pre", "sweeps_post", "storage"]
+        return dict(zip(keys, (int(v) for v in info)))

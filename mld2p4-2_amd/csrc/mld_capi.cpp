@@ -4,10 +4,12 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/mld2p4_amd.h"
 #include "mld_device.hpp"
 #include "mld_host.hpp"
+#include "mld_multi.hpp"
 
 struct mldhip_prec {
   mld::Prec host;
@@ -15,6 +17,7 @@ struct mldhip_prec {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::unique_ptr<mld::DevPrec> dev;
+  std::unique_ptr<mld::MultiEngine> blk;  // block (32-column) engine, built on first use
   std::string err;
 };
 
@@ -69,6 +72,7 @@ int mldhip_destroy(mldhip_prec* p) {
   if (!p) return 0;
   if (p->device >= 0) (void)hipSetDevice(p->device);
   p->dev.reset();
+  p->blk.reset();
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return 0;
@@ -77,6 +81,7 @@ int mldhip_destroy(mldhip_prec* p) {
 int mldhip_init(mldhip_prec* p, const char* ptype) {
   if (!p) return E_BAD_ARG;
   p->dev.reset();
+  p->blk.reset();
   int rc = mld::prec_init(p->host, S(ptype));
   if (rc) return fail(p, rc, p->host.last_error);
   return 0;
@@ -108,6 +113,7 @@ int mldhip_set_format(mldhip_prec* p, int fmt) {
 int mldhip_hierarchy_build(mldhip_prec* p, int n, const int64_t* irp, const int* ja, const double* val) {
   if (!p || n < 0 || !irp || (n > 0 && (!ja || !val))) return E_BAD_ARG;
   p->dev.reset();
+  p->blk.reset();
   int rc = mld::hierarchy_build(p->host, n, irp, ja, val);
   if (rc) return fail(p, rc, p->host.last_error.empty() ? "hierarchy_build failed" : p->host.last_error);
   return 0;
@@ -116,6 +122,7 @@ int mldhip_hierarchy_build(mldhip_prec* p, int n, const int64_t* irp, const int*
 int mldhip_smoothers_build(mldhip_prec* p) {
   if (!p) return E_BAD_ARG;
   p->dev.reset();
+  p->blk.reset();
   int rc = mld::smoothers_build(p->host);
   if (rc) return fail(p, rc, "smoothers_build failed");
   if (p->device < 0) return 0;
@@ -357,6 +364,154 @@ int mldhip_get_op_format(mldhip_prec* p, int lev, int what, int* fmt, int* lanes
   if (rc) return rc;
   if (!fmt || !lanes || !width) return E_BAD_ARG;
   return p->dev->op_format(lev, what, fmt, lanes, width);
+}
+
+// ---- batched engines --------------------------------------------------------
+static int ensure_block(mldhip_prec* p) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  if (p->blk) return 0;
+  std::vector<const mld::Prec*> sys{&p->host};
+  std::string e;
+  p->blk.reset(mld::make_multi(sys, p->host.value_kind, 32, p->device, rc, e));
+  if (!p->blk) return fail(p, rc ? rc : E_DEVICE, e);
+  return 0;
+}
+
+int mldhip_apply_block(mldhip_prec* p, int nrhs, const double* x, int64_t ldx, double* y, int64_t ldy) {
+  int rc = ensure_block(p);
+  if (rc) return rc;
+  if (!x || !y) return E_BAD_ARG;
+  return mld::dev_guard([&]() { return p->blk->apply_cm(nrhs, x, ldx, y, ldy, p->stream); }, p->err);
+}
+
+int mldhip_krylov_block(mldhip_prec* p, const char* method, int nrhs, const double* b, int64_t ldb, double* x,
+                        int64_t ldx, double eps, int itmax, int istop, int* iters, double* errs) {
+  int rc = ensure_block(p);
+  if (rc) return rc;
+  int mc = method_code(method);
+  if (mc < 0 || mc > 1) return fail(p, mld::E_UNSUPPORTED, "block Krylov method not provided: " + S(method));
+  if (istop != 2) return fail(p, mld::E_UNSUPPORTED, "istop must be 2");
+  if (!b || !x || !iters || !errs) return E_BAD_ARG;
+  return mld::dev_guard(
+      [&]() { return p->blk->krylov_cm(mc, nrhs, b, ldb, x, ldx, eps, itmax, iters, errs, nullptr, p->stream); },
+      p->err);
+}
+
+}  // extern "C"
+
+struct mldhip_group {
+  int device = 0;
+  int nb = 1;
+  hipStream_t stream = nullptr;
+  std::vector<mldhip_prec*> sys;
+  std::unique_ptr<mld::MultiEngine> eng;
+  std::string err;
+};
+
+static int gfail(mldhip_group* g, int code, const std::string& msg) {
+  if (g) g->err = msg;
+  return code;
+}
+
+extern "C" {
+
+int mldhip_group_create(mldhip_group** gg, int device, int nb) {
+  if (!gg || (nb != 1 && nb != 32) || device < 0) return E_BAD_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device >= ndev) return E_DEVICE;
+  auto* g = new mldhip_group;
+  g->device = device;
+  g->nb = nb;
+  *gg = g;
+  return 0;
+}
+
+int mldhip_group_destroy(mldhip_group* g) {
+  if (!g) return 0;
+  (void)hipSetDevice(g->device);
+  g->eng.reset();
+  delete g;
+  return 0;
+}
+
+int mldhip_group_add(mldhip_group* g, mldhip_prec* p) {
+  if (!g || !p) return E_BAD_ARG;
+  if (!p->host.smoothers_built) return gfail(g, E_NOT_BUILT, "system not built");
+  g->eng.reset();
+  g->sys.push_back(p);
+  return 0;
+}
+
+int mldhip_group_build(mldhip_group* g, int precision) {
+  if (!g || (precision != 0 && precision != 1)) return E_BAD_ARG;
+  std::vector<const mld::Prec*> sys;
+  for (auto* p : g->sys) sys.push_back(&p->host);
+  return mld::dev_guard(
+      [&]() -> int {
+        if (hipSetDevice(g->device) != hipSuccess) return E_DEVICE;
+        int rc = 0;
+        std::string e;
+        g->eng.reset(mld::make_multi(sys, precision, g->nb, g->device, rc, e));
+        if (!g->eng) return gfail(g, rc ? rc : E_DEVICE, e);
+        return 0;
+      },
+      g->err);
+}
+
+int mldhip_group_info(mldhip_group* g, int* nsys, int64_t* nrows, int64_t* offsets) {
+  if (!g) return E_BAD_ARG;
+  if (!g->eng) return gfail(g, E_NOT_BUILT, "group not built");
+  if (nsys) *nsys = g->eng->nsys();
+  if (nrows) *nrows = g->eng->rows();
+  if (offsets) {
+    const auto& o = g->eng->offsets();
+    for (size_t i = 0; i < o.size(); ++i) offsets[i] = o[i];
+  }
+  return 0;
+}
+
+int mldhip_group_set_stream(mldhip_group* g, void* stream) {
+  if (!g) return E_BAD_ARG;
+  g->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int mldhip_group_apply(mldhip_group* g, int nrhs, const double* x, int64_t ldx, double* y, int64_t ldy) {
+  if (!g || !x || !y) return E_BAD_ARG;
+  if (!g->eng) return gfail(g, E_NOT_BUILT, "group not built");
+  return mld::dev_guard([&]() { return g->eng->apply_cm(nrhs, x, ldx, y, ldy, g->stream); }, g->err);
+}
+
+int mldhip_group_krylov(mldhip_group* g, const char* method, int nrhs, const double* b, int64_t ldb, double* x,
+                        int64_t ldx, double eps, int itmax, int istop, int* iters, double* errs, int* launches) {
+  if (!g || !b || !x || !iters || !errs) return E_BAD_ARG;
+  if (!g->eng) return gfail(g, E_NOT_BUILT, "group not built");
+  int mc = method_code(method);
+  if (mc < 0 || mc > 1) return gfail(g, mld::E_UNSUPPORTED, "group Krylov method not provided: " + S(method));
+  if (istop != 2) return gfail(g, mld::E_UNSUPPORTED, "istop must be 2");
+  return mld::dev_guard(
+      [&]() { return g->eng->krylov_cm(mc, nrhs, b, ldb, x, ldx, eps, itmax, iters, errs, launches, g->stream); },
+      g->err);
+}
+
+int mldhip_group_time_apply(mldhip_group* g, int reps, double* avg_ms) {
+  if (!g || !avg_ms || reps <= 0) return E_BAD_ARG;
+  if (!g->eng) return gfail(g, E_NOT_BUILT, "group not built");
+  return mld::dev_guard([&]() { return g->eng->time_apply(reps, avg_ms); }, g->err);
+}
+
+int mldhip_group_level_info(mldhip_group* g, int lev, int64_t* info) {
+  if (!g || !info) return E_BAD_ARG;
+  if (!g->eng) return gfail(g, E_NOT_BUILT, "group not built");
+  return g->eng->level_info(lev, info);
+}
+
+int mldhip_group_get_error(mldhip_group* g, char* buf, int len) {
+  if (!g || !buf || len <= 0) return E_BAD_ARG;
+  std::strncpy(buf, g->err.c_str(), (size_t)len - 1);
+  buf[len - 1] = 0;
+  return 0;
 }
 
 }  // extern "C"

@@ -105,7 +105,8 @@ __global__ __launch_bounds__(256) void trsm_cols_kernel(int n, const double* __r
 // C[m x nc] -= A[m x kd] * B[kd x nc] (row-major, leading dims lda/ldb/ldc);
 // k beyond kd is zero-padded.  One wavefront per 16x16 tile of C, four tiles per
 // block; v_mfma_f64_16x16x4f64 with A[i][k] in lane i%16 + 16*k and
-// B[k][j] in lane j + 16*k; result row 4*(lane/16)+r, column lane%16.
+// B[k][j] in lane j + 16*k; result register r of a lane holds row
+// lane/16 + 4*r, column lane%16 (layout measured on gfx950).
 __global__ __launch_bounds__(256) void gemm_sub_mfma_kernel(int m, int nc, int kd, const double* __restrict__ A,
                                                              int lda, const double* __restrict__ B, int ldb,
                                                              double* C, int ldc) {
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(256) void gemm_sub_mfma_kernel(int m, int nc, int k
   const int col = c0 + (lane & 15);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int row = r0 + 4 * (lane >> 4) + r;
+    const int row = r0 + (lane >> 4) + 4 * r;
     if (row < m && col < nc) C[(size_t)row * ldc + col] = C[(size_t)row * ldc + col] - acc[r];
   }
 }

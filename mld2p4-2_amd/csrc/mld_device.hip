@@ -16,350 +16,12 @@
 
 #include "mld_device.hpp"
 #include "mld_dense.hpp"
+#include "mld_devcommon.hpp"
 #include "mld_kernels.hpp"
 
 namespace mld {
 
 using namespace mld::k;
-
-struct DevError : std::runtime_error {
-  explicit DevError(const std::string& s) : std::runtime_error(s) {}
-};
-
-#define HIPCHECK(x)                                                                                 \
-  do {                                                                                              \
-    hipError_t e_ = (x);                                                                            \
-    if (e_ != hipSuccess)                                                                           \
-      throw DevError(std::string(#x) + ": " + hipGetErrorString(e_) + " at " + __FILE__ + ":" +     \
-                     std::to_string(__LINE__));                                                     \
-  } while (0)
-
-enum { FMT_CSR = 0, FMT_ELL = 1, FMT_SELL = 2 };
-
-template <typename V>
-struct DMat {
-  int nr = 0, nc = 0;
-  int64_t nnz = 0;
-  int fmt = FMT_CSR, lanes = 1, w = 0, ld = 0;
-  int iters = 0;        // CSR: entries per lane loaded up front (0, 2, 4, 8)
-  int* irp = nullptr;   // CSR row pointers
-  int* sptr = nullptr;  // SELL slice offsets (nslices + 1)
-  int* perm = nullptr;  // SELL position -> row
-  int* ja = nullptr;
-  V* val = nullptr;
-};
-
-template <typename V>
-struct DSmoother {
-  int smoother = JAC, solver = DIAG_SCALE;
-  int64_t nnz_nd = 0;
-  DMat<V> nd, ndt;     // ND and its transpose (uploaded for 'T' applies)
-  V* dv = nullptr;
-  // ILU(0): factors and level schedules; for 'T': transposed factors
-  DMat<V> l, u, lt, ut;
-  int* lrows = nullptr; int* urows = nullptr; int* lptr_d = nullptr; int* uptr_d = nullptr;
-  std::vector<int> lptr, uptr;
-  int* ltrows = nullptr; int* utrows = nullptr; int* ltptr_d = nullptr; int* utptr_d = nullptr;
-  std::vector<int> ltptr, utptr;
-  bool has_t = false;
-  // dense LU
-  double* lu = nullptr; int* piv = nullptr;
-  double* dinv = nullptr;  // DENSE_LU: explicit inverse (row-major)
-  int ndense = 0;
-};
-
-template <typename V>
-struct DLevel {
-  int n = 0;
-  DMat<V> a, at, p, r;  // at: A^T for 'T' applies
-  DSmoother<V> sm, sm2;
-  bool has_sm2 = false;
-  int sweeps_pre = 1, sweeps_post = 1, ml_cycle = VCYCLE;
-  V *x2l = nullptr, *y2l = nullptr, *tx = nullptr, *ty = nullptr, *t1 = nullptr, *t2 = nullptr;
-  double* wlu = nullptr;
-};
-
-// ---------------------------------------------------------------------------
-// Allocation bookkeeping: every device allocation of a handle is freed with it.
-// ---------------------------------------------------------------------------
-struct DevArena {
-  std::vector<void*> ptrs;
-  size_t bytes = 0;
-  template <typename T>
-  T* alloc(size_t n) {
-    if (n == 0) n = 1;
-    void* p = nullptr;
-    HIPCHECK(hipMalloc(&p, n * sizeof(T)));
-    ptrs.push_back(p);
-    bytes += n * sizeof(T);
-    return (T*)p;
-  }
-  template <typename T>
-  T* upload(const T* h, size_t n, hipStream_t s) {
-    T* d = alloc<T>(n);
-    if (n) HIPCHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
-    return d;
-  }
-  ~DevArena() {
-    for (void* p : ptrs) (void)hipFree(p);
-  }
-};
-
-static inline int nblk(int64_t n) { return (int)((n + BS - 1) / BS); }
-
-// ---------------------------------------------------------------------------
-// Storage choice (force: 0 automatic, 1 CSR, 2 ELL, 3 SELL).  Automatic:
-// ELL when the widest row is short and padding small (the 7-point fine
-// level: 0.3 % padding); SELL for other matrices with enough rows to fill
-// slices; CSR with a power-of-two number of lanes per row (matched to the
-// mean row length) for small matrices.
-// ---------------------------------------------------------------------------
-struct FmtChoice { int fmt, lanes, w; };
-
-static FmtChoice choose_format(const HCsr& h, int force) {
-  int maxrow = 0;
-  for (int i = 0; i < h.nr; ++i) maxrow = std::max(maxrow, (int)(h.irp[i + 1] - h.irp[i]));
-  const double nnz = (double)h.nnz();
-  const double avg = h.nr ? nnz / h.nr : 0.0;
-  FmtChoice c{FMT_CSR, 1, maxrow};
-  int lanes = 1;
-  while (lanes < 64 && lanes * 4 <= avg) lanes *= 2;
-  c.lanes = lanes;
-  if (force == 1 || maxrow == 0) return c;
-  if (force == 2 && maxrow <= 64) { c.fmt = FMT_ELL; return c; }
-  if (force >= 3 && force <= 5) { c.fmt = FMT_SELL; return c; }
-  if (maxrow <= 16 && (double)h.nr * maxrow <= 1.10 * nnz) { c.fmt = FMT_ELL; return c; }
-  if (h.nr >= 4096) { c.fmt = FMT_SELL; return c; }
-  return c;
-}
-
-// Rows are sorted by decreasing length inside windows of sigma rows (stable,
-// so equal-length rows keep their order); slices of 64 positions.  Large
-// windows cut padding; small windows keep consecutive rows (and so the
-// columns they gather) together in a wavefront.  force 3: sigma 4096,
-// force 4: sigma 256, force 5: no sorting.
-static int sell_sigma(int force) { return force == 4 ? 256 : (force == 5 ? 1 : 4096); }
-
-template <typename V>
-static DMat<V> upload_mat(const HCsr& h, DevArena& ar, hipStream_t s, int force, int lanes_override = 0) {
-  DMat<V> m;
-  m.nr = h.nr; m.nc = h.nc; m.nnz = h.nnz();
-  FmtChoice c = choose_format(h, force);
-  if (lanes_override > 0) { c.fmt = FMT_CSR; c.lanes = lanes_override; }
-  m.fmt = c.fmt; m.lanes = c.lanes; m.w = c.w;
-  {
-    const int per_lane = (c.w + c.lanes - 1) / std::max(1, c.lanes);
-    m.iters = per_lane <= 2 ? 2 : (per_lane <= 4 ? 4 : 8);
-  }
-  if (c.fmt == FMT_ELL) {
-    m.ld = h.nr;
-    std::vector<int> ja((size_t)m.ld * m.w);
-    std::vector<V> val((size_t)m.ld * m.w);
-    for (int i = 0; i < h.nr; ++i) {
-      int k = 0;
-      for (int64_t q = h.irp[i]; q < h.irp[i + 1]; ++q, ++k) {
-        ja[(size_t)k * m.ld + i] = h.ja[q];
-        val[(size_t)k * m.ld + i] = (V)h.val[q];
-      }
-      for (; k < m.w; ++k) {  // padding: zero value on a valid column
-        ja[(size_t)k * m.ld + i] = std::min(i, h.nc - 1);
-        val[(size_t)k * m.ld + i] = (V)0;
-      }
-    }
-    m.ja = ar.upload(ja.data(), ja.size(), s);
-    m.val = ar.upload(val.data(), val.size(), s);
-    HIPCHECK(hipStreamSynchronize(s));
-  } else if (c.fmt == FMT_SELL) {
-    const int nr = h.nr;
-    const int nsl = (nr + SELL_C - 1) / SELL_C;
-    std::vector<int> perm(nr);
-    for (int i = 0; i < nr; ++i) perm[i] = i;
-    auto len = [&](int i) { return (int)(h.irp[i + 1] - h.irp[i]); };
-    const int sigma = sell_sigma(force);
-    for (int w0 = 0; w0 < nr && sigma > 1; w0 += sigma) {
-      const int w1 = std::min(nr, w0 + sigma);
-      std::stable_sort(perm.begin() + w0, perm.begin() + w1, [&](int a, int b) { return len(a) > len(b); });
-    }
-    std::vector<int> sptr(nsl + 1, 0);
-    for (int sl = 0; sl < nsl; ++sl) {
-      int wmax = 0;
-      for (int l = 0; l < SELL_C; ++l) {
-        const int p = sl * SELL_C + l;
-        if (p < nr) wmax = std::max(wmax, len(perm[p]));
-      }
-      const int64_t next = (int64_t)sptr[sl] + (int64_t)wmax * SELL_C;
-      if (next >= ((int64_t)1 << 31)) throw DevError("SELL storage exceeds 2^31 entries");
-      sptr[sl + 1] = (int)next;
-    }
-    const size_t tot = (size_t)sptr[nsl];
-    std::vector<int> ja(tot);
-    std::vector<V> val(tot);
-    for (int sl = 0; sl < nsl; ++sl) {
-      const int w = (sptr[sl + 1] - sptr[sl]) / SELL_C;
-      for (int l = 0; l < SELL_C; ++l) {
-        const int p = sl * SELL_C + l;
-        const int row = (p < nr) ? perm[p] : -1;
-        int k = 0;
-        if (row >= 0)
-          for (int64_t q = h.irp[row]; q < h.irp[row + 1]; ++q, ++k) {
-            ja[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = h.ja[q];
-            val[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = (V)h.val[q];
-          }
-        for (; k < w; ++k) {  // padding: zero value on a valid column
-          ja[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = (row >= 0) ? std::min(row, h.nc - 1) : 0;
-          val[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = (V)0;
-        }
-      }
-    }
-    m.sptr = ar.upload(sptr.data(), sptr.size(), s);
-    m.perm = ar.upload(perm.data(), perm.size(), s);
-    m.ja = ar.upload(ja.data(), ja.size(), s);
-    m.val = ar.upload(val.data(), val.size(), s);
-    HIPCHECK(hipStreamSynchronize(s));
-  } else {
-    std::vector<int> irp(h.nr + 1);
-    for (int i = 0; i <= h.nr; ++i) irp[i] = (int)h.irp[i];
-    std::vector<V> val(h.val.size());
-    for (size_t q = 0; q < val.size(); ++q) val[q] = (V)h.val[q];
-    m.irp = ar.upload(irp.data(), irp.size(), s);
-    m.ja = ar.upload(h.ja.data(), h.ja.size(), s);
-    m.val = ar.upload(val.data(), val.size(), s);
-    HIPCHECK(hipStreamSynchronize(s));
-  }
-  return m;
-}
-
-static HCsr host_transpose(const HCsr& a) {
-  HCsr t;
-  t.nr = a.nc; t.nc = a.nr;
-  t.irp.assign((size_t)t.nr + 1, 0);
-  for (int64_t k = 0; k < a.nnz(); ++k) t.irp[a.ja[k] + 1]++;
-  for (int i = 0; i < t.nr; ++i) t.irp[i + 1] += t.irp[i];
-  t.ja.resize(a.nnz()); t.val.resize(a.nnz());
-  std::vector<int64_t> pos(t.irp.begin(), t.irp.end() - 1);
-  for (int i = 0; i < a.nr; ++i)
-    for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
-      int64_t q = pos[a.ja[k]]++;
-      t.ja[q] = i; t.val[q] = a.val[k];
-    }
-  return t;
-}
-
-// Blocks of the grid of a sparse launch (row- or lane-per-thread).
-template <typename V>
-static int spmv_blocks(const DMat<V>& m) {
-  if (m.fmt == FMT_CSR) return nblk((int64_t)m.nr * m.lanes);
-  return nblk(m.nr);
-}
-
-template <int LANES, int K, typename V, class G, class E>
-static void csr_launch_l(const DMat<V>& m, int nb, G g, E e, hipStream_t s, Red red) {
-  switch (m.iters) {
-    case 2: csr_kernel<V, LANES, 2, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
-    case 4: csr_kernel<V, LANES, 4, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
-    case 8: csr_kernel<V, LANES, 8, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
-    default: csr_kernel<V, LANES, 0, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Generic sparse launch: dispatch on format / width / lanes.  K > 0: the
-// epilogue's dot contributions are reduced into red.result[0..K).
-// ---------------------------------------------------------------------------
-template <int K = 0, typename V, class G, class E>
-static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red = Red{nullptr, nullptr}) {
-  if (m.nr == 0) return;
-  const int nb = spmv_blocks(m);
-  if (m.fmt == FMT_ELL) {
-    switch (m.w) {
-      case 4: ell_kernel<V, 4, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      case 5: ell_kernel<V, 5, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      case 6: ell_kernel<V, 6, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      case 7: ell_kernel<V, 7, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      default: ell_kernel<V, 0, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-    }
-  } else if (m.fmt == FMT_SELL) {
-    sell_kernel<V, K><<<nb, BS, 0, s>>>(m.nr, m.sptr, m.perm, m.ja, m.val, g, e, red);
-  } else {
-    switch (m.lanes) {
-      case 1: csr_launch_l<1, K>(m, nb, g, e, s, red); break;
-      case 2: csr_launch_l<2, K>(m, nb, g, e, s, red); break;
-      case 4: csr_launch_l<4, K>(m, nb, g, e, s, red); break;
-      case 8: csr_launch_l<8, K>(m, nb, g, e, s, red); break;
-      case 16: csr_launch_l<16, K>(m, nb, g, e, s, red); break;
-      case 32: csr_launch_l<32, K>(m, nb, g, e, s, red); break;
-      default: csr_launch_l<64, K>(m, nb, g, e, s, red); break;
-    }
-  }
-  if constexpr (K > 0) finalize_kernel<K><<<1, BS, 0, s>>>(nb, red.partial, red.result);
-  HIPCHECK(hipGetLastError());
-}
-
-// ---------------------------------------------------------------------------
-// Autotuner: storage of each large operator picked by timing y = M x in every
-// candidate layout (ELL when eligible, SELL, CSR with 1..16 lanes per row up
-// to the mean row length) on the device.  Layouts that keep each row's stored
-// order (ELL, SELL, one-lane CSR) win ties within 3 %.  Disabled by
-// MLD_AUTOTUNE=0 (then the static rule of choose_format applies).
-// ---------------------------------------------------------------------------
-constexpr int TUNE_MIN_ROWS = 32768;
-
-static bool autotune_enabled() {
-  const char* e = std::getenv("MLD_AUTOTUNE");
-  return !(e && e[0] == '0');
-}
-
-template <typename V>
-static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int force) {
-  if (force != 0 || h.nr < TUNE_MIN_ROWS || !autotune_enabled()) return upload_mat<V>(h, ar, s, force);
-  int maxrow = 0;
-  for (int i = 0; i < h.nr; ++i) maxrow = std::max(maxrow, (int)(h.irp[i + 1] - h.irp[i]));
-  const double avg = (double)h.nnz() / std::max(1, h.nr);
-  // an almost padding-free ELL is not worth a search
-  if (maxrow <= 16 && (double)h.nr * maxrow <= 1.02 * (double)h.nnz()) return upload_mat<V>(h, ar, s, 2);
-  struct Cand { int force, lanes; bool ordered; };
-  std::vector<Cand> cands;
-  if (maxrow <= 64 && (double)h.nr * maxrow <= 1.25 * (double)h.nnz()) cands.push_back({2, 0, true});
-  cands.push_back({3, 0, true});
-  cands.push_back({4, 0, true});
-  cands.push_back({5, 0, true});
-  for (int l = 1; l <= 64; l *= 2)
-    if (l == 1 || l <= avg) cands.push_back({1, l, l == 1});
-  V* x = nullptr;
-  V* y = nullptr;
-  HIPCHECK(hipMalloc(&x, sizeof(V) * (size_t)std::max(1, h.nc)));
-  HIPCHECK(hipMalloc(&y, sizeof(V) * (size_t)std::max(1, h.nr)));
-  fill_kernel<V><<<nblk(h.nc), BS, 0, s>>>(h.nc, x, (V)1);
-  hipEvent_t e0, e1;
-  HIPCHECK(hipEventCreate(&e0));
-  HIPCHECK(hipEventCreate(&e1));
-  double best = 1e300;
-  int bi = 0;
-  std::vector<double> tm(cands.size());
-  for (size_t c = 0; c < cands.size(); ++c) {
-    DevArena tmp;
-    DMat<V> m = upload_mat<V>(h, tmp, s, cands[c].force, cands[c].lanes);
-    spmv_launch(m, GatherX<V, V>{x}, EpiSet<V, V>{y}, s);  // warm-up
-    float msmin = 1e30f;
-    const int reps = 5;
-    for (int trial = 0; trial < 3; ++trial) {  // best of three batches
-      HIPCHECK(hipEventRecord(e0, s));
-      for (int r = 0; r < reps; ++r) spmv_launch(m, GatherX<V, V>{x}, EpiSet<V, V>{y}, s);
-      HIPCHECK(hipEventRecord(e1, s));
-      HIPCHECK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-      msmin = std::min(msmin, ms);
-    }
-    tm[c] = msmin / reps * (cands[c].ordered ? 1.0 : 1.03);
-    if (tm[c] < best) { best = tm[c]; bi = (int)c; }
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(x);
-  (void)hipFree(y);
-  return upload_mat<V>(h, ar, s, cands[bi].force, cands[bi].lanes);
-}
 
 // ---------------------------------------------------------------------------
 // The device preconditioner for one value type.

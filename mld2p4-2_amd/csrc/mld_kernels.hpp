@@ -22,6 +22,9 @@
 
 namespace mld {
 namespace k {
+// Internal linkage: every translation unit that includes this header gets
+// its own copy of the kernels.
+namespace {
 
 constexpr int BS = 256;  // threads per block (4 wavefronts)
 constexpr int SELL_C = 64;
@@ -523,5 +526,6 @@ __global__ __launch_bounds__(1024) void dense_lu_small_kernel(int n, const doubl
   }
 }
 
+}  // namespace
 }  // namespace k
 }  // namespace mld

@@ -165,9 +165,10 @@ def test_apply_transpose_nonsymmetric(eng, orc):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("method", ["BICGSTAB", "FCG"])
-def test_krylov_adjoint_nonsymmetric(eng, method):
-    _say(f"[krylov adjoint] {method} idim=14")
+def test_krylov_adjoint_nonsymmetric(eng):
+    """BiCGStab on A^T lam = g with the hierarchy applied transposed, for a
+    nonsymmetric A (CG-type methods need a symmetric A)."""
+    _say("[krylov adjoint] BICGSTAB idim=14")
     A = convection_matrix(14)
     p = eng.MldPrec(device=0).init("ML")
     for k, v in [("SMOOTHER_TYPE", "L1-JACOBI"), ("SMOOTHER_SWEEPS", 2), ("AGGR_THRESH", 0.0),
@@ -176,13 +177,36 @@ def test_krylov_adjoint_nonsymmetric(eng, method):
         p.set(k, v)
     p.build(A)
     g = np.random.default_rng(5).standard_normal(A.shape[0])
-    lam, it, err = eng.psb_krylov_adjoint(method, p, dev(g), eps=1e-11, itmax=2000)
+    lam, it, err = eng.psb_krylov_adjoint("BICGSTAB", p, dev(g), eps=1e-11, itmax=2000)
     assert err <= 1e-11
     lr = lam.cpu().numpy()
     assert np.linalg.norm(A.T @ lr - g) <= 1.01e-11 * np.linalg.norm(g)
     # forward solve with the same handle still works afterwards
-    xf, itf, errf = eng.psb_krylov(method, A, p, dev(g), eps=1e-11, itmax=2000)
+    xf, itf, errf = eng.psb_krylov("BICGSTAB", A, p, dev(g), eps=1e-11, itmax=2000)
     assert np.linalg.norm(A @ xf.cpu().numpy() - g) <= 1.01e-11 * np.linalg.norm(g)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["CG", "FCG"])
+def test_krylov_adjoint_symmetric(eng, method):
+    """Symmetric diffusion operator: the adjoint solve (transposed hierarchy)
+    solves A^T lam = g = A lam to the tolerance."""
+    _say(f"[krylov adjoint] {method} diffusion idim=16")
+    from mld2p4_amd import problems
+
+    th, b, _ = problems.theta_problem(16, seed=4)
+    A = fv7_sp(th)
+    p = eng.MldPrec(device=0).init("ML")
+    for k, v in [("SMOOTHER_TYPE", "L1-JACOBI"), ("SMOOTHER_SWEEPS", 2), ("AGGR_THRESH", 0.0),
+                 ("MIN_COARSE_SIZE", 200), ("COARSE_SOLVE", "BJAC"), ("COARSE_SUBSOLVE", "ILU"),
+                 ("COARSE_SWEEPS", 4)]:
+        p.set(k, v)
+    p.build(A)
+    lam, it, err = eng.psb_krylov_adjoint(method, p, dev(b), eps=1e-10)
+    assert err <= 1e-10
+    assert np.linalg.norm(A.T @ lam.cpu().numpy() - b) <= 1.01e-10 * np.linalg.norm(b)
+    _, itf, _ = eng.psb_krylov(method, A, p, dev(b), eps=1e-10)
+    assert abs(it - itf) <= 1
 
 
 def _grad_setup(idim):
