@@ -11,12 +11,25 @@ Outer CG (istop 2) to ||r||/||b|| <= 1e-8.
 
 A step is one complete solve from x0 = 0 (the hierarchy is built and
 uploaded once before timing; inputs resident in HBM).  value = mean seconds
-per solve over the timed steps.  The FP32-mixed form (FP32 hierarchy inside
-FP64 FCG) of the same problem is reported beside it.
+per solve over the timed steps.
+
+Reported beside it, in the same JSON line:
+  mixed      : FP32 hierarchy inside FP64 FCG, same problem and hierarchy;
+  run_b      : block-Jacobi ILU(0), 1 sweep, level-scheduled solves;
+  diffusion  : 256^3 heterogeneous diffusion, FP32-mixed FCG with a dense
+               FP64 LU (MFMA) coarsest level vs the FP64 path, error vs a
+               FP64 reference solution;
+  batched    : 32 right-hand sides at idim=192 through the block V-cycle,
+               and 64 independent systems (idim uniform in [24, 96]) in one
+               graph-captured solve, each against one-at-a-time solves;
+  adjoint    : dL/dtheta at idim=128 (forward + adjoint solve + the
+               edge-product gradient kernel);
+  cpu_baseline: the oracle (sequential CPU restatement) on a bounded sample.
 """
 from __future__ import annotations
 
 import argparse
+import gc
 import importlib.util
 import json
 import os
@@ -45,23 +58,27 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_prec(eng, A, smoother, sweeps, precision):
+BASE = [("AGGR_THRESH", 0.0), ("AGGR_PROL", "SMOOTHED")]
+
+
+def make_prec(eng, smoother="L1-JACOBI", sweeps=2, precision="double", mcs=200, coarse="BJAC"):
     p = eng.MldPrec(device=0).init("ML")
     p.set("SMOOTHER_TYPE", smoother)
     p.set("SMOOTHER_SWEEPS", sweeps)
-    p.set("AGGR_THRESH", 0.0)
-    p.set("AGGR_PROL", "SMOOTHED")
-    p.set("MIN_COARSE_SIZE", 200)
-    p.set("COARSE_SOLVE", "BJAC")
-    p.set("COARSE_SUBSOLVE", "ILU")
-    p.set("COARSE_SWEEPS", 4)
+    for k, v in BASE:
+        p.set(k, v)
+    p.set("MIN_COARSE_SIZE", mcs)
+    p.set("COARSE_SOLVE", coarse)
+    if coarse == "BJAC":
+        p.set("COARSE_SUBSOLVE", "ILU")
+        p.set("COARSE_SWEEPS", 4)
+    else:
+        p.set("COARSE_SWEEPS", 1)
     p.set_precision(precision)
     return p
 
 
-def level_table(eng, p):
-    from importlib import import_module
-
+def level_table(p):
     levels = []
     for l in range(p.nlev):
         i = p.level_info(l)
@@ -77,18 +94,18 @@ def level_table(eng, p):
     return levels
 
 
-def solve_timed(eng, torch, A, p, method, b, eps, steps, warmup):
+def solve_timed(eng, torch, p, method, b, eps, steps, warmup, itmax=1000):
     x = torch.zeros_like(b)
     for _ in range(warmup):
         x.zero_()
-        eng.psb_krylov(method, None, p, b, x, eps=eps, itmax=1000)
+        eng.psb_krylov(method, None, p, b, x, eps=eps, itmax=itmax)
     torch.cuda.synchronize()
     times, its, errs = [], [], []
     for _ in range(steps):
         x.zero_()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        _, it, err = eng.psb_krylov(method, None, p, b, x, eps=eps, itmax=1000)
+        _, it, err = eng.psb_krylov(method, None, p, b, x, eps=eps, itmax=itmax)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         its.append(it)
@@ -109,6 +126,12 @@ def time_vcycle(torch, p, n, reps=20):
     e1.record(s)
     e1.synchronize()
     return e0.elapsed_time(e1) / reps / 1e3
+
+
+def release(torch):
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
 
 
 def cpu_baseline(idim, gpu_iters, sample_iters):
@@ -138,6 +161,237 @@ def cpu_baseline(idim, gpu_iters, sample_iters):
     }
 
 
+# ----------------------------------------------------------------- sections
+def section_run_b(eng, torch, problems, irp, ja, val, n, steps):
+    """Block-Jacobi ILU(0), one sweep, on the same Poisson problem."""
+    p = make_prec(eng, "BJAC", 1)
+    p.set("SUB_SOLVE", "ILU")
+    t0 = time.perf_counter()
+    p.build((irp, ja, val))
+    t_b = time.perf_counter() - t0
+    b = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    _, tm, it, er = solve_timed(eng, torch, p, "CG", b, 1e-8, steps, 1)
+    t_vc = time_vcycle(torch, p, n, reps=5)
+    out = {"smoother": "BJAC/ILU(0) 1 sweep", "solve_s": round(float(np.mean(tm)), 6), "iters": int(it[-1]),
+           "relres": er[-1], "vcycle_ms": round(t_vc * 1e3, 3), "setup_s": round(t_b, 2)}
+    del p
+    return out
+
+
+def section_diffusion(eng, torch, problems, idim, steps):
+    """Heterogeneous diffusion: FP64 CG vs FP32-mixed FCG, dense LU coarse."""
+    t0 = time.perf_counter()
+    irp, ja, val, bh = problems.diffusion3d(idim, seed=2024)
+    t_gen = time.perf_counter() - t0
+    n = len(irp) - 1
+    b = torch.from_numpy(bh).to("cuda:0")
+    p = make_prec(eng, "L1-JACOBI", 2, "double", mcs=4096, coarse="DENSE_LU")
+    t0 = time.perf_counter()
+    p.hierarchy_build((irp, ja, val))
+    t_h = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    p.smoothers_build()
+    t_s = time.perf_counter() - t0
+    nc = p.level_info(p.nlev - 1)["n"]
+    log(f"diffusion: nlev={p.nlev} coarsest n={nc} hierarchy {t_h:.1f}s smoothers+upload {t_s:.1f}s")
+    xd, td, itd, erd = solve_timed(eng, torch, p, "CG", b, 1e-8, steps, 1)
+    # FP64 reference solution (same FP64 path, tight tolerance)
+    xr = torch.zeros_like(b)
+    _, itr, err_r = eng.psb_krylov("CG", None, p, b, xr, eps=1e-12, itmax=2000)
+    xref = xr.cpu().numpy()
+    e64 = float(np.linalg.norm(xd.cpu().numpy() - xref) / np.linalg.norm(xref))
+    p.set_precision("mixed")
+    t0 = time.perf_counter()
+    p.smoothers_build()
+    t_sm = time.perf_counter() - t0
+    xm, tmx, itm, erm = solve_timed(eng, torch, p, "FCG", b, 1e-8, steps, 1)
+    em = float(np.linalg.norm(xm.cpu().numpy() - xref) / np.linalg.norm(xref))
+    rtrue = float(np.linalg.norm(bh - problems.csr_matvec(irp, ja, val, xm.cpu().numpy())) / np.linalg.norm(bh))
+    out = {"workload": f"3-D FV diffusion idim={idim}, k=exp(2 g), g ~ N(0,1) (Philox seed 2024) smoothed by 3 "
+                       "box-filter passes (edge replication), harmonic faces, b ~ U[-1,1]; L1-JACOBI 2/2, "
+                       f"MIN_COARSE_SIZE 4096, coarsest dense FP64 LU (n={nc}, MFMA)",
+           "fp64_cg": {"solve_s": round(float(np.mean(td)), 6), "iters": int(itd[-1]), "relres": erd[-1],
+                       "rel_err_vs_ref": e64},
+           "mixed_fcg": {"solve_s": round(float(np.mean(tmx)), 6), "iters": int(itm[-1]), "relres": erm[-1],
+                         "true_relres_host": rtrue, "rel_err_vs_ref": em},
+           "speedup_mixed_vs_fp64": round(float(np.mean(td)) / float(np.mean(tmx)), 3),
+           "reference_solution": f"FP64 GPU CG to 1e-12 ({itr} iterations, relres {err_r:.2e})",
+           "setup_s": {"generate": round(t_gen, 1), "hierarchy_build": round(t_h, 2),
+                       "smoothers_build_upload_fp64": round(t_s, 2), "smoothers_build_upload_fp32": round(t_sm, 2)}}
+    del p
+    return out
+
+
+def section_block(eng, torch, problems, perfmodel, idim, nrhs, steps):
+    """32 seeded Gaussian right-hand sides through the block V-cycle (FP32
+    preconditioner, FP64 CG per column) vs one-at-a-time solves."""
+    irp, ja, val = problems.pde3d_poisson(idim)
+    n = len(irp) - 1
+    p = make_prec(eng, "L1-JACOBI", 2, "mixed")
+    p.build((irp, ja, val))
+    rng = np.random.Generator(np.random.Philox(192))
+    B = torch.from_numpy(rng.standard_normal((nrhs, n))).to("cuda:0")
+    X = torch.zeros_like(B)
+    p.krylov_block("CG", B, X, eps=1e-8)  # warm-up (uploads the block engine, captures nothing)
+    tms = []
+    for _ in range(steps):
+        X.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, it, er = p.krylov_block("CG", B, X, eps=1e-8)
+        torch.cuda.synchronize()
+        tms.append(time.perf_counter() - t0)
+    t_block = float(np.mean(tms))
+    # one-at-a-time: the single-vector mixed CG on two of the columns
+    singles = []
+    its1 = []
+    for j in (0, 1):
+        xs = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+        eng.psb_krylov("CG", None, p, B[j].contiguous(), xs, eps=1e-8)
+        torch.cuda.synchronize()
+        xs.zero_()
+        t0 = time.perf_counter()
+        _, i1, _ = eng.psb_krylov("CG", None, p, B[j].contiguous(), xs, eps=1e-8)
+        torch.cuda.synchronize()
+        singles.append(time.perf_counter() - t0)
+        its1.append(i1)
+    t_single = float(np.mean(singles)) * nrhs
+    Xh = X.cpu().numpy()
+    Bh = B.cpu().numpy()
+    worst = max(float(np.linalg.norm(Bh[j] - problems.csr_matvec(irp, ja, val, Xh[j])) / np.linalg.norm(Bh[j]))
+                for j in range(0, nrhs, 8))
+    # block V-cycle throughput
+    Xa = torch.ones_like(B)
+    p.apply_block(Xa)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s = torch.cuda.current_stream()
+    e0.record(s)
+    for _ in range(5):
+        p.apply_block(Xa)
+    e1.record(s)
+    e1.synchronize()
+    t_bvc = e0.elapsed_time(e1) / 5 / 1e3
+    levels = level_table(p)
+    vb = perfmodel.block_vcycle_bytes(levels, 4, 4, nrhs, levels[-1]["n"])
+    out = {"workload": f"3-D Poisson idim={idim} (n={n}), {nrhs} RHS ~ N(0,1) (Philox seed 192), FP32 preconditioner "
+                       "(L1-JACOBI 2/2, coarsest ILU(0) as explicit operator), FP64 CG per column to 1e-8",
+           "block_solve_s": round(t_block, 5), "iters_max": int(it.max()), "iters_min": int(it.min()),
+           "relres_max": float(er.max()), "true_relres_max_sampled": worst,
+           "one_at_a_time_s": round(t_single, 5), "one_at_a_time_iters": its1,
+           "speedup_vs_one_at_a_time": round(t_single / t_block, 3),
+           "block_vcycle_ms": round(t_bvc * 1e3, 3), "block_vcycle_GBps": round(vb / t_bvc / 1e9, 1)}
+    del p
+    return out
+
+
+def section_group(eng, torch, problems, nsys, lo, hi, steps):
+    """64 independent Poisson systems, idim uniform in [lo, hi], one RHS each,
+    solved together (one union V-cycle, iterations in one hipGraph) vs one
+    system at a time."""
+    rng = np.random.Generator(np.random.Philox(64))
+    idims = rng.integers(lo, hi + 1, size=nsys)
+    systems = []
+    t0 = time.perf_counter()
+    for d in idims:
+        p = make_prec(eng, "L1-JACOBI", 2, "mixed")
+        p.build(problems.pde3d_poisson(int(d)))
+        systems.append(p)
+    t_b = time.perf_counter() - t0
+    g = eng.MldGroup(device=0, nb=1)
+    for p in systems:
+        g.add(p)
+    t0 = time.perf_counter()
+    g.build("mixed")
+    t_g = time.perf_counter() - t0
+    ns, nr, off = g.info()
+    bh = rng.standard_normal(nr)
+    b = torch.from_numpy(bh).to("cuda:0")
+    x, it, er, nl = g.krylov("FCG", b, eps=1e-8)  # warm-up + graph capture
+    tms = []
+    for _ in range(steps):
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, it, er, nl = g.krylov("FCG", b, x, eps=1e-8)
+        torch.cuda.synchronize()
+        tms.append(time.perf_counter() - t0)
+    t_group = float(np.mean(tms))
+    # one system at a time (same handles, single-system device path)
+    t_seq = 0.0
+    it_seq = []
+    for s, p in enumerate(systems):
+        bs = b[off[s]:off[s + 1]].contiguous()
+        xs = torch.zeros_like(bs)
+        eng.psb_krylov("FCG", None, p, bs, xs, eps=1e-8)
+        xs.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, i1, _ = eng.psb_krylov("FCG", None, p, bs, xs, eps=1e-8)
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t0
+        it_seq.append(i1)
+    t_apply = g.time_apply(5)
+    out = {"workload": f"{nsys} 3-D Poisson systems, idim uniform in [{lo},{hi}] (Philox seed 64; total n={nr}), "
+                       "RHS ~ N(0,1), FP32 preconditioners (L1-JACOBI 2/2, coarsest ILU(0) explicit), FP64 FCG per "
+                       "system to 1e-8",
+           "idims": [int(d) for d in idims], "nlev_range": [min(p.nlev for p in systems), max(p.nlev for p in systems)],
+           "group_solve_s": round(t_group, 5), "graph_launches": nl, "iters_max": int(it.max()),
+           "relres_max": float(er.max()), "one_at_a_time_s": round(t_seq, 5), "one_at_a_time_iters_max": max(it_seq),
+           "speedup_vs_one_at_a_time": round(t_seq / t_group, 3), "group_vcycle_ms": round(t_apply, 3),
+           "setup_s": {"hierarchies": round(t_b, 2), "group_union_upload": round(t_g, 2)}}
+    del g, systems
+    return out
+
+
+def section_adjoint(eng, torch, problems, perfmodel, idim):
+    """dL/dtheta at idim: forward mixed CG to 1e-10, adjoint solve with the
+    same hierarchy, edge-product gradient kernel."""
+    from mld2p4_amd import autograd as ag
+
+    th, bh, ths = problems.theta_problem(idim, seed=5)
+    opts = ag.SolveOptions(method="CG", eps=1e-10, precision="mixed")
+    # target x* at the second field (same solver)
+    with torch.no_grad():
+        xs = ag.solve(torch.from_numpy(ths).to("cuda:0"), torch.from_numpy(bh).to("cuda:0"), idim, opts)
+    t = torch.from_numpy(th).to("cuda:0").requires_grad_(True)
+    b = torch.from_numpy(bh).to("cuda:0").requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x = ag.solve(t, b, idim, opts)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    loss = 0.5 * torch.sum((x - xs) ** 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    last = dict(opts.last)
+    # gradient kernel alone, HIP events on torch's stream (the kernel's stream)
+    lam = torch.randn_like(x)
+    gbuf = torch.empty_like(x)
+    th_d = t.detach()
+    eng.fv7_grad_theta(idim, th_d, x.detach(), lam, gbuf)
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(20):
+        eng.fv7_grad_theta(idim, th_d, x.detach(), lam, gbuf)
+    e1.record(s)
+    e1.synchronize()
+    t_k = e0.elapsed_time(e1) / 20 / 1e3
+    by = perfmodel.fv7_grad_bytes(idim ** 3)
+    return {"workload": f"3-D FV diffusion idim={idim}, theta ~ N(0, 0.5^2) (Philox seed 5), b ~ U[-1,1], x* at "
+                        "theta* (seed 6); L = ||x - x*||^2/2; FP32-mixed AMG-CG to 1e-10 forward, adjoint CG with the "
+                        "same hierarchy transposed",
+            "forward_s": round(t_fwd, 4), "backward_s": round(t_bwd, 4), "solves": last,
+            "loss": float(loss.detach()), "grad_theta_norm": float(t.grad.norm()),
+            "grad_kernel": {"avg_ms": round(t_k * 1e3, 4), "algorithmic_bytes": by,
+                            "GBps": round(by / t_k / 1e9, 1), "frac_of_hbm_peak": round(by / t_k / 1e9 / HBM_PEAK_GBS, 4)},
+            "note": "setup (host assembly + hierarchy build + upload) is inside forward_s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +403,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=2)
+    ap.add_argument("--sections", default="run_b,diffusion,batched,adjoint",
+                    help="extra sections to run (comma list; empty for none)")
+    ap.add_argument("--small", action="store_true", help="small sizes for the extra sections (quick check)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box here has one GPU). "
@@ -161,6 +418,7 @@ def main():
     from mld2p4_amd import problems, perfmodel  # noqa: E402  (package registered by _engine)
 
     idim = args.idim
+    sections = [s for s in args.sections.split(",") if s]
     log(f"generating 3-D Poisson idim={idim}")
     t0 = time.perf_counter()
     irp, ja, val = problems.pde3d_poisson(idim)
@@ -169,7 +427,7 @@ def main():
 
     # ---------------- FP64 path
     t0 = time.perf_counter()
-    p = make_prec(eng, None, args.smoother, args.sweeps, "double")
+    p = make_prec(eng, args.smoother, args.sweeps, "double")
     p.hierarchy_build((irp, ja, val))
     t_h = time.perf_counter() - t0
     t0 = time.perf_counter()
@@ -177,7 +435,8 @@ def main():
     t_s = time.perf_counter() - t0
     log(f"hierarchy_build {t_h:.1f}s smoothers_build+upload {t_s:.1f}s nlev={p.nlev} "
         f"complexity={p.complexity():.3f}")
-    levels = level_table(eng, p)
+    levels = level_table(p)
+
     def fmt_str(l, what):
         f = p.op_format(l, what)
         return f["fmt"] + (f"x{f['lanes']}" if f["fmt"] == "CSR" else "")
@@ -188,11 +447,10 @@ def main():
         log(f"  level {l}: n={L['n']} nnz={L['nnz_a']} nnzP={L['nnz_p']} {ops}")
     b = torch.ones(n, dtype=torch.float64, device="cuda:0")
     log("FP64 CG solves")
-    x, times, its, errs = solve_timed(eng, torch, None, p, "CG", b, 1e-8, args.steps, args.warmup)
+    x, times, its, errs = solve_timed(eng, torch, p, "CG", b, 1e-8, args.steps, args.warmup)
     t_solve = float(np.mean(times))
     iters = int(its[-1])
     log(f"FP64 CG: {t_solve * 1e3:.2f} ms/solve, iters={its}, relres={errs[-1]:.3e}")
-    # true residual on the host (FP64)
     xh = x.cpu().numpy()
     rtrue = np.linalg.norm(1.0 - problems.csr_matvec(irp, ja, val, xh)) / np.sqrt(n)
 
@@ -232,19 +490,19 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "kernel": f"fine-level {dom} (level 1 of {p.nlev})", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "algorithmic_bytes": int(by), "avg_ms": round(ms, 5)}
+                "traffic": traffic, "algorithmic_bytes": int(by), "avg_ms": round(ms, 5),
+                "steps_GBps": {k: round(v[1] / v[0] / 1e6, 1) for k, v in cand.items()}}
 
     # ---------------- FP32-mixed path (FP32 hierarchy inside FP64 FCG)
     mixed = None
     if not args.no_mixed:
         # same host hierarchy, uploaded again with FP32 values
-        pm = p
-        pm.set_precision("mixed")
-        pm.smoothers_build()
+        p.set_precision("mixed")
+        p.smoothers_build()
         torch.cuda.synchronize()
         log("mixed FCG solves")
-        xm, tm, itm, erm = solve_timed(eng, torch, None, pm, "FCG", b, 1e-8, args.steps, args.warmup)
-        t_vcm = time_vcycle(torch, pm, n)
+        xm, tm, itm, erm = solve_timed(eng, torch, p, "FCG", b, 1e-8, args.steps, args.warmup)
+        t_vcm = time_vcycle(torch, p, n)
         vbm, _ = perfmodel.vcycle_bytes(levels, 4, 4)
         xd = xm.cpu().numpy()
         mixed = {"solve_s": round(float(np.mean(tm)), 6), "iters": int(itm[-1]), "relres": erm[-1],
@@ -252,7 +510,39 @@ def main():
                  "rel_err_vs_fp64_solution": float(np.linalg.norm(xd - xh) / np.linalg.norm(xh)),
                  "vcycle_ms": round(t_vcm * 1e3, 4), "vcycle_GBps": round(vbm / t_vcm / 1e9, 1)}
         log(f"mixed FCG: {mixed}")
+    cplx = p_cplx(levels)
     del p
+    release(torch)
+
+    extra = {}
+    small = args.small
+    if "run_b" in sections:
+        log("run B: BJAC/ILU(0) smoother")
+        extra["run_b"] = section_run_b(eng, torch, problems, irp, ja, val, n, min(args.steps, 2))
+        log(f"run B: {extra['run_b']}")
+        release(torch)
+    del irp, ja, val
+    if "diffusion" in sections:
+        log("diffusion: FP64 vs mixed")
+        extra["diffusion"] = section_diffusion(eng, torch, problems, 48 if small else idim, min(args.steps, 3))
+        log(f"diffusion: {extra['diffusion']}")
+        release(torch)
+    if "batched" in sections:
+        log("batched: 32 RHS block V-cycle")
+        extra["batched_block"] = section_block(eng, torch, problems, perfmodel, 40 if small else 192, 32,
+                                               min(args.steps, 3))
+        log(f"batched block: {extra['batched_block']}")
+        release(torch)
+        log("batched: 64 grouped systems")
+        extra["batched_group"] = section_group(eng, torch, problems, 64, 8 if small else 24, 24 if small else 96,
+                                               min(args.steps, 3))
+        log(f"batched group: {extra['batched_group']}")
+        release(torch)
+    if "adjoint" in sections:
+        log("adjoint: dL/dtheta")
+        extra["adjoint"] = section_adjoint(eng, torch, problems, perfmodel, 32 if small else 128)
+        log(f"adjoint: {extra['adjoint']}")
+        release(torch)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -273,10 +563,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"3-D Poisson 7-point idim={idim} (n={n}, nnz={len(ja)}), FP64 values, int32 indices, "
+        "config": {"workload": f"3-D Poisson 7-point idim={idim} (n={n}), FP64 values, int32 indices, "
                                f"b=1, x0=0; SA-AMG V-cycle {args.sweeps}/{args.sweeps} {args.smoother} sweeps, "
                                "coarsest BJAC/ILU(0), CG to 1e-8",
-                   "idim": idim, "nlev": len(levels), "operator_complexity": round(p_cplx(levels), 4)},
+                   "idim": idim, "nlev": len(levels), "operator_complexity": round(cplx, 4)},
         "iterations": iters,
         "relres": errs[-1],
         "true_relres_host": float(rtrue / 1.0),
@@ -286,6 +576,7 @@ def main():
         "setup_s": {"hierarchy_build": round(t_h, 2), "smoothers_build_upload": round(t_s, 2)},
         "roofline": roofline,
         "mixed": mixed,
+        **extra,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -12,27 +12,28 @@ from __future__ import annotations
 DIAG_SCALE, ILU_N, L1_DIAG, DENSE_LU, F_NONE = 5, 7, 101, 102, 4
 
 
-def spmv_bytes(nr, nc, nnz, sv, sx, beta_read=False, csr=True):
-    b = nnz * (sv + 4) + nc * sx + nr * sx * (2 if beta_read else 1)
+def spmv_bytes(nr, nc, nnz, sv, sx, beta_read=False, csr=True, nb=1):
+    """y = A x (beta_read: y = y + A x); nb columns share one pass over A."""
+    b = nnz * (sv + 4) + nb * (nc * sx + nr * sx * (2 if beta_read else 1))
     if csr:
         b += (nr + 1) * 4
     return b
 
 
-def residual_bytes(n, nnz, sv, sx, csr=False):
+def residual_bytes(n, nnz, sv, sx, csr=False, nb=1):
     # r = b - A y : A, y (gathered, once), b, r
-    return nnz * (sv + 4) + 3 * n * sx + ((n + 1) * 4 if csr else 0)
+    return nnz * (sv + 4) + 3 * n * sx * nb + ((n + 1) * 4 if csr else 0)
 
 
-def smoother_bytes(n, solver, sweeps, nnz_nd, nnz_lu, sv, sx, beta, nd_csr=False):
-    """One sm%apply(1, x, beta, y, sweeps, init='Z')."""
-    yread = n * sx if beta else 0
+def smoother_bytes(n, solver, sweeps, nnz_nd, nnz_lu, sv, sx, beta, nd_csr=False, nb=1):
+    """One sm%apply(1, x, beta, y, sweeps, init='Z') on nb columns."""
+    yread = n * sx * nb if beta else 0
     if solver in (DIAG_SCALE, L1_DIAG):
         if sweeps <= 1 or nnz_nd == 0:
-            return 2 * n * sx + n * sv + yread  # y = d*x
+            return 2 * n * sx * nb + n * sv + yread  # y = d*x
         rp = (n + 1) * 4 if nd_csr else 0
-        first = nnz_nd * (sv + 4) + rp + n * (2 * sx + sv)  # D^-1(x - ND D^-1 x)
-        more = (sweeps - 2) * (nnz_nd * (sv + 4) + rp + n * (3 * sx + sv))
+        first = nnz_nd * (sv + 4) + rp + n * (2 * sx * nb + sv)  # D^-1(x - ND D^-1 x)
+        more = (sweeps - 2) * (nnz_nd * (sv + 4) + rp + n * (3 * sx * nb + sv))
         return first + more + yread
     if solver == ILU_N:
         return nnz_lu * (sv + 4) + 2 * (n + 1) * 4 + n * sv + 4 * n * sx + yread
@@ -64,3 +65,29 @@ def vcycle_bytes(levels, sv, sx):
             steps.append((l, "coarse", smoother_bytes(n, L["solver"], L["sweeps_pre"], L["nnz_nd"],
                                                       L.get("nnz_lu", 0), sv, sx, 0)))
     return sum(s[2] for s in steps), steps
+
+
+def block_vcycle_bytes(levels, sv, sx, nb, coarse_n):
+    """Block V-cycle on nb columns (every operator CSR, read once per
+    product for all columns); the coarsest solve is a dense coarse_n^2
+    operator."""
+    tot = 0
+    nlev = len(levels)
+    for l, L in enumerate(levels):
+        n = L["n"]
+        if l < nlev - 1:
+            C = levels[l + 1]
+            nc = C["n"]
+            tot += smoother_bytes(n, L["solver"], L["sweeps_pre"], L["nnz_nd"], 0, sv, sx, 0, True, nb)
+            tot += 2 * residual_bytes(n, L["nnz_a"], sv, sx, True, nb)
+            tot += spmv_bytes(nc, n, C["nnz_r"], sv, sx, nb=nb)
+            tot += spmv_bytes(n, nc, C["nnz_p"], sv, sx, beta_read=True, nb=nb)
+            tot += smoother_bytes(n, L["solver"], L["sweeps_post"], L["nnz_nd"], 0, sv, sx, 1, True, nb)
+        else:
+            tot += spmv_bytes(coarse_n, coarse_n, coarse_n * coarse_n, sv, sx, nb=nb)
+    return tot
+
+
+def fv7_grad_bytes(ncell):
+    """theta, x, lambda read and the gradient written, 8 bytes each."""
+    return 32 * ncell
