@@ -86,8 +86,13 @@ def level_table(p):
                  solver=i["solver"], sweeps_pre=i["sweeps_pre"], sweeps_post=i["sweeps_post"])
         fa = p.op_format(l, "A")
         d["csr_a"] = fa["fmt"] == "CSR"
+        d["fmt_a"] = fa["fmt"]
+        if l > 0:
+            d["fmt_p"] = p.op_format(l, "P")["fmt"]
+            d["fmt_r"] = p.op_format(l, "R")["fmt"]
         if i["nnz_nd"] > 0:
             d["csr_nd"] = p.op_format(l, "ND")["fmt"] == "CSR"
+            d["fmt_nd"] = p.op_format(l, "ND")["fmt"]
         if i["solver"] == 7:
             d["nnz_lu"] = p.level_csr(l, "L").nnz + p.level_csr(l, "U").nnz
         levels.append(d)
@@ -464,18 +469,19 @@ def main():
     cand = {}
     reps = 20
     cand["residual"] = (p.time_step(0, "residual", reps),
-                        perfmodel.residual_bytes(n, L0["nnz_a"], 8, 8, L0["csr_a"]), 2)
+                        perfmodel.residual_bytes(n, L0["nnz_a"], 8, 8, fmt=L0["fmt_a"]), 2)
     cand["pre"] = (p.time_step(0, "pre", reps),
                    perfmodel.smoother_bytes(n, L0["solver"], L0["sweeps_pre"], L0["nnz_nd"], 0, 8, 8, 0,
-                                            L0.get("csr_nd", False)), 1)
+                                            nd_fmt=L0.get("fmt_nd")), 1)
     cand["post"] = (p.time_step(0, "post", reps),
                     perfmodel.smoother_bytes(n, L0["solver"], L0["sweeps_post"], L0["nnz_nd"], 0, 8, 8, 1,
-                                             L0.get("csr_nd", False)), 1)
-    cand["restrict"] = (p.time_step(1, "restrict", reps), perfmodel.spmv_bytes(L1["n"], n, L1["nnz_r"], 8, 8), 1)
+                                             nd_fmt=L0.get("fmt_nd")), 1)
+    cand["restrict"] = (p.time_step(1, "restrict", reps),
+                        perfmodel.spmv_bytes(L1["n"], n, L1["nnz_r"], 8, 8, fmt=L1["fmt_r"]), 1)
     cand["prolong"] = (p.time_step(1, "prolong", reps),
-                       perfmodel.spmv_bytes(n, L1["n"], L1["nnz_p"], 8, 8, beta_read=True), 1)
+                       perfmodel.spmv_bytes(n, L1["n"], L1["nnz_p"], 8, 8, beta_read=True, fmt=L1["fmt_p"]), 1)
     cand["krylov_spmv"] = (p.time_step(0, "krylov_spmv", reps),
-                           perfmodel.spmv_bytes(n, n, L0["nnz_a"], 8, 8, csr=L0["csr_a"]), 1)
+                           perfmodel.spmv_bytes(n, n, L0["nnz_a"], 8, 8, fmt=L0["fmt_a"]), 1)
     for k, (ms, by, cnt) in cand.items():
         log(f"  step {k}: {ms * 1e3:.1f} us, {by / 1e6:.1f} MB -> {by / ms / 1e6:.0f} GB/s")
     dom = max(cand, key=lambda k: cand[k][0] * cand[k][2])

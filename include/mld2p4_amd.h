@@ -54,7 +54,10 @@ int mldhip_setr(mldhip_prec* p, const char* what, double val, int ilev, int ilma
 int mldhip_set_precision(mldhip_prec* p, int kind);
 /* Storage of uploaded operators: 0 = automatic (default), 1 = CSR, 2 = ELL,
  * 3 = SELL (sliced ELL, 64-row slices, rows length-sorted in windows of
- * 4096), 4 = SELL with windows of 256 rows, 5 = SELL without sorting. */
+ * 4096), 4 = SELL with windows of 256 rows, 5 = SELL without sorting, 6 = DIA
+ * (diagonal storage, for operators with at most 8 distinct diagonals; others
+ * fall back to the automatic choice).  The automatic choice takes DIA for such
+ * banded operators. */
 int mldhip_set_format(mldhip_prec* p, int fmt);
 
 /* ---- build -------------------------------------------------------------- */
@@ -194,7 +197,8 @@ int mldhip_level_smooth(mldhip_prec* p, int lev, int which, const double* x, int
  * 4 post smoother, 5 Krylov SpMV with its dot (level 0). */
 int mldhip_time_step(mldhip_prec* p, int lev, int kind, int reps, double* avg_ms);
 /* Device storage of a level operator (what 0 A, 1 P, 2 R, 3 ND):
- * fmt 0 = CSR with `lanes` lanes per row, 1 = ELL of width `width`, 2 = SELL. */
+ * fmt 0 = CSR with `lanes` lanes per row, 1 = ELL of width `width`, 2 = SELL,
+ * 3 = DIA with `width` diagonals. */
 int mldhip_get_op_format(mldhip_prec* p, int lev, int what, int* fmt, int* lanes, int* width);
 
 #ifdef __cplusplus

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mld_host.hpp"
@@ -31,7 +32,7 @@ struct DevError : std::runtime_error {
                      std::to_string(__LINE__));                                                     \
   } while (0)
 
-enum { FMT_CSR = 0, FMT_ELL = 1, FMT_SELL = 2 };
+enum { FMT_CSR = 0, FMT_ELL = 1, FMT_SELL = 2, FMT_DIA = 3 };
 
 template <typename V>
 struct DMat {
@@ -40,6 +41,7 @@ struct DMat {
   int fmt = FMT_CSR, lanes = 1, w = 0, ld = 0;
   int iters = 0;        // CSR: entries per lane loaded up front (0, 2, 4, 8)
   int rpl = 1;          // block products (32 columns): columns per lane (1, 2, 4)
+  DiaOff off{};         // DIA: diagonal offsets (w diagonals)
   int* irp = nullptr;   // CSR row pointers
   int* sptr = nullptr;  // SELL slice offsets (nslices + 1)
   int* perm = nullptr;  // SELL position -> row
@@ -60,6 +62,7 @@ struct DSmoother {
   int* ltrows = nullptr; int* utrows = nullptr; int* ltptr_d = nullptr; int* utptr_d = nullptr;
   std::vector<int> ltptr, utptr;
   bool has_t = false;
+  int tri_lanes = 1;  // lanes per row of the ILU triangular solves (1, 4, 16, 32)
   // dense LU
   double* lu = nullptr; int* piv = nullptr;
   double* dinv = nullptr;  // DENSE_LU: explicit inverse (row-major)
@@ -114,6 +117,27 @@ static inline int nblk(int64_t n) { return (int)((n + BS - 1) / BS); }
 // ---------------------------------------------------------------------------
 struct FmtChoice { int fmt, lanes, w; };
 
+// Diagonal offsets of h when it has at most 8 distinct ones (ascending), else
+// empty.
+static std::vector<int> dia_offsets(const HCsr& h) {
+  std::vector<int> offs;
+  for (int i = 0; i < h.nr; ++i)
+    for (int64_t k = h.irp[i]; k < h.irp[i + 1]; ++k) {
+      const int o = h.ja[k] - i;
+      if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
+        offs.push_back(o);
+        if (offs.size() > 8) return {};
+      }
+    }
+  std::sort(offs.begin(), offs.end());
+  return offs;
+}
+
+// DIA pays when the padded diagonals hold at most 10 % more entries than A.
+static bool dia_eligible(const HCsr& h, const std::vector<int>& offs) {
+  return !offs.empty() && h.nr == h.nc && (double)h.nr * offs.size() <= 1.10 * (double)h.nnz();
+}
+
 static FmtChoice choose_format(const HCsr& h, int force) {
   int maxrow = 0;
   for (int i = 0; i < h.nr; ++i) maxrow = std::max(maxrow, (int)(h.irp[i + 1] - h.irp[i]));
@@ -124,6 +148,10 @@ static FmtChoice choose_format(const HCsr& h, int force) {
   while (lanes < 64 && lanes * 4 <= avg) lanes *= 2;
   c.lanes = lanes;
   if (force == 1 || maxrow == 0) return c;
+  if (force == 0 || force == 6) {
+    const std::vector<int> offs = dia_offsets(h);
+    if (dia_eligible(h, offs)) { c.fmt = FMT_DIA; c.w = (int)offs.size(); return c; }
+  }
   if (force == 2 && maxrow <= 64) { c.fmt = FMT_ELL; return c; }
   if (force >= 3 && force <= 5) { c.fmt = FMT_SELL; return c; }
   if (maxrow <= 16 && (double)h.nr * maxrow <= 1.10 * nnz) { c.fmt = FMT_ELL; return c; }
@@ -149,7 +177,19 @@ static DMat<V> upload_mat(const HCsr& h, DevArena& ar, hipStream_t s, int force,
     const int per_lane = (c.w + c.lanes - 1) / std::max(1, c.lanes);
     m.iters = per_lane <= 2 ? 2 : (per_lane <= 4 ? 4 : 8);
   }
-  if (c.fmt == FMT_ELL) {
+  if (c.fmt == FMT_DIA) {
+    const std::vector<int> offs = dia_offsets(h);
+    m.w = (int)offs.size();
+    for (int d = 0; d < 8; ++d) m.off.o[d] = d < m.w ? offs[d] : 0;
+    std::vector<V> val((size_t)h.nr * m.w, (V)0);
+    for (int i = 0; i < h.nr; ++i)
+      for (int64_t k = h.irp[i]; k < h.irp[i + 1]; ++k) {
+        const int d = (int)(std::lower_bound(offs.begin(), offs.end(), h.ja[k] - i) - offs.begin());
+        val[(size_t)d * h.nr + i] = (V)h.val[k];
+      }
+    m.val = ar.upload(val.data(), val.size(), s);
+    HIPCHECK(hipStreamSynchronize(s));
+  } else if (c.fmt == FMT_ELL) {
     m.ld = h.nr;
     std::vector<int> ja((size_t)m.ld * m.w);
     std::vector<V> val((size_t)m.ld * m.w);
@@ -250,47 +290,63 @@ static int spmv_blocks(const DMat<V>& m) {
   return nblk(m.nr);
 }
 
-template <int LANES, int K, typename V, class G, class E>
+template <int LANES, int K, typename C, typename V, class G, class E>
 static void csr_launch_l(const DMat<V>& m, int nb, G g, E e, hipStream_t s, Red red) {
   switch (m.iters) {
-    case 2: csr_kernel<V, LANES, 2, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
-    case 4: csr_kernel<V, LANES, 4, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
-    case 8: csr_kernel<V, LANES, 8, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
-    default: csr_kernel<V, LANES, 0, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
+    case 2: csr_kernel<C, LANES, 2, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
+    case 4: csr_kernel<C, LANES, 4, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
+    case 8: csr_kernel<C, LANES, 8, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
+    default: csr_kernel<C, LANES, 0, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Generic sparse launch: dispatch on format / width / lanes.  K > 0: the
-// epilogue's dot contributions are reduced into red.result[0..K).
+// epilogue's dot contributions are reduced into red.result[0..K).  CT: the
+// arithmetic type when it differs from the stored value type (FP32-stored
+// values that are exact in FP32, summed in FP64).
 // ---------------------------------------------------------------------------
-template <int K = 0, typename V, class G, class E>
+template <int K = 0, typename CT = void, typename V, class G, class E>
 static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red = Red{nullptr, nullptr}) {
+  using C = typename std::conditional<std::is_void<CT>::value, V, CT>::type;
   if (m.nr == 0) return;
   const int nb = spmv_blocks(m);
-  if (m.fmt == FMT_ELL) {
+  if (m.fmt == FMT_DIA) {
+    if (m.w == 7) dia_kernel<C, 7, K><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red);
+    else dia_kernel<C, 0, K><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red);
+  } else if (m.fmt == FMT_ELL) {
     switch (m.w) {
-      case 4: ell_kernel<V, 4, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      case 5: ell_kernel<V, 5, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      case 6: ell_kernel<V, 6, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      case 7: ell_kernel<V, 7, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      default: ell_kernel<V, 0, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
+      case 4: ell_kernel<C, 4, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
+      case 5: ell_kernel<C, 5, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
+      case 6: ell_kernel<C, 6, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
+      case 7: ell_kernel<C, 7, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
+      default: ell_kernel<C, 0, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
     }
   } else if (m.fmt == FMT_SELL) {
-    sell_kernel<V, K><<<nb, BS, 0, s>>>(m.nr, m.sptr, m.perm, m.ja, m.val, g, e, red);
+    sell_kernel<C, K><<<nb, BS, 0, s>>>(m.nr, m.sptr, m.perm, m.ja, m.val, g, e, red);
   } else {
     switch (m.lanes) {
-      case 1: csr_launch_l<1, K>(m, nb, g, e, s, red); break;
-      case 2: csr_launch_l<2, K>(m, nb, g, e, s, red); break;
-      case 4: csr_launch_l<4, K>(m, nb, g, e, s, red); break;
-      case 8: csr_launch_l<8, K>(m, nb, g, e, s, red); break;
-      case 16: csr_launch_l<16, K>(m, nb, g, e, s, red); break;
-      case 32: csr_launch_l<32, K>(m, nb, g, e, s, red); break;
-      default: csr_launch_l<64, K>(m, nb, g, e, s, red); break;
+      case 1: csr_launch_l<1, K, C>(m, nb, g, e, s, red); break;
+      case 2: csr_launch_l<2, K, C>(m, nb, g, e, s, red); break;
+      case 4: csr_launch_l<4, K, C>(m, nb, g, e, s, red); break;
+      case 8: csr_launch_l<8, K, C>(m, nb, g, e, s, red); break;
+      case 16: csr_launch_l<16, K, C>(m, nb, g, e, s, red); break;
+      case 32: csr_launch_l<32, K, C>(m, nb, g, e, s, red); break;
+      default: csr_launch_l<64, K, C>(m, nb, g, e, s, red); break;
     }
   }
   if constexpr (K > 0) finalize_kernel<K><<<1, BS, 0, s>>>(nb, red.partial, red.result);
   HIPCHECK(hipGetLastError());
+}
+
+// True when every value of h is exactly representable in FP32 (then an
+// FP32-stored copy, summed in FP64, gives bit-identical FP64 products).
+static bool fp32_exact(const HCsr& h) {
+  for (double v : h.val) {
+    const float f = (float)v;
+    if (!std::isfinite(v) || (double)f != v) return false;
+  }
+  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -310,6 +366,8 @@ static bool autotune_enabled() {
 template <typename V>
 static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int force) {
   if (force != 0 || h.nr < TUNE_MIN_ROWS || !autotune_enabled()) return upload_mat<V>(h, ar, s, force);
+  // banded operators: DIA reads strictly fewer bytes than any other layout
+  if (dia_eligible(h, dia_offsets(h))) return upload_mat<V>(h, ar, s, 6);
   int maxrow = 0;
   for (int i = 0; i < h.nr; ++i) maxrow = std::max(maxrow, (int)(h.irp[i + 1] - h.irp[i]));
   const double avg = (double)h.nnz() / std::max(1, h.nr);

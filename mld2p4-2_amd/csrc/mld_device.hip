@@ -57,17 +57,25 @@ class DevPrecT : public DevPrec {
       D.t1 = ar_.alloc<V>(n); D.t2 = ar_.alloc<V>(n);
       if (H.smd.solver == DENSE_LU || (D.has_sm2 && H.smd2.solver == DENSE_LU)) D.wlu = ar_.alloc<double>(n);
     }
-    // Krylov: FP64 copy of the fine matrix (shared with level 0 in FP64)
+    // Krylov matrix, summed in FP64: level 0's own copy in FP64; in the FP32
+    // form, level 0's FP32 copy when every value of A is exact in FP32 (the
+    // products are then bit-identical to FP64 storage), else an FP64 copy.
+    akry_is32_ = false;
     if constexpr (std::is_same<V, double>::value) {
       akry_ = L_[0].a;
     } else {
-      akry_ = upload_mat<double>(p.precv[0].a, ar_, s_, p.dev_format);
+      if (fp32_exact(p.precv[0].a)) {
+        akry32_ = L_[0].a;
+        akry_is32_ = true;
+      } else {
+        akry_ = upload_mat<double>(p.precv[0].a, ar_, s_, p.dev_format);
+      }
     }
     n0_ = L_[0].n;
     {
       // partial sums of every reducing launch: Krylov SpMV, vector reductions,
       // fused output of the fine level's post smoother
-      int nbmax = std::max(spmv_blocks(akry_), red_blocks(n0_));
+      int nbmax = std::max(std::max(spmv_blocks(akry_), spmv_blocks(akry32_)), red_blocks(n0_));
       nbmax = std::max(nbmax, spmv_blocks(L_[0].sm.nd));
       if (L_[0].has_sm2) nbmax = std::max(nbmax, spmv_blocks(L_[0].sm2.nd));
       nbmax = std::max(nbmax, nblk(n0_));
@@ -230,7 +238,7 @@ class DevPrecT : public DevPrec {
         }
         case 4: return smooth(D, D.has_sm2 ? D.sm2 : D.sm, D.ty, 1, D.y2l, D.sweeps_post, 'N');
         case 5:
-          spmv_launch<1>(akry_, GatherX<double, double>{kx}, EpiSetDot<double>{kq, kx, nullptr}, s_, red(S_TMP0));
+          kspmv<1>('N', GatherX<double, double>{kx}, EpiSetDot<double>{kq, kx, nullptr}, red(S_TMP0));
           return 0;
       }
       return E_BAD_ARG;
@@ -262,7 +270,16 @@ class DevPrecT : public DevPrec {
   int outer_sweeps_ = 1;
   int n0_ = 0;
   DMat<double> akry_;
+  DMat<float> akry32_;   // FP32-stored A (exact values), used when akry_is32_
+  bool akry_is32_ = false;
   DMat<double> akry_t_;  // FP64 A^T for adjoint solves, uploaded on first use
+  // Krylov product with A ('N') or A^T ('T'), FP64 arithmetic
+  template <int K, class G, class E>
+  void kspmv(char trans, G g, E e, Red r = Red{nullptr, nullptr}) {
+    if (trans == 'T') spmv_launch<K>(akry_t_, g, e, s_, r);
+    else if (akry_is32_) spmv_launch<K, double>(akry32_, g, e, s_, r);
+    else spmv_launch<K>(akry_, g, e, s_, r);
+  }
   bool akry_t_set_ = false;
   double* partial_ = nullptr;
   Red red(int slot) { return Red{partial_, scal_ + slot}; }
@@ -309,8 +326,13 @@ class DevPrecT : public DevPrec {
     }
     if (H.nnz_nd > 0) S.nd = upload_tuned<V>(H.nd, ar_, s_, force);
     if (H.solver == ILU_N) {
-      S.l = upload_mat<V>(H.l, ar_, s_, 1, 1);  // CSR, one lane per row (stored order)
+      S.l = upload_mat<V>(H.l, ar_, s_, 1, 1);  // CSR (rows summed in stored order)
       S.u = upload_mat<V>(H.u, ar_, s_, 1, 1);
+      {
+        // lanes per row of the triangular solves, from the mean row length
+        const double avg = (double)std::max(H.l.nnz(), H.u.nnz()) / std::max(1, H.l.nr);
+        S.tri_lanes = avg <= 1.5 ? 1 : (avg <= 6.0 ? 4 : (avg <= 24.0 ? 16 : 32));
+      }
       S.lrows = ar_.upload(H.lsched_rows.data(), H.lsched_rows.size(), s_);
       S.urows = ar_.upload(H.usched_rows.data(), H.usched_rows.size(), s_);
       S.lptr = H.lsched_ptr;
@@ -414,48 +436,10 @@ class DevPrecT : public DevPrec {
         if (beta) vmul_kernel<V, V, true><<<nblk(n), BS, 0, s_>>>(n, S.dv, x, y);
         else vmul_kernel<V, V, false><<<nblk(n), BS, 0, s_>>>(n, S.dv, x, y);
         break;
-      case ILU_N: {
-        const bool t = (trans != 'N');
-        const DMat<V>& F1 = t ? S.ut : S.l;   // first (lower-triangular) solve
-        const DMat<V>& F2 = t ? S.lt : S.u;   // second (upper-triangular) solve
-        const std::vector<int>& p1 = t ? S.utptr : S.lptr;
-        const std::vector<int>& p2 = t ? S.ltptr : S.uptr;
-        const int* r1 = t ? S.utrows : S.lrows;
-        const int* r2 = t ? S.ltrows : S.urows;
-        const int* p1d = t ? S.utptr_d : S.lptr_d;
-        const int* p2d = t ? S.ltptr_d : S.uptr_d;
-        const int nl1 = (int)p1.size() - 1, nl2 = (int)p2.size() - 1;
-        V* t1 = D.tx;
-        V* t2 = beta ? D.t2 : y;
-        if (n <= 8192) {
-          // one workgroup; a single wavefront when the matrix is tiny
-          const int nt = std::min(1024, ((n + 63) / 64) * 64);
-          if (beta)
-            ilu_small_kernel<V, 1><<<1, nt, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d, r1,
-                                                     nl1, p2d, r2, nl2, x, t1, t2, y);
-          else
-            ilu_small_kernel<V, 0><<<1, nt, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d, r1,
-                                                     nl1, p2d, r2, nl2, x, t1, t2, y);
-          HIPCHECK(hipGetLastError());
-        } else {
-          for (int lv = 0; lv < nl1; ++lv) {
-            const int cnt = p1[lv + 1] - p1[lv];
-            trsv_level_kernel<V, false, 2><<<nblk(cnt), BS, 0, s_>>>(cnt, r1 + p1[lv], F1.irp, F1.ja, F1.val, x,
-                                                                    nullptr, t1, nullptr);
-          }
-          for (int lv = 0; lv < nl2; ++lv) {
-            const int cnt = p2[lv + 1] - p2[lv];
-            if (beta)
-              trsv_level_kernel<V, true, 1><<<nblk(cnt), BS, 0, s_>>>(cnt, r2 + p2[lv], F2.irp, F2.ja, F2.val, t1,
-                                                                     S.dv, t2, y);
-            else
-              trsv_level_kernel<V, true, 2><<<nblk(cnt), BS, 0, s_>>>(cnt, r2 + p2[lv], F2.irp, F2.ja, F2.val, t1,
-                                                                     S.dv, t2, y);
-          }
-          HIPCHECK(hipGetLastError());
-        }
+      case ILU_N:
+        if (trans != 'N') ilu_apply<true>(D, S, x, beta, y);
+        else ilu_apply<false>(D, S, x, beta, y);
         break;
-      }
       case DENSE_LU:  // y = A^-1 x with the inverse formed at upload (FP64)
         if (trans != 'N') return E_UNSUPPORTED;
         if (beta) dense_gemv_kernel<V, 1><<<(n + 3) / 4, 256, 0, s_>>>(n, S.dinv, x, y);
@@ -468,14 +452,95 @@ class DevPrecT : public DevPrec {
     return 0;
   }
 
+  // ILU(0) solver, y = U^-1 D L^-1 x ('N') or L^-T D U^-T x ('T'), beta 0/1
+  // (mld_d_ilu_solver_apply_vect.f90:130-142).
+  template <bool TR>
+  void ilu_apply(DLevel<V>& D, DSmoother<V>& S, const V* x, int beta, V* y) {
+    switch (S.tri_lanes) {
+      case 1: ilu_apply_l<TR, 1>(D, S, x, beta, y); break;
+      case 4: ilu_apply_l<TR, 4>(D, S, x, beta, y); break;
+      case 16: ilu_apply_l<TR, 16>(D, S, x, beta, y); break;
+      default: ilu_apply_l<TR, 32>(D, S, x, beta, y); break;
+    }
+  }
+  template <bool TR, int LN>
+  void ilu_apply_l(DLevel<V>& D, DSmoother<V>& S, const V* x, int beta, V* y) {
+    const int n = D.n;
+    const DMat<V>& F1 = TR ? S.ut : S.l;  // first (lower-triangular) solve
+    const DMat<V>& F2 = TR ? S.lt : S.u;  // second (upper-triangular) solve
+    const std::vector<int>& p1 = TR ? S.utptr : S.lptr;
+    const std::vector<int>& p2 = TR ? S.ltptr : S.uptr;
+    const int* r1 = TR ? S.utrows : S.lrows;
+    const int* r2 = TR ? S.ltrows : S.urows;
+    const int* p1d = TR ? S.utptr_d : S.lptr_d;
+    const int* p2d = TR ? S.ltptr_d : S.uptr_d;
+    const int nl1 = (int)p1.size() - 1, nl2 = (int)p2.size() - 1;
+    V* t1 = D.tx;
+    V* t2 = beta ? D.t2 : y;
+    if (n <= 8192) {
+      // one workgroup; a single wavefront when the matrix is tiny
+      const int nt = std::min(1024, (int)(((int64_t)n * LN + 63) / 64) * 64);
+      if (beta)
+        ilu_small_kernel<V, 1, TR, LN><<<1, nt, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d,
+                                                         r1, nl1, p2d, r2, nl2, x, t1, t2, y);
+      else
+        ilu_small_kernel<V, 0, TR, LN><<<1, nt, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d,
+                                                         r1, nl1, p2d, r2, nl2, x, t1, t2, y);
+      HIPCHECK(hipGetLastError());
+      return;
+    }
+    // level sets of at most RUN_ROWS rows (two passes of the workgroup):
+    // consecutive ones in one single-workgroup launch; larger ones: one grid
+    // launch each
+    constexpr int RUN_ROWS = 2048 / LN;
+    auto nbl = [](int cnt) { return (int)(((int64_t)cnt * LN + BS - 1) / BS); };
+    for (int lv = 0; lv < nl1;) {
+      int le = lv;
+      while (le < nl1 && p1[le + 1] - p1[le] <= RUN_ROWS) ++le;
+      if (le - lv >= 2) {
+        trsv_run_kernel<V, false, 2, TR, LN><<<1, 1024, 0, s_>>>(lv, le, p1d, r1, F1.irp, F1.ja, F1.val, x,
+                                                                nullptr, t1, nullptr);
+        lv = le;
+        continue;
+      }
+      const int cnt = p1[lv + 1] - p1[lv];
+      trsv_level_kernel<V, false, 2, TR, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r1 + p1[lv], F1.irp, F1.ja, F1.val, x,
+                                                                     nullptr, t1, nullptr);
+      ++lv;
+    }
+    for (int lv = 0; lv < nl2;) {
+      int le = lv;
+      while (le < nl2 && p2[le + 1] - p2[le] <= RUN_ROWS) ++le;
+      if (le - lv >= 2) {
+        if (beta)
+          trsv_run_kernel<V, true, 1, TR, LN><<<1, 1024, 0, s_>>>(lv, le, p2d, r2, F2.irp, F2.ja, F2.val, t1, S.dv,
+                                                                 t2, y);
+        else
+          trsv_run_kernel<V, true, 2, TR, LN><<<1, 1024, 0, s_>>>(lv, le, p2d, r2, F2.irp, F2.ja, F2.val, t1, S.dv,
+                                                                 t2, y);
+        lv = le;
+        continue;
+      }
+      const int cnt = p2[lv + 1] - p2[lv];
+      if (beta)
+        trsv_level_kernel<V, true, 1, TR, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r2 + p2[lv], F2.irp, F2.ja, F2.val, t1,
+                                                                      S.dv, t2, y);
+      else
+        trsv_level_kernel<V, true, 2, TR, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r2 + p2[lv], F2.irp, F2.ja, F2.val, t1,
+                                                                      S.dv, t2, y);
+      ++lv;
+    }
+    HIPCHECK(hipGetLastError());
+  }
+
   // One Jacobi sweep y_out = D^-1 (b - ND y_in) (FIRST: y_in = D^-1 b).
-  // FIRST on an ELL operator gathers d_j*x_j directly; on SELL/CSR, whose
-  // gathers are the costly part, d.*x is formed first in one streaming pass
-  // (same products, so the same result) and gathered once.
+  // FIRST on an ELL or DIA operator gathers d_j*x_j directly; on SELL/CSR,
+  // whose gathers are the costly part, d.*x is formed first in one streaming
+  // pass (same products, so the same result) and gathered once.
   template <bool FIRST, int K, class E>
   void jac_launch(DLevel<V>& D, DSmoother<V>& S, const V* b, const V* yin, E e, char trans, Red red) {
     const DMat<V>& nd = (trans != 'N') ? S.ndt : S.nd;
-    if (FIRST && nd.fmt == FMT_ELL) {
+    if (FIRST && (nd.fmt == FMT_ELL || nd.fmt == FMT_DIA)) {
       spmv_launch<K>(nd, GatherDX<V, V>{S.dv, b}, e, s_, red);
     } else if (FIRST) {
       vmul_kernel<V, V, false><<<nblk(D.n), BS, 0, s_>>>(D.n, S.dv, b, D.t2);
@@ -666,14 +731,13 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       akry_t_set_ = true;
     }
   }
-  const DMat<double>& akry = (trans == 'T') ? akry_t_ : akry_;
-  double* r = kvec(0);
+    double* r = kvec(0);
   double* z = kvec(1);
   double* p = kvec(2);
   double* q = kvec(3);
   // bn2 and r = b - A x
   dot1(b, b, S_TMP0);
-  spmv_launch(akry, GatherX<double, double>{x}, EpiRes<double, double>{b, r}, s_);
+  kspmv<0>(trans, GatherX<double, double>{x}, EpiRes<double, double>{b, r});
   dot1(r, r, S_RR);
   HIPCHECK(hipMemcpyAsync(hscal_, scal_, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, s_));
   HIPCHECK(hipStreamSynchronize(s_));
@@ -689,7 +753,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       int rc = apply_impl(r, z, trans, r, (it & 1) ? S_RHO1 : S_RHO0);  // z = M^-1 r, rho = r.z
       if (rc) return rc;
       cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, z, p, scal_, it);
-      spmv_launch<1>(akry, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, s_, red(S_SIGMA));
+      kspmv<1>(trans, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, red(S_SIGMA));
       xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, (it & 1) ? S_RHO1 : S_RHO0, S_SIGMA, red(S_RR));
       finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
@@ -707,7 +771,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       if (rc) return rc;
       fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z, p, r, scal_, it, red(S_PR));
       finalize(1, S_PR);
-      spmv_launch<1>(akry, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, s_, red(S_DELTA));
+      kspmv<1>(trans, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, red(S_DELTA));
       xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, S_PR, S_DELTA, red(S_RR));
       finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
@@ -739,7 +803,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       }
       int rc = apply(p, ph, trans);
       if (rc) return rc;
-      spmv_launch<1>(akry, GatherX<double, double>{ph}, EpiSetDot<double>{v, rt, nullptr}, s_, red(S_TMP1));
+      kspmv<1>(trans, GatherX<double, double>{ph}, EpiSetDot<double>{v, rt, nullptr}, red(S_TMP1));
       const double rtv = read_scal(S_TMP1);
       if (rtv == 0.0) break;
       alpha = rho / rtv;
@@ -755,7 +819,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       }
       rc = apply(s, sh, trans);
       if (rc) return rc;
-      spmv_launch<2>(akry, GatherX<double, double>{sh}, EpiSetDot<double>{t, nullptr, s}, s_, red(S_TMP0));  // t.t, s.t
+      kspmv<2>(trans, GatherX<double, double>{sh}, EpiSetDot<double>{t, nullptr, s}, red(S_TMP0));  // t.t, s.t
       read_scal(S_TMP0);
       const double tt = hscal_[S_TMP0], ts = hscal_[S_TMP1];
       omega = (tt == 0.0) ? 0.0 : ts / tt;

@@ -181,9 +181,9 @@ __device__ __forceinline__ int xcd_block() { return blockIdx.x; }
 // (predicated), so that all index/value loads and then all gathers of a lane
 // are in flight together; rows longer than ITERS*LANES finish in a loop.
 // The lane's summation order is the same as the plain strided loop.
-template <typename V, int LANES, int ITERS, int K, class G, class E>
+template <typename V, int LANES, int ITERS, int K, class G, class E, typename S>
 __global__ __launch_bounds__(BS) void csr_kernel(int nr, const int* __restrict__ irp,
-                                                  const int* __restrict__ ja, const V* __restrict__ val,
+                                                  const int* __restrict__ ja, const S* __restrict__ val,
                                                   G g, E e, Red red) {
   const int lb = xcd_block();
   const int gid = lb * BS + threadIdx.x;
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(BS) void csr_kernel(int nr, const int* __restrict__
         const int k = k0 + it * LANES;
         const bool in = k < en;
         jj[it] = in ? ldnt(ja + k) : -1;
-        vv[it] = in ? ldnt(val + k) : (V)0;
+        vv[it] = in ? (V)ldnt(val + k) : (V)0;
       }
       V xx[ITERS];
 #pragma unroll
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(BS) void csr_kernel(int nr, const int* __restrict__
         if (jj[it] >= 0) acc = acc + vv[it] * xx[it];
       k0 += ITERS * LANES;
     }
-    for (int k = k0; k < en; k += LANES) acc = acc + ldnt(val + k) * g(ldnt(ja + k));
+    for (int k = k0; k < en; k += LANES) acc = acc + (V)ldnt(val + k) * g(ldnt(ja + k));
   }
   if (LANES > 1) {
 #pragma unroll
@@ -222,9 +222,9 @@ __global__ __launch_bounds__(BS) void csr_kernel(int nr, const int* __restrict__
   if constexpr (K > 0) block_partials<K>(d, red.partial, lb);
 }
 
-template <typename V, int W, int K, class G, class E>
+template <typename V, int W, int K, class G, class E, typename S>
 __global__ __launch_bounds__(BS) void ell_kernel(int nr, int w, int ld, const int* __restrict__ ja,
-                                                  const V* __restrict__ val, G g, E e, Red red) {
+                                                  const S* __restrict__ val, G g, E e, Red red) {
   const int lb = xcd_block();
   const int row = lb * BS + threadIdx.x;
   double d[2] = {0.0, 0.0};
@@ -234,12 +234,12 @@ __global__ __launch_bounds__(BS) void ell_kernel(int nr, int w, int ld, const in
 #pragma unroll
       for (int k = 0; k < W; ++k) {
         const int idx = k * ld + row;
-        acc = acc + ldnt(val + idx) * g(ldnt(ja + idx));
+        acc = acc + (V)ldnt(val + idx) * g(ldnt(ja + idx));
       }
     } else {
       for (int k = 0; k < w; ++k) {
         const int idx = k * ld + row;
-        acc = acc + ldnt(val + idx) * g(ldnt(ja + idx));
+        acc = acc + (V)ldnt(val + idx) * g(ldnt(ja + idx));
       }
     }
     e(row, acc, d);
@@ -247,12 +247,40 @@ __global__ __launch_bounds__(BS) void ell_kernel(int nr, int w, int ld, const in
   if constexpr (K > 0) block_partials<K>(d, red.partial, lb);
 }
 
+// DIA: nd diagonals with offsets off.o[0..nd) ascending; entry (i, i+o_d) at
+// val[d*nr + i], zero where the row has no such entry (its column is then
+// clamped into range; 0 * x adds nothing).  No column indices are stored or
+// loaded, and the gathers of a wavefront are contiguous.  Summed over the
+// diagonals in ascending offset order = the row's stored column order.
+struct DiaOff {
+  int o[8];
+};
+template <typename V, int ND, int K, class G, class E, typename S>
+__global__ __launch_bounds__(BS) void dia_kernel(int nr, int nc, int nd, DiaOff off, const S* __restrict__ val, G g,
+                                                  E e, Red red) {
+  const int lb = xcd_block();
+  const int row = lb * BS + threadIdx.x;
+  double dd[2] = {0.0, 0.0};
+  if (row < nr) {
+    V acc = 0;
+    const int ndg = (ND > 0) ? ND : nd;
+#pragma unroll
+    for (int k = 0; k < ((ND > 0) ? ND : 8); ++k) {
+      if (ND == 0 && k >= ndg) break;
+      const int j = min(max(row + off.o[k], 0), nc - 1);
+      acc = acc + (V)ldnt(val + (int64_t)k * nr + row) * g(j);
+    }
+    e(row, acc, dd);
+  }
+  if constexpr (K > 0) block_partials<K>(dd, red.partial, lb);
+}
+
 // SELL: position p = 64*slice + lane holds row perm[p]; slice s occupies
 // [sptr[s], sptr[s+1]) with entry k of its lane at sptr[s] + 64*k + lane.
-template <typename V, int K, class G, class E>
+template <typename V, int K, class G, class E, typename S>
 __global__ __launch_bounds__(BS) void sell_kernel(int nr, const int* __restrict__ sptr,
                                                    const int* __restrict__ perm, const int* __restrict__ ja,
-                                                   const V* __restrict__ val, G g, E e, Red red) {
+                                                   const S* __restrict__ val, G g, E e, Red red) {
   const int lb = xcd_block();
   const int p = lb * BS + threadIdx.x;
   double d[2] = {0.0, 0.0};
@@ -265,7 +293,8 @@ __global__ __launch_bounds__(BS) void sell_kernel(int nr, const int* __restrict_
     for (; k + 4 <= w; k += 4) {
       const int i0 = base + 64 * k;
       const int j0 = ldnt(ja + i0), j1 = ldnt(ja + i0 + 64), j2 = ldnt(ja + i0 + 128), j3 = ldnt(ja + i0 + 192);
-      const V v0 = ldnt(val + i0), v1 = ldnt(val + i0 + 64), v2 = ldnt(val + i0 + 128), v3 = ldnt(val + i0 + 192);
+      const V v0 = (V)ldnt(val + i0), v1 = (V)ldnt(val + i0 + 64), v2 = (V)ldnt(val + i0 + 128),
+              v3 = (V)ldnt(val + i0 + 192);
       const V x0 = g(j0), x1 = g(j1), x2 = g(j2), x3 = g(j3);
       acc = acc + v0 * x0;
       acc = acc + v1 * x1;
@@ -274,7 +303,7 @@ __global__ __launch_bounds__(BS) void sell_kernel(int nr, const int* __restrict_
     }
     for (; k < w; ++k) {
       const int i0 = base + 64 * k;
-      acc = acc + ldnt(val + i0) * g(ldnt(ja + i0));
+      acc = acc + (V)ldnt(val + i0) * g(ldnt(ja + i0));
     }
     e(perm[p], acc, d);
   }
@@ -422,57 +451,129 @@ __global__ __launch_bounds__(BS) void fcg_pupdate(int n, const double* __restric
 }
 
 // ---------------------------------------------------- triangular solves
-// Unit lower / upper solve of the rows of one level set, one lane per row,
-// row sum in stored order:  t_i = b_i - sum_j l_ij t_j  (for the upper solve
-// of ILU b_i = d_i * t1_i, the D scaling of psb_spsm).  BETA: 0 also writes
-// y = t, 1 writes y = y + t, 2 writes t only.
-template <typename V, bool UPPER, int BETA>
+// One row of a unit triangular solve by a group of LANES lanes.  b_i is the
+// right-hand side (times d_i for the second solve of ILU, the D scaling of
+// psb_spsm).  The lanes form the products f_ij * t_j of LANES stored entries
+// at once (independent loads in flight together); every lane then adds them
+// up in the sequential order with cross-lane reads, so the result is the
+// sequential loop's:
+//   TR = false (L, U as stored): t_i = b_i - (f_ij1 t_j1 + f_ij2 t_j2 + ...);
+//   TR = true (rows of U^T / L^T, the transposed apply): products subtracted
+//     one at a time in the order of the column-oriented sweep over the
+//     untransposed factor (U^T: ascending, L^T: descending column index),
+//     t_i = (...((b_i - f_ij1 t_j1) - f_ij2 t_j2) ...).
+template <typename V, bool UPPER, bool TR, int LANES>
+__device__ __forceinline__ V tri_row(int i, int lane, const int* __restrict__ irp, const int* __restrict__ ja,
+                                     const V* __restrict__ val, const V* __restrict__ b,
+                                     const V* __restrict__ d, const V* t) {
+  const int k0 = irp[i], k1 = irp[i + 1];
+  const V bi = UPPER ? d[i] * b[i] : b[i];
+  V acc = TR ? bi : (V)0;
+  if constexpr (!(TR && UPPER)) {
+    for (int kb = k0; kb < k1; kb += LANES) {
+      const int k = kb + lane;
+      V pk = (V)0;
+      if (k < k1) pk = val[k] * t[ja[k]];
+      const int cnt = min(LANES, k1 - kb);
+      if constexpr (LANES == 1) {
+        acc = TR ? acc - pk : acc + pk;
+      } else {
+        for (int q = 0; q < cnt; ++q) {
+          const V pq = __shfl(pk, q, LANES);
+          acc = TR ? acc - pq : acc + pq;
+        }
+      }
+    }
+  } else {
+    for (int ke = k1; ke > k0; ke -= LANES) {
+      const int kb = max(k0, ke - LANES);
+      const int k = kb + lane;
+      V pk = (V)0;
+      if (k < ke) pk = val[k] * t[ja[k]];
+      if constexpr (LANES == 1) {
+        acc = acc - pk;
+      } else {
+        for (int q = ke - kb - 1; q >= 0; --q) acc = acc - __shfl(pk, q, LANES);
+      }
+    }
+  }
+  return TR ? acc : bi - acc;
+}
+
+// The rows of one level set, LANES lanes per row.  BETA: 0 also writes
+// y = t, 1 writes y = y + t, 2 writes t only (y only in the second solve).
+template <typename V, bool UPPER, int BETA, bool TR, int LANES>
 __global__ __launch_bounds__(BS) void trsv_level_kernel(int cnt, const int* __restrict__ rows,
                                                          const int* __restrict__ irp, const int* __restrict__ ja,
                                                          const V* __restrict__ val, const V* __restrict__ b,
-                                                         const V* __restrict__ d, V* __restrict__ t,
-                                                         V* __restrict__ y) {
-  const int idx = blockIdx.x * BS + threadIdx.x;
-  if (idx >= cnt) return;
+                                                         const V* __restrict__ d, V* t, V* __restrict__ y) {
+  const int gid = blockIdx.x * BS + threadIdx.x;
+  const int idx = gid / LANES, lane = gid % LANES;
+  if (idx >= cnt) return;  // whole lane groups leave together
   const int i = rows[idx];
-  V acc = 0;
-  for (int k = irp[i]; k < irp[i + 1]; ++k) acc = acc + val[k] * t[ja[k]];
-  V bi = UPPER ? d[i] * b[i] : b[i];
-  V ti = bi - acc;
-  t[i] = ti;
-  if (UPPER) {
-    if (BETA == 0) y[i] = ti;
-    else if (BETA == 1) y[i] = y[i] + ti;
+  const V ti = tri_row<V, UPPER, TR, LANES>(i, lane, irp, ja, val, b, d, t);
+  if (lane == 0) {
+    t[i] = ti;
+    if (UPPER) {
+      if (BETA == 0) y[i] = ti;
+      else if (BETA == 1) y[i] = y[i] + ti;
+    }
+  }
+}
+
+// A run of consecutive small level sets [lv0, lv1) of one triangular solve
+// in one workgroup, with a barrier between level sets; per row the same
+// arithmetic as trsv_level_kernel.  Replaces one launch per small level set.
+template <typename V, bool UPPER, int BETA, bool TR, int LANES>
+__global__ __launch_bounds__(1024) void trsv_run_kernel(int lv0, int lv1, const int* __restrict__ ptr,
+                                                         const int* __restrict__ rows, const int* __restrict__ irp,
+                                                         const int* __restrict__ ja, const V* __restrict__ val,
+                                                         const V* __restrict__ b, const V* __restrict__ d, V* t,
+                                                         V* __restrict__ y) {
+  const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES, ngrp = blockDim.x / LANES;
+  for (int lv = lv0; lv < lv1; ++lv) {
+    for (int q = ptr[lv] + grp; q < ptr[lv + 1]; q += ngrp) {
+      const int i = rows[q];
+      const V ti = tri_row<V, UPPER, TR, LANES>(i, lane, irp, ja, val, b, d, t);
+      if (lane == 0) {
+        t[i] = ti;
+        if (UPPER) {
+          if (BETA == 0) y[i] = ti;
+          else if (BETA == 1) y[i] = y[i] + ti;
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
 // Whole ILU(0) apply for a small matrix in one workgroup: level sets
 // processed in order with a barrier in between.
-template <typename V, int BETA>
+template <typename V, int BETA, bool TR, int LANES>
 __global__ __launch_bounds__(1024) void ilu_small_kernel(
     int n, const int* __restrict__ lirp, const int* __restrict__ lja, const V* __restrict__ lval,
     const int* __restrict__ uirp, const int* __restrict__ uja, const V* __restrict__ uval,
     const V* __restrict__ d, const int* __restrict__ lptr, const int* __restrict__ lrows, int nlevl,
     const int* __restrict__ uptr, const int* __restrict__ urows, int nlevu, const V* __restrict__ x,
-    V* __restrict__ t1, V* __restrict__ t2, V* __restrict__ y) {
+    V* t1, V* t2, V* __restrict__ y) {
+  const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES, ngrp = blockDim.x / LANES;
   for (int lv = 0; lv < nlevl; ++lv) {
-    for (int q = lptr[lv] + threadIdx.x; q < lptr[lv + 1]; q += blockDim.x) {
+    for (int q = lptr[lv] + grp; q < lptr[lv + 1]; q += ngrp) {
       const int i = lrows[q];
-      V acc = 0;
-      for (int k = lirp[i]; k < lirp[i + 1]; ++k) acc = acc + lval[k] * t1[lja[k]];
-      t1[i] = x[i] - acc;
+      const V ti = tri_row<V, false, TR, LANES>(i, lane, lirp, lja, lval, x, nullptr, t1);
+      if (lane == 0) t1[i] = ti;
     }
     __syncthreads();
   }
   for (int lv = 0; lv < nlevu; ++lv) {
-    for (int q = uptr[lv] + threadIdx.x; q < uptr[lv + 1]; q += blockDim.x) {
+    for (int q = uptr[lv] + grp; q < uptr[lv + 1]; q += ngrp) {
       const int i = urows[q];
-      V acc = 0;
-      for (int k = uirp[i]; k < uirp[i + 1]; ++k) acc = acc + uval[k] * t2[uja[k]];
-      const V ti = d[i] * t1[i] - acc;
-      t2[i] = ti;
-      if (BETA == 0) y[i] = ti;
-      else y[i] = y[i] + ti;
+      const V ti = tri_row<V, true, TR, LANES>(i, lane, uirp, uja, uval, t1, d, t2);
+      if (lane == 0) {
+        t2[i] = ti;
+        if (BETA == 0) y[i] = ti;
+        else y[i] = y[i] + ti;
+      }
     }
     __syncthreads();
   }

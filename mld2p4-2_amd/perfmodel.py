@@ -12,28 +12,38 @@ from __future__ import annotations
 DIAG_SCALE, ILU_N, L1_DIAG, DENSE_LU, F_NONE = 5, 7, 101, 102, 4
 
 
-def spmv_bytes(nr, nc, nnz, sv, sx, beta_read=False, csr=True, nb=1):
-    """y = A x (beta_read: y = y + A x); nb columns share one pass over A."""
-    b = nnz * (sv + 4) + nb * (nc * sx + nr * sx * (2 if beta_read else 1))
-    if csr:
+def mat_bytes(nr, nnz, sv, fmt):
+    """Compulsory bytes of one pass over a stored operator: values and column
+    indices (CSR also row pointers; DIA stores no indices)."""
+    if fmt == "DIA":
+        return nnz * sv
+    b = nnz * (sv + 4)
+    if fmt == "CSR":
         b += (nr + 1) * 4
     return b
 
 
-def residual_bytes(n, nnz, sv, sx, csr=False, nb=1):
+def spmv_bytes(nr, nc, nnz, sv, sx, beta_read=False, csr=True, nb=1, fmt=None):
+    """y = A x (beta_read: y = y + A x); nb columns share one pass over A."""
+    fmt = fmt or ("CSR" if csr else "ELL")
+    return mat_bytes(nr, nnz, sv, fmt) + nb * (nc * sx + nr * sx * (2 if beta_read else 1))
+
+
+def residual_bytes(n, nnz, sv, sx, csr=False, nb=1, fmt=None):
     # r = b - A y : A, y (gathered, once), b, r
-    return nnz * (sv + 4) + 3 * n * sx * nb + ((n + 1) * 4 if csr else 0)
+    fmt = fmt or ("CSR" if csr else "ELL")
+    return mat_bytes(n, nnz, sv, fmt) + 3 * n * sx * nb
 
 
-def smoother_bytes(n, solver, sweeps, nnz_nd, nnz_lu, sv, sx, beta, nd_csr=False, nb=1):
+def smoother_bytes(n, solver, sweeps, nnz_nd, nnz_lu, sv, sx, beta, nd_csr=False, nb=1, nd_fmt=None):
     """One sm%apply(1, x, beta, y, sweeps, init='Z') on nb columns."""
     yread = n * sx * nb if beta else 0
     if solver in (DIAG_SCALE, L1_DIAG):
         if sweeps <= 1 or nnz_nd == 0:
             return 2 * n * sx * nb + n * sv + yread  # y = d*x
-        rp = (n + 1) * 4 if nd_csr else 0
-        first = nnz_nd * (sv + 4) + rp + n * (2 * sx * nb + sv)  # D^-1(x - ND D^-1 x)
-        more = (sweeps - 2) * (nnz_nd * (sv + 4) + rp + n * (3 * sx * nb + sv))
+        m = mat_bytes(n, nnz_nd, sv, nd_fmt or ("CSR" if nd_csr else "ELL"))
+        first = m + n * (2 * sx * nb + sv)  # D^-1(x - ND D^-1 x)
+        more = (sweeps - 2) * (m + n * (3 * sx * nb + sv))
         return first + more + yread
     if solver == ILU_N:
         return nnz_lu * (sv + 4) + 2 * (n + 1) * 4 + n * sv + 4 * n * sx + yread
@@ -54,13 +64,15 @@ def vcycle_bytes(levels, sv, sx):
             C = levels[l + 1]
             nc = C["n"]
             steps.append((l, "pre", smoother_bytes(n, L["solver"], L["sweeps_pre"], L["nnz_nd"], L.get("nnz_lu", 0),
-                                                   sv, sx, 0, L.get("csr_nd", False))))
-            steps.append((l, "residual", residual_bytes(n, L["nnz_a"], sv, sx, L.get("csr_a", False))))
-            steps.append((l + 1, "restrict", spmv_bytes(nc, n, C["nnz_r"], sv, sx)))
-            steps.append((l + 1, "prolong", spmv_bytes(n, nc, C["nnz_p"], sv, sx, beta_read=True)))
-            steps.append((l, "residual", residual_bytes(n, L["nnz_a"], sv, sx, L.get("csr_a", False))))
+                                                   sv, sx, 0, L.get("csr_nd", False), nd_fmt=L.get("fmt_nd"))))
+            steps.append((l, "residual", residual_bytes(n, L["nnz_a"], sv, sx, L.get("csr_a", False),
+                                                        fmt=L.get("fmt_a"))))
+            steps.append((l + 1, "restrict", spmv_bytes(nc, n, C["nnz_r"], sv, sx, fmt=C.get("fmt_r"))))
+            steps.append((l + 1, "prolong", spmv_bytes(n, nc, C["nnz_p"], sv, sx, beta_read=True, fmt=C.get("fmt_p"))))
+            steps.append((l, "residual", residual_bytes(n, L["nnz_a"], sv, sx, L.get("csr_a", False),
+                                                        fmt=L.get("fmt_a"))))
             steps.append((l, "post", smoother_bytes(n, L["solver"], L["sweeps_post"], L["nnz_nd"], L.get("nnz_lu", 0),
-                                                    sv, sx, 1, L.get("csr_nd", False))))
+                                                    sv, sx, 1, L.get("csr_nd", False), nd_fmt=L.get("fmt_nd"))))
         else:
             steps.append((l, "coarse", smoother_bytes(n, L["solver"], L["sweeps_pre"], L["nnz_nd"],
                                                       L.get("nnz_lu", 0), sv, sx, 0)))

@@ -280,3 +280,28 @@ def test_autograd_vs_central_fd(eng):
         worst = max(worst, abs(fd - g[c]) / np.abs(g).max())
     print(f"worst FD difference / max|g| = {worst:.2e}")
     assert worst <= 1e-5
+
+
+@pytest.mark.gpu
+def test_autograd_mixed_vs_fp64_reference(eng):
+    """The differentiation workload's solver settings (FP32-mixed AMG-CG to
+    1e-10, forward and adjoint): gradient within 1e-6 (max-norm, relative)
+    of the FP64 reference gradient."""
+    _say("[autograd mixed vs reference] idim=20")
+    import torch
+    from mld2p4_amd import autograd as ag
+
+    idim = 20
+    th, b, xs = _grad_setup(idim)
+    opts = ag.SolveOptions(method="CG", eps=1e-10, precision="mixed")
+    t = dev(th).requires_grad_(True)
+    x = ag.solve(t, dev(b), idim, opts)
+    loss = 0.5 * torch.sum((x - dev(xs)) ** 2)
+    loss.backward()
+    A = fv7_sp(th)
+    xr = spla.spsolve(A.tocsc(), b)
+    lr = spla.spsolve(A.T.tocsc(), xr - xs)
+    gr = fv7_grad_ref(th, xr, lr)
+    rel = np.abs(t.grad.cpu().numpy() - gr).max() / np.abs(gr).max()
+    print(f"iters fwd {opts.last['forward_iters']} adj {opts.last['adjoint_iters']}; rel diff {rel:.2e}")
+    assert rel <= 1e-6

@@ -71,7 +71,7 @@ def test_apply_fp64(eng, orc, idim, smoother, sweeps):
         assert relmax(y, yo) <= 1e-12
 
 
-@pytest.mark.parametrize("fmt", ["auto", "csr", "ell", "sell"])
+@pytest.mark.parametrize("fmt", ["auto", "csr", "ell", "sell", "dia"])
 def test_level_spmv_bitexact(eng, orc, fmt):
     _say(f"[level spmv] fmt={fmt}")
     a = orc.gen_pde3d(20)
@@ -91,7 +91,7 @@ def test_level_spmv_bitexact(eng, orc, fmt):
             yo = m.spmv(x)
             yg = y.cpu().numpy()
             f = p.op_format(lev, what)
-            if f["fmt"] in ("ELL", "SELL") or f["lanes"] == 1:
+            if f["fmt"] in ("ELL", "SELL", "DIA") or f["lanes"] == 1:
                 # one lane per row, stored order, no fused multiply-add:
                 # identical to the sequential CSR loop
                 assert (yg == yo).all(), (lev, what, f)
@@ -116,6 +116,30 @@ def test_level_smoother(eng, orc, smoother, sweeps, beta):
     p.level_smooth(0, dev(x), y, beta=beta, sweeps=sweeps)
     yo = po.smooth(0, x, alpha=1.0, beta=float(beta), y=y0, sweeps=sweeps)
     assert relmax(y.cpu().numpy(), yo) <= 1e-13
+
+
+@pytest.mark.parametrize("idim,beta", [(30, 0), (30, 1), (52, 0)])
+def test_ilu_level_scheduled_bitexact(eng, orc, idim, beta):
+    """ILU(0) smoother on levels large enough for the level-scheduled path
+    (grid launches for large level sets, single-workgroup runs for
+    consecutive small ones, 4 or 16 lanes per row) and on a small coarse
+    level (one workgroup): identical to the sequential substitution, for
+    'N' and for 'T' (column-oriented order)."""
+    _say(f"[ILU level-scheduled] idim={idim} beta={beta}")
+    a = orc.gen_pde3d(idim)
+    A = a.to_scipy()
+    p = engine_prec(eng, A, "BJAC", 1)
+    po = oracle_prec(orc, a, "BJAC", 1)
+    rng = np.random.default_rng(8)
+    for lev in range(min(2, p.nlev - 1)):
+        n = p.level_info(lev)["n"]
+        x = rng.standard_normal(n)
+        y0 = rng.standard_normal(n)
+        for trans in ("N", "T"):
+            y = dev(y0.copy())
+            p.level_smooth(lev, dev(x), y, beta=beta, sweeps=1, trans=trans)
+            yo = po.smooth(lev, x, alpha=1.0, beta=float(beta), y=y0, sweeps=1, trans=trans)
+            assert (y.cpu().numpy() == yo).all(), (lev, trans)
 
 
 @pytest.mark.parametrize("method", ["CG", "FCG", "BICGSTAB"])
