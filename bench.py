@@ -397,6 +397,34 @@ def section_adjoint(eng, torch, problems, perfmodel, idim):
             "note": "setup (host assembly + hierarchy build + upload) is inside forward_s"}
 
 
+def traffic_probe(idim):
+    """Launch sequence for counter runs: 3 copies of 2^27 doubles (1 GiB read
+    + 1 GiB written each), then 3 launches of each fine-level step of the FP64
+    benchmark hierarchy (storage autotuning off: the fine level is DIA
+    regardless, and the tuner's trial launches would swamp the counters)."""
+    import torch
+
+    os.environ["MLD_AUTOTUNE"] = "0"
+    eng = _engine()
+    from mld2p4_amd import problems
+
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    irp, ja, val = problems.pde3d_poisson(idim)
+    p = make_prec(eng, "L1-JACOBI", 2, "double")
+    p.build((irp, ja, val))
+    for kind in ("residual", "pre", "post", "krylov_spmv"):
+        p.time_step(0, kind, 2)  # one warm-up launch sequence + 2 timed
+    for kind in ("restrict", "prolong"):
+        p.time_step(1, kind, 2)
+    torch.cuda.synchronize()
+    log("traffic probe done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,11 +439,17 @@ def main():
     ap.add_argument("--sections", default="run_b,diffusion,batched,adjoint",
                     help="extra sections to run (comma list; empty for none)")
     ap.add_argument("--small", action="store_true", help="small sizes for the extra sections (quick check)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run the kernels whose HBM traffic is counted under rocprofv3 --pmc: a 1 GiB copy "
+                         "(the calibration kernel), then the fine-level steps of the benchmark's hierarchy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box here has one GPU). "
               "The multi-GPU sharding design is described in DESIGN.md.", file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe:
+        traffic_probe(args.idim)
+        return
 
     import torch
 

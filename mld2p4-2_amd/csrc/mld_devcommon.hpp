@@ -108,6 +108,13 @@ struct DevArena {
 
 static inline int nblk(int64_t n) { return (int)((n + BS - 1) / BS); }
 
+// Rows per thread of a DIA product with dot outputs: two rows per thread
+// halve the blocks and their reduction epilogues (Krylov product at 256^3:
+// 265 -> 239 us FP64); plain products keep one row per thread (faster for
+// them: residual 239 vs 247 us).
+template <int K>
+constexpr int dia_rpt() { return K > 0 ? 2 : 1; }
+
 // ---------------------------------------------------------------------------
 // Storage choice (force: 0 automatic, 1 CSR, 2 ELL, 3 SELL).  Automatic:
 // ELL when the widest row is short and padding small (the 7-point fine
@@ -287,7 +294,7 @@ static HCsr host_transpose(const HCsr& a) {
 template <typename V>
 static int spmv_blocks(const DMat<V>& m) {
   if (m.fmt == FMT_CSR) return nblk((int64_t)m.nr * m.lanes);
-  return nblk(m.nr);
+  return nblk(m.nr);  // DIA: an upper bound (products with dots use fewer blocks)
 }
 
 template <int LANES, int K, typename C, typename V, class G, class E>
@@ -310,10 +317,11 @@ template <int K = 0, typename CT = void, typename V, class G, class E>
 static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red = Red{nullptr, nullptr}) {
   using C = typename std::conditional<std::is_void<CT>::value, V, CT>::type;
   if (m.nr == 0) return;
-  const int nb = spmv_blocks(m);
+  constexpr int RPT = dia_rpt<K>();
+  const int nb = (m.fmt == FMT_DIA) ? (int)(((int64_t)m.nr + BS * RPT - 1) / (BS * RPT)) : spmv_blocks(m);
   if (m.fmt == FMT_DIA) {
-    if (m.w == 7) dia_kernel<C, 7, K><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red);
-    else dia_kernel<C, 0, K><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red);
+    if (m.w == 7) dia_kernel<C, 7, K, RPT><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red);
+    else dia_kernel<C, 0, K, RPT><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red);
   } else if (m.fmt == FMT_ELL) {
     switch (m.w) {
       case 4: ell_kernel<C, 4, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
@@ -335,7 +343,7 @@ static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red = Red
       default: csr_launch_l<64, K, C>(m, nb, g, e, s, red); break;
     }
   }
-  if constexpr (K > 0) finalize_kernel<K><<<1, BS, 0, s>>>(nb, red.partial, red.result);
+  if constexpr (K > 0) finalize_kernel<K><<<1, FIN_T, 0, s>>>(nb, red.partial, red.result);
   HIPCHECK(hipGetLastError());
 }
 

@@ -286,8 +286,8 @@ class DevPrecT : public DevPrec {
   // one-block sum of the partials of a streaming reduction over n0_ entries
   void finalize(int K, int slot) {
     const int nb = red_blocks(n0_);
-    if (K == 1) finalize_kernel<1><<<1, BS, 0, s_>>>(nb, partial_, scal_ + slot);
-    else finalize_kernel<2><<<1, BS, 0, s_>>>(nb, partial_, scal_ + slot);
+    if (K == 1) finalize_kernel<1><<<1, FIN_T, 0, s_>>>(nb, partial_, scal_ + slot);
+    else finalize_kernel<2><<<1, FIN_T, 0, s_>>>(nb, partial_, scal_ + slot);
     HIPCHECK(hipGetLastError());
   }
   double* scal_ = nullptr;
@@ -582,7 +582,7 @@ class DevPrecT : public DevPrec {
         const int nb = nblk(n);
         if (dot_u_) {
           vmul_out_kernel<V, 1><<<nb, BS, 0, s_>>>(n, S.dv, x, y, out64_, dot_u_, red(dot_slot_));
-          finalize_kernel<1><<<1, BS, 0, s_>>>(nb, partial_, scal_ + dot_slot_);
+          finalize_kernel<1><<<1, FIN_T, 0, s_>>>(nb, partial_, scal_ + dot_slot_);
         } else {
           vmul_out_kernel<V, 0><<<nb, BS, 0, s_>>>(n, S.dv, x, y, out64_, nullptr, Red{nullptr, nullptr});
         }

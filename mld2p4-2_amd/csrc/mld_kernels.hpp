@@ -151,16 +151,38 @@ __device__ __forceinline__ void block_partials(const double* v, double* partial)
   block_partials<K>(v, partial, blockIdx.x);
 }
 
+// One workgroup of FIN_T threads adds the nb partials of each of K sums:
+// thread t sums entries t, t+FIN_T, t+2*FIN_T, ... in index order, loading 8
+// at a time (independent loads in flight; a plain strided loop was latency
+// bound at ~60 us for 64k partials), then a fixed-order wave / block tree.
+constexpr int FIN_T = 1024;
 template <int K>
-__global__ __launch_bounds__(BS) void finalize_kernel(int nb, const double* __restrict__ partial,
-                                                       double* __restrict__ result) {
-  __shared__ double lds[BS / 64];
+__global__ __launch_bounds__(FIN_T) void finalize_kernel(int nb, const double* __restrict__ partial,
+                                                          double* __restrict__ result) {
+  __shared__ double lds[K][FIN_T / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < K; ++q) {
+    const double* pp = partial + (size_t)q * nb;
+    double acc = 0.0;
+    int b = threadIdx.x;
+    for (; b + 7 * FIN_T < nb; b += 8 * FIN_T) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = pp[b + u * FIN_T];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; b < nb; b += FIN_T) acc += pp[b];
+    acc = wave_sum(acc);
+    if (lane == 0) lds[q][wid] = acc;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < K) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += BS) s += partial[(size_t)q * nb + i];
-    s = block_sum(s, lds);
-    if (threadIdx.x == 0) result[q] = s;
+#pragma unroll
+    for (int w = 0; w < FIN_T / 64; ++w) s += lds[threadIdx.x][w];
+    result[threadIdx.x] = s;
   }
 }
 
@@ -170,11 +192,24 @@ struct Red {  // reduction buffers of a kernel with dot outputs
 };
 
 // Block -> row-block order.  Workgroups are dispatched to the 8 XCDs
-// round-robin in blockIdx order.  Giving each XCD one contiguous run of row
-// blocks (x*q + min(x,r) + i for block 8i+x) was measured slower on the
-// fine-level stencil (residual 326 -> 358 us at 256^3) than the identity,
-// which spreads neighbouring row blocks over all XCDs; the identity is kept.
-__device__ __forceinline__ int xcd_block() { return blockIdx.x; }
+// round-robin in blockIdx order, and each XCD has its own L2.  Row blocks are
+// handed out in chunks of C consecutive blocks (C * rows-per-block = 8192
+// rows), XCD x taking chunks x, x+8, x+16, ...: an XCD's rows then recur with
+// a period of 65536 rows, so the neighbours one plane away in a 256^2-plane
+// grid, and those one or a few rows away, are gathered through the same L2
+// (measured: FP32 fine-level residual 111 -> 96 us at 256^3).  Plain
+// contiguous runs per XCD were slower, and the identity order re-reads the
+// gathered vector through all eight L2s.  Blocks past the last full period
+// keep their index.
+template <int C>
+__device__ __forceinline__ int xcd_block_c() {
+  const int p = blockIdx.x;
+  const int full = (gridDim.x / (8 * C)) * (8 * C);
+  if (p >= full) return p;
+  const int x = p & 7, k = p >> 3;
+  return ((k / C) * 8 + x) * C + (k % C);
+}
+__device__ __forceinline__ int xcd_block() { return xcd_block_c<8192 / BS>(); }
 
 // ----------------------------------------------------------- SpMV kernels
 // ITERS > 0: the first ITERS strided entries of each lane are loaded up front
@@ -255,22 +290,28 @@ __global__ __launch_bounds__(BS) void ell_kernel(int nr, int w, int ld, const in
 struct DiaOff {
   int o[8];
 };
-template <typename V, int ND, int K, class G, class E, typename S>
+template <typename V, int ND, int K, int RPT, class G, class E, typename S>
 __global__ __launch_bounds__(BS) void dia_kernel(int nr, int nc, int nd, DiaOff off, const S* __restrict__ val, G g,
                                                   E e, Red red) {
-  const int lb = xcd_block();
-  const int row = lb * BS + threadIdx.x;
+  const int lb = xcd_block_c<8192 / (BS * RPT)>();
   double dd[2] = {0.0, 0.0};
-  if (row < nr) {
-    V acc = 0;
-    const int ndg = (ND > 0) ? ND : nd;
 #pragma unroll
-    for (int k = 0; k < ((ND > 0) ? ND : 8); ++k) {
-      if (ND == 0 && k >= ndg) break;
-      const int j = min(max(row + off.o[k], 0), nc - 1);
-      acc = acc + (V)ldnt(val + (int64_t)k * nr + row) * g(j);
+  for (int rr = 0; rr < RPT; ++rr) {
+    const int row = (lb * RPT + rr) * BS + threadIdx.x;
+    if (row < nr) {
+      V acc = 0;
+      const int ndg = (ND > 0) ? ND : nd;
+#pragma unroll
+      for (int k = 0; k < ((ND > 0) ? ND : 8); ++k) {
+        if (ND == 0 && k >= ndg) break;
+        const int j = min(max(row + off.o[k], 0), nc - 1);
+        acc = acc + (V)ldnt(val + (int64_t)k * nr + row) * g(j);
+      }
+      double d1[2] = {0.0, 0.0};
+      e(row, acc, d1);
+      dd[0] += d1[0];
+      dd[1] += d1[1];
     }
-    e(row, acc, dd);
   }
   if constexpr (K > 0) block_partials<K>(dd, red.partial, lb);
 }

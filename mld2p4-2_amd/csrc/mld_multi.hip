@@ -208,8 +208,17 @@ __global__ __launch_bounds__(BS) void mfin_kernel(const int* __restrict__ bstart
                                                   double* __restrict__ out) {
   __shared__ double sh[BS];
   const int c = blockIdx.x, s = c / NB, j = c % NB;
+  const int b1 = bstart[s + 1];
   double acc = 0.0;
-  for (int b = bstart[s] + threadIdx.x; b < bstart[s + 1]; b += BS) acc += partial[(int64_t)b * NB + j];
+  int b = bstart[s] + threadIdx.x;
+  for (; b + 7 * BS < b1; b += 8 * BS) {  // 8 independent loads in flight, added in order
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = partial[(int64_t)(b + u * BS) * NB + j];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  for (; b < b1; b += BS) acc += partial[(int64_t)b * NB + j];
   sh[threadIdx.x] = acc;
   __syncthreads();
 #pragma unroll
