@@ -298,8 +298,12 @@ class DevPrecT : public DevPrec {
   double* kr_ = nullptr;
 
   // --- vector plumbing between the FP64 interface and the V-typed cycle
+  // FP64 vector whose FP32 copy the last Krylov update already left in the
+  // fine level's input buffer (FP32 form); null when there is none
+  const double* preconv_ = nullptr;
   const V* in_vec(const double* x, V* buf, int n) {
     if (std::is_same<V, double>::value) return (const V*)x;
+    if (x == preconv_ && buf == L_[0].x2l) return buf;
     convert_kernel<V, double><<<nblk(n), BS, 0, s_>>>(n, x, buf);
     HIPCHECK(hipGetLastError());
     return buf;
@@ -747,6 +751,17 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
   int it = 0;
   if (*err <= eps) { *iter = 0; return 0; }
   const int nbd = red_blocks(n);
+  // FP32 form, untransposed cycle with the plain fine-level input: the
+  // residual update also writes the FP32 input of the next application
+  float* r32 = nullptr;
+  if constexpr (!std::is_same<V, double>::value) {
+    if (trans == 'N' && outer_sweeps_ == 1 && L_.size() > 1) r32 = (float*)L_[0].x2l;
+  }
+  preconv_ = nullptr;
+  struct PreconvReset {
+    const double*& p;
+    ~PreconvReset() { p = nullptr; }
+  } preconv_reset{preconv_};
   if (method == 0) {  // CG
     while (it < itmax) {
       ++it;
@@ -754,7 +769,8 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       if (rc) return rc;
       cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, z, p, scal_, it);
       kspmv<1>(trans, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, red(S_SIGMA));
-      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, (it & 1) ? S_RHO1 : S_RHO0, S_SIGMA, red(S_RR));
+      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, (it & 1) ? S_RHO1 : S_RHO0, S_SIGMA, red(S_RR), r32);
+      preconv_ = r32 ? r : nullptr;
       finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
       read_scal(S_RR);
@@ -772,7 +788,8 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z, p, r, scal_, it, red(S_PR));
       finalize(1, S_PR);
       kspmv<1>(trans, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, red(S_DELTA));
-      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, S_PR, S_DELTA, red(S_RR));
+      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, S_PR, S_DELTA, red(S_RR), r32);
+      preconv_ = r32 ? r : nullptr;
       finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
       read_scal(S_RR);

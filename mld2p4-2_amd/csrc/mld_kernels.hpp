@@ -451,10 +451,12 @@ __global__ __launch_bounds__(BS) void cg_pupdate(int n, const double* __restrict
 
 // x += alpha p; r -= alpha q; sum r_i^2.  alpha = num/den from device slots;
 // den == 0 leaves x and r unchanged (breakdown, detected on the host).
+// r32 (may be null): also writes r in FP32, the input of the next FP32
+// preconditioner application (saves its conversion pass).
 __global__ __launch_bounds__(BS) void xr_update(int n, double* __restrict__ x, double* __restrict__ r,
                                                  const double* __restrict__ p, const double* __restrict__ q,
                                                  const double* __restrict__ scal, int num_slot, int den_slot,
-                                                 Red red) {
+                                                 Red red, float* __restrict__ r32) {
   const double den = scal[den_slot];
   const double alpha = (den != 0.0) ? scal[num_slot] / den : 0.0;
   double s[2] = {0.0, 0.0};
@@ -466,6 +468,7 @@ __global__ __launch_bounds__(BS) void xr_update(int n, double* __restrict__ x, d
       x[i] = alpha * p[i] + x[i];
       const double ri = (-alpha) * q[i] + r[i];
       r[i] = ri;
+      if (r32) r32[i] = (float)ri;
       s[0] += ri * ri;
     }
   }
