@@ -42,6 +42,7 @@ struct DMat {
   int iters = 0;        // CSR: entries per lane loaded up front (0, 2, 4, 8)
   int rpl = 1;          // block products (32 columns): columns per lane (1, 2, 4)
   DiaOff off{};         // DIA: diagonal offsets (w diagonals)
+  int xmap = 1;         // row-block order: 0 identity, 1 chunks of 8192 rows, 2 one range per XCD
   int* irp = nullptr;   // CSR row pointers
   int* sptr = nullptr;  // SELL slice offsets (nslices + 1)
   int* perm = nullptr;  // SELL position -> row
@@ -297,13 +298,22 @@ static int spmv_blocks(const DMat<V>& m) {
   return nblk(m.nr);  // DIA: an upper bound (products with dots use fewer blocks)
 }
 
+// Chunk (in blocks) of the row-block order for a launch of nb blocks of
+// rows_per_block rows (see xcd_block).
+static inline int xcd_chunk(int xmap, int nb, int rows_per_block) {
+  if (xmap == 1) return std::max(1, 8192 / rows_per_block);
+  if (xmap == 2) return std::max(1, nb / 8);
+  return 0;
+}
+
 template <int LANES, int K, typename C, typename V, class G, class E>
 static void csr_launch_l(const DMat<V>& m, int nb, G g, E e, hipStream_t s, Red red) {
+  const int xc = xcd_chunk(m.xmap, nb, BS / LANES);
   switch (m.iters) {
-    case 2: csr_kernel<C, LANES, 2, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
-    case 4: csr_kernel<C, LANES, 4, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
-    case 8: csr_kernel<C, LANES, 8, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
-    default: csr_kernel<C, LANES, 0, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red); break;
+    case 2: csr_kernel<C, LANES, 2, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red, xc); break;
+    case 4: csr_kernel<C, LANES, 4, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red, xc); break;
+    case 8: csr_kernel<C, LANES, 8, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red, xc); break;
+    default: csr_kernel<C, LANES, 0, K><<<nb, BS, 0, s>>>(m.nr, m.irp, m.ja, m.val, g, e, red, xc); break;
   }
 }
 
@@ -320,18 +330,20 @@ static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red = Red
   constexpr int RPT = dia_rpt<K>();
   const int nb = (m.fmt == FMT_DIA) ? (int)(((int64_t)m.nr + BS * RPT - 1) / (BS * RPT)) : spmv_blocks(m);
   if (m.fmt == FMT_DIA) {
-    if (m.w == 7) dia_kernel<C, 7, K, RPT><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red);
-    else dia_kernel<C, 0, K, RPT><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red);
+    const int xc = xcd_chunk(m.xmap, nb, BS * RPT);
+    if (m.w == 7) dia_kernel<C, 7, K, RPT><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red, xc);
+    else dia_kernel<C, 0, K, RPT><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red, xc);
   } else if (m.fmt == FMT_ELL) {
+    const int xc = xcd_chunk(m.xmap, nb, BS);
     switch (m.w) {
-      case 4: ell_kernel<C, 4, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      case 5: ell_kernel<C, 5, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      case 6: ell_kernel<C, 6, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      case 7: ell_kernel<C, 7, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
-      default: ell_kernel<C, 0, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red); break;
+      case 4: ell_kernel<C, 4, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red, xc); break;
+      case 5: ell_kernel<C, 5, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red, xc); break;
+      case 6: ell_kernel<C, 6, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red, xc); break;
+      case 7: ell_kernel<C, 7, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red, xc); break;
+      default: ell_kernel<C, 0, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red, xc); break;
     }
   } else if (m.fmt == FMT_SELL) {
-    sell_kernel<C, K><<<nb, BS, 0, s>>>(m.nr, m.sptr, m.perm, m.ja, m.val, g, e, red);
+    sell_kernel<C, K><<<nb, BS, 0, s>>>(m.nr, m.sptr, m.perm, m.ja, m.val, g, e, red, xcd_chunk(m.xmap, nb, BS));
   } else {
     switch (m.lanes) {
       case 1: csr_launch_l<1, K, C>(m, nb, g, e, s, red); break;
@@ -371,24 +383,27 @@ static bool autotune_enabled() {
   return !(e && e[0] == '0');
 }
 
+// Average time (ms) of y = M x over reps launches, best of three batches.
+template <typename V>
+static float time_spmv(const DMat<V>& m, const V* x, V* y, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  spmv_launch(m, GatherX<V, V>{x}, EpiSet<V, V>{y}, s);  // warm-up
+  float msmin = 1e30f;
+  const int reps = 5;
+  for (int trial = 0; trial < 3; ++trial) {
+    HIPCHECK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) spmv_launch(m, GatherX<V, V>{x}, EpiSet<V, V>{y}, s);
+    HIPCHECK(hipEventRecord(e1, s));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    msmin = std::min(msmin, ms);
+  }
+  return msmin / reps;
+}
+
 template <typename V>
 static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int force) {
   if (force != 0 || h.nr < TUNE_MIN_ROWS || !autotune_enabled()) return upload_mat<V>(h, ar, s, force);
-  // banded operators: DIA reads strictly fewer bytes than any other layout
-  if (dia_eligible(h, dia_offsets(h))) return upload_mat<V>(h, ar, s, 6);
-  int maxrow = 0;
-  for (int i = 0; i < h.nr; ++i) maxrow = std::max(maxrow, (int)(h.irp[i + 1] - h.irp[i]));
-  const double avg = (double)h.nnz() / std::max(1, h.nr);
-  // an almost padding-free ELL is not worth a search
-  if (maxrow <= 16 && (double)h.nr * maxrow <= 1.02 * (double)h.nnz()) return upload_mat<V>(h, ar, s, 2);
-  struct Cand { int force, lanes; bool ordered; };
-  std::vector<Cand> cands;
-  if (maxrow <= 64 && (double)h.nr * maxrow <= 1.25 * (double)h.nnz()) cands.push_back({2, 0, true});
-  cands.push_back({3, 0, true});
-  cands.push_back({4, 0, true});
-  cands.push_back({5, 0, true});
-  for (int l = 1; l <= 64; l *= 2)
-    if (l == 1 || l <= avg) cands.push_back({1, l, l == 1});
   V* x = nullptr;
   V* y = nullptr;
   HIPCHECK(hipMalloc(&x, sizeof(V) * (size_t)std::max(1, h.nc)));
@@ -397,32 +412,55 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
   hipEvent_t e0, e1;
   HIPCHECK(hipEventCreate(&e0));
   HIPCHECK(hipEventCreate(&e1));
-  double best = 1e300;
-  int bi = 0;
-  std::vector<double> tm(cands.size());
-  for (size_t c = 0; c < cands.size(); ++c) {
-    DevArena tmp;
-    DMat<V> m = upload_mat<V>(h, tmp, s, cands[c].force, cands[c].lanes);
-    spmv_launch(m, GatherX<V, V>{x}, EpiSet<V, V>{y}, s);  // warm-up
-    float msmin = 1e30f;
-    const int reps = 5;
-    for (int trial = 0; trial < 3; ++trial) {  // best of three batches
-      HIPCHECK(hipEventRecord(e0, s));
-      for (int r = 0; r < reps; ++r) spmv_launch(m, GatherX<V, V>{x}, EpiSet<V, V>{y}, s);
-      HIPCHECK(hipEventRecord(e1, s));
-      HIPCHECK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-      msmin = std::min(msmin, ms);
+  int bforce = 6, blanes = 0;
+  if (!dia_eligible(h, dia_offsets(h))) {  // DIA reads strictly fewer bytes than any other layout
+    int maxrow = 0;
+    for (int i = 0; i < h.nr; ++i) maxrow = std::max(maxrow, (int)(h.irp[i + 1] - h.irp[i]));
+    const double avg = (double)h.nnz() / std::max(1, h.nr);
+    struct Cand { int force, lanes; bool ordered; };
+    std::vector<Cand> cands;
+    if (maxrow <= 16 && (double)h.nr * maxrow <= 1.02 * (double)h.nnz()) {
+      cands.push_back({2, 0, true});  // an almost padding-free ELL is not worth a search
+    } else {
+      if (maxrow <= 64 && (double)h.nr * maxrow <= 1.25 * (double)h.nnz()) cands.push_back({2, 0, true});
+      cands.push_back({3, 0, true});
+      cands.push_back({4, 0, true});
+      cands.push_back({5, 0, true});
+      for (int l = 1; l <= 64; l *= 2)
+        if (l == 1 || l <= avg) cands.push_back({1, l, l == 1});
     }
-    tm[c] = msmin / reps * (cands[c].ordered ? 1.0 : 1.03);
-    if (tm[c] < best) { best = tm[c]; bi = (int)c; }
+    double best = 1e300;
+    int bi = 0;
+    for (size_t c = 0; c < cands.size() && cands.size() > 1; ++c) {
+      DevArena tmp;
+      DMat<V> m = upload_mat<V>(h, tmp, s, cands[c].force, cands[c].lanes);
+      const double t = time_spmv(m, x, y, s, e0, e1) * (cands[c].ordered ? 1.0 : 1.03);
+      if (t < best) { best = t; bi = (int)c; }
+    }
+    bforce = cands[bi].force;
+    blanes = cands[bi].lanes;
+  }
+  DMat<V> m = upload_mat<V>(h, ar, s, bforce, blanes);
+  // row-block order of the chosen layout: 8192-row chunks per XCD (default),
+  // identity, or one contiguous range per XCD; the default wins ties within 2 %
+  // (MLD_XMAP=0/1/2 fixes it instead)
+  if (const char* e = std::getenv("MLD_XMAP")) {
+    m.xmap = std::max(0, std::min(2, std::atoi(e)));
+  } else {
+    float tb = 1e30f;
+    int xb = 1;
+    for (int xm : {1, 0, 2}) {
+      m.xmap = xm;
+      const float t = time_spmv(m, x, y, s, e0, e1) * (xm == 1 ? 1.0f : 1.02f);
+      if (t < tb) { tb = t; xb = xm; }
+    }
+    m.xmap = xb;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipFree(x);
   (void)hipFree(y);
-  return upload_mat<V>(h, ar, s, cands[bi].force, cands[bi].lanes);
+  return m;
 }
 
 }  // namespace mld

@@ -191,25 +191,24 @@ struct Red {  // reduction buffers of a kernel with dot outputs
   double* result;
 };
 
-// Block -> row-block order.  Workgroups are dispatched to the 8 XCDs
-// round-robin in blockIdx order, and each XCD has its own L2.  Row blocks are
-// handed out in chunks of C consecutive blocks (C * rows-per-block = 8192
-// rows), XCD x taking chunks x, x+8, x+16, ...: an XCD's rows then recur with
-// a period of 65536 rows, so the neighbours one plane away in a 256^2-plane
-// grid, and those one or a few rows away, are gathered through the same L2
-// (measured: FP32 fine-level residual 111 -> 96 us at 256^3).  Plain
-// contiguous runs per XCD were slower, and the identity order re-reads the
-// gathered vector through all eight L2s.  Blocks past the last full period
-// keep their index.
-template <int C>
-__device__ __forceinline__ int xcd_block_c() {
+// Block -> row-block order of a sparse product.  Workgroups reach the 8
+// XCDs round-robin in blockIdx order, and each XCD has its own L2.  With
+// chunk = C > 0, row blocks are handed out in chunks of C consecutive blocks,
+// XCD x taking chunks x, x+8, x+16, ...; blocks past the last full period of
+// 8C keep their index.  chunk = 0 is the identity.  The launcher picks C per
+// operator (autotuned among the identity, 8192-row chunks and one contiguous
+// range per XCD): with 8192-row chunks an XCD's rows recur every 65536 rows,
+// so for a 256^2-plane grid the planes a block gathers sit in that XCD's L2
+// (fine-level traffic 1.20x -> 1.01x of the algorithmic bytes); operators
+// whose rows gather from far apart (restriction) prefer a contiguous range.
+__device__ __forceinline__ int xcd_block(int chunk) {
   const int p = blockIdx.x;
-  const int full = (gridDim.x / (8 * C)) * (8 * C);
+  if (chunk <= 0) return p;
+  const int full = (gridDim.x / (8 * chunk)) * (8 * chunk);
   if (p >= full) return p;
   const int x = p & 7, k = p >> 3;
-  return ((k / C) * 8 + x) * C + (k % C);
+  return ((k / chunk) * 8 + x) * chunk + (k % chunk);
 }
-__device__ __forceinline__ int xcd_block() { return xcd_block_c<8192 / BS>(); }
 
 // ----------------------------------------------------------- SpMV kernels
 // ITERS > 0: the first ITERS strided entries of each lane are loaded up front
@@ -219,8 +218,8 @@ __device__ __forceinline__ int xcd_block() { return xcd_block_c<8192 / BS>(); }
 template <typename V, int LANES, int ITERS, int K, class G, class E, typename S>
 __global__ __launch_bounds__(BS) void csr_kernel(int nr, const int* __restrict__ irp,
                                                   const int* __restrict__ ja, const S* __restrict__ val,
-                                                  G g, E e, Red red) {
-  const int lb = xcd_block();
+                                                  G g, E e, Red red, int xchunk) {
+  const int lb = xcd_block(xchunk);
   const int gid = lb * BS + threadIdx.x;
   const int row = gid / LANES;
   const int lane = gid & (LANES - 1);
@@ -259,8 +258,8 @@ __global__ __launch_bounds__(BS) void csr_kernel(int nr, const int* __restrict__
 
 template <typename V, int W, int K, class G, class E, typename S>
 __global__ __launch_bounds__(BS) void ell_kernel(int nr, int w, int ld, const int* __restrict__ ja,
-                                                  const S* __restrict__ val, G g, E e, Red red) {
-  const int lb = xcd_block();
+                                                  const S* __restrict__ val, G g, E e, Red red, int xchunk) {
+  const int lb = xcd_block(xchunk);
   const int row = lb * BS + threadIdx.x;
   double d[2] = {0.0, 0.0};
   if (row < nr) {
@@ -292,8 +291,8 @@ struct DiaOff {
 };
 template <typename V, int ND, int K, int RPT, class G, class E, typename S>
 __global__ __launch_bounds__(BS) void dia_kernel(int nr, int nc, int nd, DiaOff off, const S* __restrict__ val, G g,
-                                                  E e, Red red) {
-  const int lb = xcd_block_c<8192 / (BS * RPT)>();
+                                                  E e, Red red, int xchunk) {
+  const int lb = xcd_block(xchunk);
   double dd[2] = {0.0, 0.0};
 #pragma unroll
   for (int rr = 0; rr < RPT; ++rr) {
@@ -321,8 +320,8 @@ __global__ __launch_bounds__(BS) void dia_kernel(int nr, int nc, int nd, DiaOff 
 template <typename V, int K, class G, class E, typename S>
 __global__ __launch_bounds__(BS) void sell_kernel(int nr, const int* __restrict__ sptr,
                                                    const int* __restrict__ perm, const int* __restrict__ ja,
-                                                   const S* __restrict__ val, G g, E e, Red red) {
-  const int lb = xcd_block();
+                                                   const S* __restrict__ val, G g, E e, Red red, int xchunk) {
+  const int lb = xcd_block(xchunk);
   const int p = lb * BS + threadIdx.x;
   double d[2] = {0.0, 0.0};
   if (p < nr) {
