@@ -63,6 +63,20 @@ struct BJac {  // t = d_row*(b - ND y);  y = t  or  y = y + t
   }
 };
 
+// Last post-smoothing sweep of the fine level writing the cycle's FP64
+// output directly: yn = yin + d_row*(b - ND y); yout = (double)yn.
+template <typename V>
+struct BJacOut {
+  const V* __restrict__ d;
+  const V* __restrict__ b;
+  const V* __restrict__ yin;
+  double* __restrict__ yout;
+  __device__ __forceinline__ void operator()(int row, int64_t i, V acc) const {
+    const V t = d[row] * (b[i] - acc);
+    yout[i] = (double)(yin[i] + t);
+  }
+};
+
 // Y = A X for row-interleaved X (nc x 32) and Y (nr x 32), CSR A.  A group
 // of 32/RPL lanes owns one row; each lane owns RPL adjacent columns.  The
 // group reads 32/RPL entries of the row at once (one coalesced load) and
@@ -81,6 +95,7 @@ __global__ __launch_bounds__(BS) void bspmm_kernel(int nr, const int* __restrict
   V acc[RPL];
 #pragma unroll
   for (int r = 0; r < RPL; ++r) acc[r] = (V)0;
+  typedef V vec_t __attribute__((ext_vector_type(RPL)));
   for (int kb = k0; kb < k1; kb += LPR) {
     const int kk = kb + lane;
     V v = (V)0;
@@ -90,18 +105,72 @@ __global__ __launch_bounds__(BS) void bspmm_kernel(int nr, const int* __restrict
       c = ldnt(ja + kk);
     }
     const int cnt = min(LPR, k1 - kb);
-    for (int t = 0; t < cnt; ++t) {
-      const V vt = __shfl(v, t, LPR);
-      const int ct = __shfl(c, t, LPR);
-      const V* xp = x + (int64_t)ct * MB + lane * RPL;
-      if constexpr (RPL == 1) {
-        acc[0] = acc[0] + vt * xp[0];
-      } else {
-        typedef V vec_t __attribute__((ext_vector_type(RPL)));
-        const vec_t w = *reinterpret_cast<const vec_t*>(xp);
+    // eight entries at a time: broadcast, then all eight gathers in flight,
+    // then the products added in stored order
 #pragma unroll
-        for (int r = 0; r < RPL; ++r) acc[r] = acc[r] + vt * w[r];
+    for (int t0 = 0; t0 < LPR; t0 += 8) {
+      if (t0 >= cnt) break;
+      V vt[8];
+      vec_t xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        vt[u] = __shfl(v, t0 + u, LPR);
+        const int ct = __shfl(c, t0 + u, LPR);
+        if (t0 + u < cnt) {
+          if constexpr (RPL == 1) xv[u][0] = x[(int64_t)ct * MB + lane];
+          else xv[u] = *reinterpret_cast<const vec_t*>(x + (int64_t)ct * MB + lane * RPL);
+        }
       }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (t0 + u < cnt) {
+#pragma unroll
+          for (int r = 0; r < RPL; ++r) acc[r] = acc[r] + vt[u] * xv[u][r];
+        }
+      }
+    }
+  }
+  const int64_t base = (int64_t)row * MB + lane * RPL;
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) e(row, base + r, acc[r]);
+}
+
+// Y = A X for a DIA operator (see dia_kernel): a lane group per row reads
+// the row's diagonal values (one address per group) and the 32-column rows
+// i + o_d of X, diagonals in ascending offset order (= stored column order).
+template <typename V, int RPL, int ND, class E>
+__global__ __launch_bounds__(BS) void bdia_kernel(int nr, int nc, int nd, DiaOff off, const V* __restrict__ val,
+                                                  const V* __restrict__ x, E e, const V* __restrict__ dx) {
+  constexpr int LPR = MB / RPL;
+  constexpr int RPB = BS / LPR;
+  const int row = blockIdx.x * RPB + threadIdx.x / LPR;
+  const int lane = threadIdx.x % LPR;
+  if (row >= nr) return;
+  typedef V vec_t __attribute__((ext_vector_type(RPL)));
+  V acc[RPL];
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) acc[r] = (V)0;
+  const int ndg = (ND > 0) ? ND : nd;
+#pragma unroll
+  for (int k = 0; k < ((ND > 0) ? ND : 8); ++k) {
+    if (ND == 0 && k >= ndg) break;
+    const int j = min(max(row + off.o[k], 0), nc - 1);
+    const V v = val[(int64_t)k * nr + row];
+    // dx != null: the gathered values are d_j * x_j (the first Jacobi sweep
+    // from zero, formed here instead of in a separate pass; same products)
+    if constexpr (RPL == 1) {
+      V w = x[(int64_t)j * MB + lane];
+      if (dx) w = dx[j] * w;
+      acc[0] = acc[0] + v * w;
+    } else {
+      vec_t w = *reinterpret_cast<const vec_t*>(x + (int64_t)j * MB + lane * RPL);
+      if (dx) {
+        const V dj = dx[j];
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) w[r] = dj * w[r];
+      }
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) acc[r] = acc[r] + v * w[r];
     }
   }
   const int64_t base = (int64_t)row * MB + lane * RPL;
@@ -278,7 +347,7 @@ __global__ __launch_bounds__(BS) void mxr_kernel(const int4* __restrict__ bt, do
                                                  double* __restrict__ r, const double* __restrict__ p,
                                                  const double* __restrict__ q, const double* __restrict__ num,
                                                  const double* __restrict__ den, const int* __restrict__ done,
-                                                 double* __restrict__ partial) {
+                                                 double* __restrict__ partial, float* __restrict__ r32) {
   constexpr int RPS = BS / NB;
   const int4 b = bt[blockIdx.x];
   const int j = threadIdx.x % NB, c = b.z * NB + j;
@@ -292,6 +361,7 @@ __global__ __launch_bounds__(BS) void mxr_kernel(const int4* __restrict__ bt, do
       r[i] = (-a) * q[i] + r[i];
     }
     const double ri = r[i];
+    if (r32) r32[i] = (float)ri;  // FP32 input of the next preconditioner application
     acc += ri * ri;
   }
   col_partial<NB>(acc, partial);
@@ -780,6 +850,10 @@ class MultiT : public MultiEngine {
     to_il(nrhs, x, ldx, kx);
     // r = b - A x, norms, converged-at-start columns
     mv64_res(kx, kb, r);
+    if (float* r32p = r32()) {  // FP32 input of the first application (later ones: the residual update)
+      const int64_t cnt = (int64_t)n0_ * NB;
+      convert64_kernel<float, double><<<nblk64(cnt), BS, 0, s_>>>(cnt, r, r32p);
+    }
     mdot(kb, kb, sc_[C_BN2]);
     mdot(r, r, sc_[C_RR]);
     mstart_kernel<<<1, BS, 0, s_>>>(ncol_, eps, sc_[C_BN2], sc_[C_RR], sc_[C_BN], ic_[I_KIT], ic_[I_DONE],
@@ -875,7 +949,8 @@ class MultiT : public MultiEngine {
     if constexpr (NB == 1) {
       return upload_tuned<V>(h, ar_, s_, 0);
     } else {
-      DMat<V> m = upload_mat<V>(h, ar_, s_, 1, 1);
+      DMat<V> m = upload_mat<V>(h, ar_, s_, dia_eligible(h, dia_offsets(h)) ? 6 : 1, 0);
+      if (m.fmt != FMT_DIA && m.fmt != FMT_CSR) m = upload_mat<V>(h, ar_, s_, 1, 1);
       m.rpl = tune_rpl<V>(m);
       return m;
     }
@@ -884,7 +959,8 @@ class MultiT : public MultiEngine {
     if constexpr (NB == 1) {
       return upload_tuned<double>(h, ar_, s_, 0);
     } else {
-      DMat<double> m = upload_mat<double>(h, ar_, s_, 1, 1);
+      DMat<double> m = upload_mat<double>(h, ar_, s_, dia_eligible(h, dia_offsets(h)) ? 6 : 1, 0);
+      if (m.fmt != FMT_DIA && m.fmt != FMT_CSR) m = upload_mat<double>(h, ar_, s_, 1, 1);
       m.rpl = tune_rpl<double>(m);
       return m;
     }
@@ -897,8 +973,27 @@ class MultiT : public MultiEngine {
   }
 
   template <typename T, class E>
-  void bspmm(const DMat<T>& m, const T* x, E e, int rpl) {
+  void bspmm(const DMat<T>& m, const T* x, E e, int rpl, const T* dx = nullptr) {
     if (m.nr == 0) return;
+    if (m.fmt == FMT_DIA) {
+      const int lpr = MB / rpl, rpb = BS / lpr, nb = (m.nr + rpb - 1) / rpb;
+      switch (rpl) {
+        case 4:
+          if (m.w == 7) bdia_kernel<T, 4, 7, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
+          else bdia_kernel<T, 4, 0, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
+          break;
+        case 2:
+          if (m.w == 7) bdia_kernel<T, 2, 7, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
+          else bdia_kernel<T, 2, 0, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
+          break;
+        default:
+          if (m.w == 7) bdia_kernel<T, 1, 7, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
+          else bdia_kernel<T, 1, 0, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
+          break;
+      }
+      HIPCHECK(hipGetLastError());
+      return;
+    }
     switch (rpl) {
       case 4:
         bspmm_kernel<T, 4, E><<<(m.nr + BS / 8 - 1) / (BS / 8), BS, 0, s_>>>(m.nr, m.irp, m.ja, m.val, x, e);
@@ -965,10 +1060,26 @@ class MultiT : public MultiEngine {
     if constexpr (NB == 1) spmv_launch(m, GatherX<V, V>{x}, EpiRes<V, V>{b, r}, s_);
     else bspmm(m, x, BRes<V>{b, r}, m.rpl);
   }
+  // y (+)= d .* (b - ND x); dx != null: x is gathered as dx_j * x_j (first
+  // sweep from zero, DIA or ELL operators)
   template <bool ADD>
-  void op_jac(const DMat<V>& m, const V* x, const V* d, const V* b, V* y) {
-    if constexpr (NB == 1) spmv_launch(m, GatherX<V, V>{x}, EpiJac<V, V, V, ADD>{d, b, y}, s_);
-    else bspmm(m, x, BJac<V, ADD>{d, b, y}, m.rpl);
+  void op_jac(const DMat<V>& m, const V* x, const V* d, const V* b, V* y, const V* dx = nullptr) {
+    if constexpr (NB == 1) {
+      if (dx) spmv_launch(m, GatherDX<V, V>{dx, x}, EpiJac<V, V, V, ADD>{d, b, y}, s_);
+      else spmv_launch(m, GatherX<V, V>{x}, EpiJac<V, V, V, ADD>{d, b, y}, s_);
+    } else {
+      bspmm(m, x, BJac<V, ADD>{d, b, y}, m.rpl, dx);
+    }
+  }
+  // last post-smoothing sweep of the fine level: FP64 output written directly
+  void op_jac_out(const DMat<V>& m, const V* x, const V* d, const V* b, const V* yin, double* yout,
+                  const V* dx) {
+    if constexpr (NB == 1) {
+      if (dx) spmv_launch(m, GatherDX<V, V>{dx, x}, EpiJacOut<V, V>{d, b, yin, yout, nullptr}, s_);
+      else spmv_launch(m, GatherX<V, V>{x}, EpiJacOut<V, V>{d, b, yin, yout, nullptr}, s_);
+    } else {
+      bspmm(m, x, BJacOut<V>{d, b, yin, yout}, m.rpl, dx);
+    }
   }
   template <bool ADD>
   void vmul(int n, const V* d, const V* x, V* y) {
@@ -997,21 +1108,44 @@ class MultiT : public MultiEngine {
       else vmul<false>(n, dv, x, y);
       return;
     }
-    vmul<false>(n, dv, x, D.t2);
+    // the cycle's FP64 output straight from the fine level's last sweep
+    const bool fo = out64_ != nullptr && beta == 1 && &D == &L_[0] && y == D.y2l;
+    // first sweep from zero: gather d_j x_j where the gathers are cheap (DIA,
+    // and ELL for one column), else form d .* x first
+    const bool dxg = nd.fmt == FMT_DIA || (NB == 1 && nd.fmt == FMT_ELL);
+    const V* xin = x;
+    const V* dxv = dv;
+    if (!dxg) {
+      vmul<false>(n, dv, x, D.t2);
+      xin = D.t2;
+      dxv = nullptr;
+    }
     if (sweeps == 2) {
-      if (beta) op_jac<true>(nd, D.t2, dv, x, y);
-      else op_jac<false>(nd, D.t2, dv, x, y);
+      if (fo) {
+        op_jac_out(nd, xin, dv, x, y, out64_, dxv);
+        out_done_ = true;
+      } else if (beta) {
+        op_jac<true>(nd, xin, dv, x, y, dxv);
+      } else {
+        op_jac<false>(nd, xin, dv, x, y, dxv);
+      }
       return;
     }
     V* ta = D.tx;
     V* tb = D.t1;
-    op_jac<false>(nd, D.t2, dv, x, ta);
+    op_jac<false>(nd, xin, dv, x, ta, dxv);
     for (int i = 2; i <= sweeps - 2; ++i) {
       op_jac<false>(nd, ta, dv, x, tb);
       std::swap(ta, tb);
     }
-    if (beta) op_jac<true>(nd, ta, dv, x, y);
-    else op_jac<false>(nd, ta, dv, x, y);
+    if (fo) {
+      op_jac_out(nd, ta, dv, x, y, out64_, nullptr);
+      out_done_ = true;
+    } else if (beta) {
+      op_jac<true>(nd, ta, dv, x, y);
+    } else {
+      op_jac<false>(nd, ta, dv, x, y);
+    }
   }
 
   // mld_d_inner_mult for every system of the union at once
@@ -1044,8 +1178,10 @@ class MultiT : public MultiEngine {
     if (D.n > D.ni) op_set(D.mc, D.x2l + (size_t)D.ni * NB, D.y2l + (size_t)D.ni * NB);
   }
 
-  // y = M^-1 x, row-interleaved FP64 device vectors of n0 x NB
-  void apply_il(const double* x, double* y) {
+  // y = M^-1 x, row-interleaved FP64 device vectors of n0 x NB.  FP32 form:
+  // preconverted = the fine level's input buffer already holds x in FP32;
+  // the last post-smoothing sweep of the fine level writes y in FP64.
+  void apply_il(const double* x, double* y, bool preconverted = false) {
     MLevel& L0 = L_[0];
     const int64_t cnt = (int64_t)n0_ * NB;
     if constexpr (std::is_same<V, double>::value) {
@@ -1057,11 +1193,22 @@ class MultiT : public MultiEngine {
       L0.x2l = xs;
       L0.y2l = ys;
     } else {
-      convert64_kernel<V, double><<<nblk64(cnt), BS, 0, s_>>>(cnt, x, L0.x2l);
+      if (!preconverted) convert64_kernel<V, double><<<nblk64(cnt), BS, 0, s_>>>(cnt, x, L0.x2l);
+      out64_ = y;
+      out_done_ = false;
       inner(0);
-      convert64_kernel<double, V><<<nblk64(cnt), BS, 0, s_>>>(cnt, L0.y2l, y);
+      out64_ = nullptr;
+      if (!out_done_) convert64_kernel<double, V><<<nblk64(cnt), BS, 0, s_>>>(cnt, L0.y2l, y);
     }
     HIPCHECK(hipGetLastError());
+  }
+  double* out64_ = nullptr;
+  bool out_done_ = false;
+  // FP32 form: where the residual update leaves r in FP32 (the fine level's
+  // input buffer); FP64 form: nowhere (the cycle reads r itself)
+  float* r32() {
+    if constexpr (std::is_same<V, double>::value) return nullptr;
+    else return L_[0].x2l;
   }
 
   // --------------------------------------------------------- layout
@@ -1117,13 +1264,13 @@ class MultiT : public MultiEngine {
     double* p = kv_[4];
     double* q = kv_[5];
     double* x = kv_[1];
-    apply_il(r, z);
+    apply_il(r, z, true);  // r's FP32 copy was written by the previous residual update
     if (method == 0) {  // CG
       mdot(r, z, sc_[C_RHO]);
       mp_cg_kernel<NB><<<nrb_, BS, 0, s_>>>(bt_, z, p, sc_[C_RHO], sc_[C_RHO_OLD], ic_[I_KIT], ic_[I_DONE]);
       mv64_set(p, q);
       mdot(p, q, sc_[C_SIGMA]);
-      mxr_kernel<NB><<<nrb_, BS, 0, s_>>>(bt_, x, r, p, q, sc_[C_RHO], sc_[C_SIGMA], ic_[I_DONE], partial_);
+      mxr_kernel<NB><<<nrb_, BS, 0, s_>>>(bt_, x, r, p, q, sc_[C_RHO], sc_[C_SIGMA], ic_[I_DONE], partial_, r32());
       mfin(sc_[C_RR]);
       mcheck_kernel<<<1, BS, 0, s_>>>(ncol_, eps, itmax, sc_[C_RR], sc_[C_BN], sc_[C_RHO], sc_[C_SIGMA], sc_[C_RHO],
                                       sc_[C_RHO_OLD], ic_[I_KIT], ic_[I_DONE], ic_[I_ITERS], sc_[C_ERR],
@@ -1135,7 +1282,7 @@ class MultiT : public MultiEngine {
       mfin(sc_[C_PR]);
       mv64_set(p, q);
       mdot(p, q, sc_[C_DELTA]);
-      mxr_kernel<NB><<<nrb_, BS, 0, s_>>>(bt_, x, r, p, q, sc_[C_PR], sc_[C_DELTA], ic_[I_DONE], partial_);
+      mxr_kernel<NB><<<nrb_, BS, 0, s_>>>(bt_, x, r, p, q, sc_[C_PR], sc_[C_DELTA], ic_[I_DONE], partial_, r32());
       mfin(sc_[C_RR]);
       mcheck_kernel<<<1, BS, 0, s_>>>(ncol_, eps, itmax, sc_[C_RR], sc_[C_BN], sc_[C_DELTA], nullptr, nullptr,
                                       nullptr, ic_[I_KIT], ic_[I_DONE], ic_[I_ITERS], sc_[C_ERR], ic_[I_ACTIVE]);
