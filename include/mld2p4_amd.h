@@ -102,9 +102,11 @@ int mldhip_krylov_adjoint(mldhip_prec* p, const char* method, const double* b, d
  * A(theta) on the unit cube (idim^3 cells, h = 1/idim, k = exp(theta),
  * harmonic face coefficients, zero Dirichlet faces contributing k_i):
  * grad[c] = -lambda^T (dA/dtheta_c) x.  theta, x, lam, grad: device, length
- * idim^3; enqueued on `stream` (a hipStream_t, NULL = default stream). */
+ * idim^3; work: device, idim^3 doubles of scratch for exp(theta) (one
+ * exponential per cell), or NULL to form the exponentials in place (seven per
+ * cell); enqueued on `stream` (a hipStream_t, NULL = default stream). */
 int mldhip_fv7_grad_theta(int idim, const double* theta, const double* x, const double* lam, double* grad,
-                          void* stream);
+                          double* work, void* stream);
 
 /* ---- batched forms (no reference counterpart: the reference applies its
  * preconditioner to one vector of one system at a time) ----------------------

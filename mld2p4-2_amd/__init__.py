@@ -81,7 +81,7 @@ def lib():
             "mldhip_time_step": [vp, C.c_int, C.c_int, C.c_int, dp],
             "mldhip_get_op_format": [vp, C.c_int, C.c_int, ip, ip, ip],
             "mldhip_krylov_adjoint": [vp, cp, vp, vp, C.c_double, C.c_int, C.c_int, ip, dp],
-            "mldhip_fv7_grad_theta": [C.c_int, vp, vp, vp, vp, vp],
+            "mldhip_fv7_grad_theta": [C.c_int, vp, vp, vp, vp, vp, vp],
             "mldhip_apply_block": [vp, C.c_int, vp, C.c_int64, vp, C.c_int64],
             "mldhip_krylov_block": [vp, cp, C.c_int, vp, C.c_int64, vp, C.c_int64, C.c_double, C.c_int, C.c_int, ip,
                                     dp],
@@ -395,18 +395,23 @@ def psb_krylov_adjoint(method, prec: MldPrec, b, x=None, eps=1e-6, itmax=500, is
     return x, it.value, err.value
 
 
-def fv7_grad_theta(idim, theta, x, lam, grad=None):
+def fv7_grad_theta(idim, theta, x, lam, grad=None, work=True):
     """grad[c] = -lam^T (dA/dtheta_c) x for the 7-point finite-volume
     diffusion operator with k = exp(theta) (device kernel, torch's current
-    stream).  All tensors float64 on the GPU, length idim^3."""
+    stream).  All tensors float64 on the GPU, length idim^3.  work: a scratch
+    tensor for exp(theta), True to allocate one, or None for the variant that
+    forms the exponentials in place."""
     import torch
 
     for t in (theta, x, lam):
         if t.dtype != torch.float64 or not t.is_cuda or t.numel() != idim ** 3 or not t.is_contiguous():
             raise MldError(-1, "fv7_grad_theta: float64 contiguous CUDA tensors of length idim^3 expected")
     grad = torch.empty_like(theta) if grad is None else grad
+    if work is True:
+        work = torch.empty_like(theta)
     s = torch.cuda.current_stream(theta.device).cuda_stream
-    rc = lib().mldhip_fv7_grad_theta(idim, _ptr(theta), _ptr(x), _ptr(lam), _ptr(grad), C.c_void_p(s))
+    rc = lib().mldhip_fv7_grad_theta(idim, _ptr(theta), _ptr(x), _ptr(lam), _ptr(grad),
+                                     None if work is None else _ptr(work), C.c_void_p(s))
     if rc:
         raise MldError(rc, "fv7_grad_theta")
     return grad

@@ -20,45 +20,76 @@
 
 namespace {
 
+// KFROM = true: k[] already holds exp(theta) (one exponential per cell
+// instead of seven: the kernel is bound by FP64 exp otherwise).
+template <bool KFROM>
 __global__ __launch_bounds__(256) void fv7_grad_theta_kernel(int idim, double inv_h2, const double* __restrict__ th,
                                                              const double* __restrict__ x,
                                                              const double* __restrict__ lam,
                                                              double* __restrict__ grad) {
-  const long n = (long)idim * idim * idim;
-  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  // 32-bit index arithmetic (idim^3 < 2^31 is checked by the launcher):
+  // 64-bit division by a runtime value is a long software sequence
+  const unsigned n = (unsigned)idim * idim * idim;
+  const unsigned c = blockIdx.x * 256u + threadIdx.x;
   if (c >= n) return;
-  const int iz = (int)(c % idim), iy = (int)((c / idim) % idim), ix = (int)(c / ((long)idim * idim));
-  const double kc = exp(th[c]), xc = x[c], lc = lam[c];
-  double s = 0.0;
-  const long strides[3] = {(long)idim * idim, (long)idim, 1};
+  const unsigned ud = (unsigned)idim, line = c / ud;
+  const int iz = (int)(c - line * ud), ix = (int)(line / ud), iy = (int)(line - (unsigned)ix * ud);
+  // all six neighbours' values are loaded up front from clamped indices
+  // (branch-free), so the 18 gathers of a thread are in flight together
   const int coord[3] = {ix, iy, iz};
+  const int strides[3] = {idim * idim, idim, 1};
+  unsigned jj[6];
+  bool in[6];
 #pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
+  for (int f = 0; f < 6; ++f) {
+    const int ax = f >> 1, sg = (f & 1) ? 1 : -1;
+    const int q = coord[ax] + sg;
+    in[f] = (q >= 0 && q < idim);
+    jj[f] = in[f] ? c + sg * strides[ax] : c;
+  }
+  double tj[6], lj[6], xj[6];
 #pragma unroll
-    for (int sg = -1; sg <= 1; sg += 2) {
-      const int q = coord[ax] + sg;
-      if (q >= 0 && q < idim) {
-        const long j = c + sg * strides[ax];
-        const double kj = exp(th[j]);
-        const double den = kc + kj;
-        const double dcf = 2.0 * kc * kj * kj / (den * den);
-        s += dcf * (lc - lam[j]) * (xc - x[j]);
-      } else {
-        s += kc * lc * xc;
-      }
+  for (int f = 0; f < 6; ++f) {
+    tj[f] = th[jj[f]];
+    lj[f] = lam[jj[f]];
+    xj[f] = x[jj[f]];
+  }
+  const double kc = KFROM ? th[c] : exp(th[c]);
+  const double xc = x[c], lc = lam[c];
+  double s = 0.0;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    if (in[f]) {
+      const double kj = KFROM ? tj[f] : exp(tj[f]);
+      const double den = kc + kj;
+      const double dcf = 2.0 * kc * kj * kj / (den * den);
+      s += dcf * (lc - lj[f]) * (xc - xj[f]);
+    } else {
+      s += kc * lc * xc;
     }
   }
   grad[c] = -s * inv_h2;
 }
 
+__global__ __launch_bounds__(256) void exp_kernel(long n, const double* __restrict__ th, double* __restrict__ k) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) k[i] = exp(th[i]);
+}
+
 }  // namespace
 
 extern "C" int mldhip_fv7_grad_theta(int idim, const double* theta, const double* x, const double* lam,
-                                     double* grad, void* stream) {
-  if (idim <= 0 || !theta || !x || !lam || !grad) return -1;
+                                     double* grad, double* work, void* stream) {
+  if (idim <= 0 || idim > 1290 || !theta || !x || !lam || !grad) return -1;
   const long n = (long)idim * idim * idim;
   const double inv_h2 = (double)idim * (double)idim;
   const int nb = (int)((n + 255) / 256);
-  fv7_grad_theta_kernel<<<nb, 256, 0, (hipStream_t)stream>>>(idim, inv_h2, theta, x, lam, grad);
+  hipStream_t s = (hipStream_t)stream;
+  if (work) {
+    exp_kernel<<<nb, 256, 0, s>>>(n, theta, work);
+    fv7_grad_theta_kernel<true><<<nb, 256, 0, s>>>(idim, inv_h2, work, x, lam, grad);
+  } else {
+    fv7_grad_theta_kernel<false><<<nb, 256, 0, s>>>(idim, inv_h2, theta, x, lam, grad);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -30;
 }

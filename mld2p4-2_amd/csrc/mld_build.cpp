@@ -426,8 +426,34 @@ static int build_smoother(const HCsr& a, const SmootherSpec& spec, SmootherData&
         for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) sd.lu[(size_t)i * n + a.ja[k]] += a.val[k];
       return 0;
     }
+    case GS:
+    case BWGS: {
+      // mld_d_gs_solver_bld.f90 / mld_d_bwgs_solver_bld.f90 (a%tril / a%triu
+      // with the diagonal on the solved side): kept as the strict lower part
+      // l, the strict upper part u and the diagonal dv (not inverted: the
+      // substitution divides by it).  ND (columns beyond the local rows) is
+      // empty on one device.
+      sd.nnz_nd = 0;
+      sd.l.nr = sd.l.nc = sd.u.nr = sd.u.nc = n;
+      sd.l.irp.assign((size_t)n + 1, 0);
+      sd.u.irp.assign((size_t)n + 1, 0);
+      sd.dv.assign(n, 0.0);
+      for (int i = 0; i < n; ++i) {
+        for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
+          const int j = a.ja[k];
+          if (j < i) { sd.l.ja.push_back(j); sd.l.val.push_back(a.val[k]); }
+          else if (j > i) { sd.u.ja.push_back(j); sd.u.val.push_back(a.val[k]); }
+          else sd.dv[i] = sd.dv[i] + a.val[k];
+        }
+        sd.l.irp[i + 1] = (int64_t)sd.l.ja.size();
+        sd.u.irp[i + 1] = (int64_t)sd.u.ja.size();
+      }
+      level_schedule(sd.l, true, sd.lsched_ptr, sd.lsched_rows);
+      level_schedule(sd.u, false, sd.usched_ptr, sd.usched_rows);
+      return 0;
+    }
     default:
-      return E_UNSUPPORTED;  // GS/BWGS/MILU/ILUT and third-party solvers
+      return E_UNSUPPORTED;  // MILU/ILUT and third-party solvers
   }
 }
 

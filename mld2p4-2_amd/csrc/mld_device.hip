@@ -123,6 +123,25 @@ class DevPrecT : public DevPrec {
       const int nsw = std::max(L0.sweeps_pre, L0.sweeps_post);
       const V* xin = in_vec(x, L0.x2l, L0.n);
       V* yout = out_vec(y, L0.y2l);
+      if (L0.has_sm2) {
+        // symmetrized pair (mld_dprecaply.f90:378-412): w1 = x; nsw times
+        // w2 = S1 w1, w1 = S2 w2; y = w1 (S1, S2 swapped for 'T')
+        DSmoother<V>& s1 = (trans == 'N') ? L0.sm : L0.sm2;
+        DSmoother<V>& s2 = (trans == 'N') ? L0.sm2 : L0.sm;
+        // w1 lives in the output, w2 in ty (the smoothers' own scratch is
+        // tx, t1, t2)
+        V* w1 = yout;
+        V* w2 = L0.ty;
+        add_kernel<V, V, false><<<nblk(L0.n), BS, 0, s_>>>(L0.n, xin, w1);
+        for (int k = 0; k < nsw; ++k) {
+          int rc = smooth(L0, s1, w1, 0, w2, 1, trans);
+          if (rc) return rc;
+          rc = smooth(L0, s2, w2, 0, w1, 1, trans);
+          if (rc) return rc;
+        }
+        finish_out(y, yout, L0.n);
+        return 0;
+      }
       int rc = smooth(L0, L0.sm, xin, 0, yout, nsw, trans);
       if (rc) return rc;
       finish_out(y, yout, L0.n);
@@ -329,7 +348,7 @@ class DevPrecT : public DevPrec {
       HIPCHECK(hipStreamSynchronize(s_));
     }
     if (H.nnz_nd > 0) S.nd = upload_tuned<V>(H.nd, ar_, s_, force);
-    if (H.solver == ILU_N) {
+    if (H.solver == ILU_N || H.solver == GS || H.solver == BWGS) {
       S.l = upload_mat<V>(H.l, ar_, s_, 1, 1);  // CSR (rows summed in stored order)
       S.u = upload_mat<V>(H.u, ar_, s_, 1, 1);
       {
@@ -502,13 +521,13 @@ class DevPrecT : public DevPrec {
       int le = lv;
       while (le < nl1 && p1[le + 1] - p1[le] <= RUN_ROWS) ++le;
       if (le - lv >= 2) {
-        trsv_run_kernel<V, false, 2, TR, LN><<<1, 1024, 0, s_>>>(lv, le, p1d, r1, F1.irp, F1.ja, F1.val, x,
+        trsv_run_kernel<V, 0, 2, TR, LN><<<1, 1024, 0, s_>>>(lv, le, p1d, r1, F1.irp, F1.ja, F1.val, x,
                                                                 nullptr, t1, nullptr);
         lv = le;
         continue;
       }
       const int cnt = p1[lv + 1] - p1[lv];
-      trsv_level_kernel<V, false, 2, TR, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r1 + p1[lv], F1.irp, F1.ja, F1.val, x,
+      trsv_level_kernel<V, 0, 2, TR, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r1 + p1[lv], F1.irp, F1.ja, F1.val, x,
                                                                      nullptr, t1, nullptr);
       ++lv;
     }
@@ -517,24 +536,95 @@ class DevPrecT : public DevPrec {
       while (le < nl2 && p2[le + 1] - p2[le] <= RUN_ROWS) ++le;
       if (le - lv >= 2) {
         if (beta)
-          trsv_run_kernel<V, true, 1, TR, LN><<<1, 1024, 0, s_>>>(lv, le, p2d, r2, F2.irp, F2.ja, F2.val, t1, S.dv,
+          trsv_run_kernel<V, 1, 1, TR, LN><<<1, 1024, 0, s_>>>(lv, le, p2d, r2, F2.irp, F2.ja, F2.val, t1, S.dv,
                                                                  t2, y);
         else
-          trsv_run_kernel<V, true, 2, TR, LN><<<1, 1024, 0, s_>>>(lv, le, p2d, r2, F2.irp, F2.ja, F2.val, t1, S.dv,
+          trsv_run_kernel<V, 1, 2, TR, LN><<<1, 1024, 0, s_>>>(lv, le, p2d, r2, F2.irp, F2.ja, F2.val, t1, S.dv,
                                                                  t2, y);
         lv = le;
         continue;
       }
       const int cnt = p2[lv + 1] - p2[lv];
       if (beta)
-        trsv_level_kernel<V, true, 1, TR, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r2 + p2[lv], F2.irp, F2.ja, F2.val, t1,
+        trsv_level_kernel<V, 1, 1, TR, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r2 + p2[lv], F2.irp, F2.ja, F2.val, t1,
                                                                       S.dv, t2, y);
       else
-        trsv_level_kernel<V, true, 2, TR, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r2 + p2[lv], F2.irp, F2.ja, F2.val, t1,
+        trsv_level_kernel<V, 1, 2, TR, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r2 + p2[lv], F2.irp, F2.ja, F2.val, t1,
                                                                       S.dv, t2, y);
       ++lv;
     }
     HIPCHECK(hipGetLastError());
+  }
+
+  // Hybrid Gauss-Seidel smoother (jacobi smoother + GS / BWGS solver, which
+  // is iterative: mld_d_jac_smoother_apply_vect.f90:117-177 always runs the
+  // sweep loop; ND is empty on one device): `sweeps` sweeps from zero,
+  //   GS:   y <- (D + L)^-1 (x - U y),   BWGS: y <- (D + U)^-1 (x - L y)
+  // (mld_d_gs_solver_apply_vect.f90:150-156, mld_d_bwgs_solver_apply_vect.f90
+  // :151-155).  The first sweep's product with the zero iterate is skipped
+  // (x - 0 = x exactly).  Only 'N', as in the reference solvers.
+  int gs_apply(DLevel<V>& D, DSmoother<V>& S, const V* x, int beta, V* y, int sweeps, char trans) {
+    if (trans != 'N') return E_UNSUPPORTED;
+    if (sweeps == 0) {
+      if (!beta) fill_kernel<V><<<nblk(D.n), BS, 0, s_>>>(D.n, y, (V)0);
+      return 0;
+    }
+    const bool fw = (S.solver == GS);
+    const DMat<V>& O = fw ? S.u : S.l;
+    V* t = D.tx;
+    for (int sw = 1; sw <= sweeps; ++sw) {
+      const V* b = x;
+      if (sw > 1) {
+        spmv_launch(O, GatherX<V, V>{t}, EpiRes<V, V>{x, D.t1}, s_);  // w = x - O t
+        b = D.t1;
+      }
+      const int bt = (sw < sweeps) ? 2 : (beta ? 1 : 0);
+      switch (S.tri_lanes) {
+        case 1: tri_solve<1>(D, S, fw, b, t, y, bt); break;
+        case 4: tri_solve<4>(D, S, fw, b, t, y, bt); break;
+        case 16: tri_solve<16>(D, S, fw, b, t, y, bt); break;
+        default: tri_solve<32>(D, S, fw, b, t, y, bt); break;
+      }
+    }
+    HIPCHECK(hipGetLastError());
+    return 0;
+  }
+  // t = (D + T)^-1 b with T the strict lower (lower = true) or upper part:
+  // level-scheduled, division by the stored diagonal; bt: 0 also y = t,
+  // 1 y = y + t, 2 t only.
+  template <int LN>
+  void tri_solve(DLevel<V>& D, DSmoother<V>& S, bool lower, const V* b, V* t, V* y, int bt) {
+    const DMat<V>& T = lower ? S.l : S.u;
+    const std::vector<int>& ph = lower ? S.lptr : S.uptr;
+    const int* rows = lower ? S.lrows : S.urows;
+    const int* pd = lower ? S.lptr_d : S.uptr_d;
+    const int nl = (int)ph.size() - 1;
+    auto run = [&](int l0, int l1) {
+      if (bt == 0) trsv_run_kernel<V, 2, 0, false, LN><<<1, 1024, 0, s_>>>(l0, l1, pd, rows, T.irp, T.ja, T.val, b, S.dv, t, y);
+      else if (bt == 1) trsv_run_kernel<V, 2, 1, false, LN><<<1, 1024, 0, s_>>>(l0, l1, pd, rows, T.irp, T.ja, T.val, b, S.dv, t, y);
+      else trsv_run_kernel<V, 2, 2, false, LN><<<1, 1024, 0, s_>>>(l0, l1, pd, rows, T.irp, T.ja, T.val, b, S.dv, t, y);
+    };
+    if (D.n <= 8192) {  // whole solve in one workgroup
+      run(0, nl);
+      return;
+    }
+    constexpr int RUN_ROWS = 2048 / LN;
+    auto nbl = [](int cnt) { return (int)(((int64_t)cnt * LN + BS - 1) / BS); };
+    for (int lv = 0; lv < nl;) {
+      int le = lv;
+      while (le < nl && ph[le + 1] - ph[le] <= RUN_ROWS) ++le;
+      if (le - lv >= 2) {
+        run(lv, le);
+        lv = le;
+        continue;
+      }
+      const int cnt = ph[lv + 1] - ph[lv];
+      const int* r = rows + ph[lv];
+      if (bt == 0) trsv_level_kernel<V, 2, 0, false, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r, T.irp, T.ja, T.val, b, S.dv, t, y);
+      else if (bt == 1) trsv_level_kernel<V, 2, 1, false, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r, T.irp, T.ja, T.val, b, S.dv, t, y);
+      else trsv_level_kernel<V, 2, 2, false, LN><<<nbl(cnt), BS, 0, s_>>>(cnt, r, T.irp, T.ja, T.val, b, S.dv, t, y);
+      ++lv;
+    }
   }
 
   // One Jacobi sweep y_out = D^-1 (b - ND y_in) (FIRST: y_in = D^-1 b).
@@ -579,6 +669,7 @@ class DevPrecT : public DevPrec {
       return 0;
     }
     if (sweeps < 0) return E_BAD_ARG;
+    if (S.solver == GS || S.solver == BWGS) return gs_apply(D, S, x, beta, y, sweeps, trans);
     const bool jac = (S.solver == DIAG_SCALE || S.solver == L1_DIAG);
     const bool fo = jac && fuse_out(D, beta, y);
     if (sweeps <= 1 || S.nnz_nd == 0) {

@@ -494,25 +494,29 @@ __global__ __launch_bounds__(BS) void fcg_pupdate(int n, const double* __restric
 }
 
 // ---------------------------------------------------- triangular solves
-// One row of a unit triangular solve by a group of LANES lanes.  b_i is the
-// right-hand side (times d_i for the second solve of ILU, the D scaling of
-// psb_spsm).  The lanes form the products f_ij * t_j of LANES stored entries
-// at once (independent loads in flight together); every lane then adds them
-// up in the sequential order with cross-lane reads, so the result is the
-// sequential loop's:
-//   TR = false (L, U as stored): t_i = b_i - (f_ij1 t_j1 + f_ij2 t_j2 + ...);
-//   TR = true (rows of U^T / L^T, the transposed apply): products subtracted
-//     one at a time in the order of the column-oriented sweep over the
-//     untransposed factor (U^T: ascending, L^T: descending column index),
-//     t_i = (...((b_i - f_ij1 t_j1) - f_ij2 t_j2) ...).
-template <typename V, bool UPPER, bool TR, int LANES>
+// One row of a triangular solve by a group of LANES lanes, off-diagonal
+// entries f_ij of the row in CSR.  SC selects the diagonal treatment:
+//   SC = 0: unit diagonal,          t_i = b_i - acc;
+//   SC = 1: scaled right-hand side, t_i = d_i * b_i - acc (ILU's second solve,
+//           the D scaling of psb_spsm);
+//   SC = 2: stored diagonal d_i,    t_i = (b_i - acc) / d_i (Gauss-Seidel).
+// The lanes form the products f_ij * t_j of LANES stored entries at once
+// (independent loads in flight together); every lane then adds them up in the
+// sequential order with cross-lane reads, so the result is the sequential
+// loop's:
+//   TR = false: acc = f_ij1 t_j1 + f_ij2 t_j2 + ... (stored order);
+//   TR = true (rows of U^T / L^T, the transposed ILU apply): products
+//     subtracted one at a time in the order of the column-oriented sweep over
+//     the untransposed factor (U^T, SC = 0: ascending; L^T, SC = 1:
+//     descending column index), t_i = (...((b'_i - f_ij1 t_j1) - f_ij2 t_j2)...).
+template <typename V, int SC, bool TR, int LANES>
 __device__ __forceinline__ V tri_row(int i, int lane, const int* __restrict__ irp, const int* __restrict__ ja,
                                      const V* __restrict__ val, const V* __restrict__ b,
                                      const V* __restrict__ d, const V* t) {
   const int k0 = irp[i], k1 = irp[i + 1];
-  const V bi = UPPER ? d[i] * b[i] : b[i];
+  const V bi = (SC == 1) ? d[i] * b[i] : b[i];
   V acc = TR ? bi : (V)0;
-  if constexpr (!(TR && UPPER)) {
+  if constexpr (!(TR && SC == 1)) {
     for (int kb = k0; kb < k1; kb += LANES) {
       const int k = kb + lane;
       V pk = (V)0;
@@ -540,12 +544,14 @@ __device__ __forceinline__ V tri_row(int i, int lane, const int* __restrict__ ir
       }
     }
   }
-  return TR ? acc : bi - acc;
+  if constexpr (TR) return acc;
+  if constexpr (SC == 2) return (bi - acc) / d[i];
+  return bi - acc;
 }
 
 // The rows of one level set, LANES lanes per row.  BETA: 0 also writes
-// y = t, 1 writes y = y + t, 2 writes t only (y only in the second solve).
-template <typename V, bool UPPER, int BETA, bool TR, int LANES>
+// y = t, 1 writes y = y + t, 2 writes t only.
+template <typename V, int SC, int BETA, bool TR, int LANES>
 __global__ __launch_bounds__(BS) void trsv_level_kernel(int cnt, const int* __restrict__ rows,
                                                          const int* __restrict__ irp, const int* __restrict__ ja,
                                                          const V* __restrict__ val, const V* __restrict__ b,
@@ -554,20 +560,18 @@ __global__ __launch_bounds__(BS) void trsv_level_kernel(int cnt, const int* __re
   const int idx = gid / LANES, lane = gid % LANES;
   if (idx >= cnt) return;  // whole lane groups leave together
   const int i = rows[idx];
-  const V ti = tri_row<V, UPPER, TR, LANES>(i, lane, irp, ja, val, b, d, t);
+  const V ti = tri_row<V, SC, TR, LANES>(i, lane, irp, ja, val, b, d, t);
   if (lane == 0) {
     t[i] = ti;
-    if (UPPER) {
-      if (BETA == 0) y[i] = ti;
-      else if (BETA == 1) y[i] = y[i] + ti;
-    }
+    if (BETA == 0) y[i] = ti;
+    else if (BETA == 1) y[i] = y[i] + ti;
   }
 }
 
-// A run of consecutive small level sets [lv0, lv1) of one triangular solve
-// in one workgroup, with a barrier between level sets; per row the same
-// arithmetic as trsv_level_kernel.  Replaces one launch per small level set.
-template <typename V, bool UPPER, int BETA, bool TR, int LANES>
+// A run of consecutive level sets [lv0, lv1) of one triangular solve in one
+// workgroup, with a barrier between level sets; per row the same arithmetic
+// as trsv_level_kernel.  Replaces one launch per small level set.
+template <typename V, int SC, int BETA, bool TR, int LANES>
 __global__ __launch_bounds__(1024) void trsv_run_kernel(int lv0, int lv1, const int* __restrict__ ptr,
                                                          const int* __restrict__ rows, const int* __restrict__ irp,
                                                          const int* __restrict__ ja, const V* __restrict__ val,
@@ -577,13 +581,11 @@ __global__ __launch_bounds__(1024) void trsv_run_kernel(int lv0, int lv1, const 
   for (int lv = lv0; lv < lv1; ++lv) {
     for (int q = ptr[lv] + grp; q < ptr[lv + 1]; q += ngrp) {
       const int i = rows[q];
-      const V ti = tri_row<V, UPPER, TR, LANES>(i, lane, irp, ja, val, b, d, t);
+      const V ti = tri_row<V, SC, TR, LANES>(i, lane, irp, ja, val, b, d, t);
       if (lane == 0) {
         t[i] = ti;
-        if (UPPER) {
-          if (BETA == 0) y[i] = ti;
-          else if (BETA == 1) y[i] = y[i] + ti;
-        }
+        if (BETA == 0) y[i] = ti;
+        else if (BETA == 1) y[i] = y[i] + ti;
       }
     }
     __syncthreads();
@@ -603,7 +605,7 @@ __global__ __launch_bounds__(1024) void ilu_small_kernel(
   for (int lv = 0; lv < nlevl; ++lv) {
     for (int q = lptr[lv] + grp; q < lptr[lv + 1]; q += ngrp) {
       const int i = lrows[q];
-      const V ti = tri_row<V, false, TR, LANES>(i, lane, lirp, lja, lval, x, nullptr, t1);
+      const V ti = tri_row<V, 0, TR, LANES>(i, lane, lirp, lja, lval, x, nullptr, t1);
       if (lane == 0) t1[i] = ti;
     }
     __syncthreads();
@@ -611,7 +613,7 @@ __global__ __launch_bounds__(1024) void ilu_small_kernel(
   for (int lv = 0; lv < nlevu; ++lv) {
     for (int q = uptr[lv] + grp; q < uptr[lv + 1]; q += ngrp) {
       const int i = urows[q];
-      const V ti = tri_row<V, true, TR, LANES>(i, lane, uirp, uja, uval, t1, d, t2);
+      const V ti = tri_row<V, 1, TR, LANES>(i, lane, uirp, uja, uval, t1, d, t2);
       if (lane == 0) {
         t2[i] = ti;
         if (BETA == 0) y[i] = ti;

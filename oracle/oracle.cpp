@@ -60,8 +60,11 @@ enum {
   SM_NOPREC = 0,   // mld_noprec_
   SM_JAC = 1,      // mld_jac_   (jacobi smoother + diag solver)
   SM_BJAC = 2,     // mld_bjac_  (jacobi smoother + ILU(0) solver)
+  SM_FBGS = 4,     // mld_fbgs_  (pre: jacobi smoother + GS solver, post: + BWGS solver)
   SM_L1JAC = 100,  // l1-Jacobi (Baker, Falgout, Kolev, Yang, SISC 33(5) 2011)
-  SM_DENSELU = 101 // dense LU of the coarsest matrix (coarse solver only)
+  SM_DENSELU = 101, // dense LU of the coarsest matrix (coarse solver only)
+  SM_GS = 102,     // jacobi smoother + forward Gauss-Seidel solver (mld_d_gs_solver)
+  SM_BWGS = 103    // jacobi smoother + backward Gauss-Seidel solver (mld_d_bwgs_solver)
 };
 enum { CYC_V = 3, CYC_W = 4 };  // mld_vcycle_ml_, mld_wcycle_ml_
 
@@ -537,7 +540,9 @@ struct Smoother {
   std::vector<double> dv;  // diag solver: 1/diag (0 -> 1); ILU: 1/pivot
   Csr l, u;                // ILU(0) factors
   DenseLU dlu;
-  bool iterative() const { return false; }  // diag, ILU, LU are non-iterative
+  // GS and BWGS are iterative (mld_d_gs_solver.f90:586-591): the jacobi
+  // smoother then always runs its sweep loop; diag, ILU and LU are not.
+  bool iterative() const { return kind == SM_GS || kind == SM_BWGS; }
 
   int build(const Csr& a) {
     ix n = a.nr;
@@ -588,7 +593,47 @@ struct Smoother {
       nd.nr = nd.nc = n; nd.irp.assign(n + 1, 0); nnz_nd = 0;
       return dlu.factor(a);
     }
+    if (kind == SM_GS || kind == SM_BWGS) {
+      // mld_d_gs_solver_bld.f90 / mld_d_bwgs_solver_bld.f90: A split into its
+      // triangle with the diagonal (solved) and the strict other triangle;
+      // kept here as strict lower l, strict upper u and the diagonal dv.
+      // ND (csclip jmin = nrow+1) is empty on one process.
+      nd.nr = nd.nc = n; nd.irp.assign(n + 1, 0); nnz_nd = 0;
+      l.nr = l.nc = u.nr = u.nc = n;
+      l.irp.assign(n + 1, 0); u.irp.assign(n + 1, 0);
+      l.ja.clear(); l.val.clear(); u.ja.clear(); u.val.clear();
+      dv.assign(n, 0.0);
+      for (ix i = 0; i < n; ++i) {
+        for (ix k = a.irp[i]; k < a.irp[i + 1]; ++k) {
+          if (a.ja[k] < i) { l.ja.push_back(a.ja[k]); l.val.push_back(a.val[k]); }
+          else if (a.ja[k] > i) { u.ja.push_back(a.ja[k]); u.val.push_back(a.val[k]); }
+          else dv[i] = dv[i] + a.val[k];
+        }
+        l.irp[i + 1] = (ix)l.ja.size();
+        u.irp[i + 1] = (ix)u.ja.size();
+      }
+      return 0;
+    }
     return -1;
+  }
+
+  // One sweep of the GS solver (mld_d_gs_solver_apply_vect.f90:150-156):
+  // wv = b - O xit (O the strict other triangle); xit = T^-1 wv, T the
+  // triangle with the diagonal, non-unit substitution with the off-diagonal
+  // products summed in stored order, then divided by the diagonal.
+  // GS: T lower, rows ascending; BWGS: T upper, rows descending.
+  void gs_sweep(const double* b, double* xit, ix n) const {
+    const bool fw = (kind == SM_GS);
+    const Csr& tri = fw ? l : u;
+    const Csr& oth = fw ? u : l;
+    std::vector<double> w(b, b + n);
+    spmv(oth, -1.0, xit, 1.0, w.data());
+    for (ix q = 0; q < n; ++q) {
+      const ix i = fw ? q : n - 1 - q;
+      double acc = 0.0;
+      for (ix k = tri.irp[i]; k < tri.irp[i + 1]; ++k) acc = acc + tri.val[k] * xit[tri.ja[k]];
+      xit[i] = (w[i] - acc) / dv[i];
+    }
   }
 
   // Solver apply: y = alpha*S^-1 x + beta*y.
@@ -618,6 +663,17 @@ struct Smoother {
   // (mld_d_base_smoother_apply_vect.f90:60-66 for NOPREC).
   int apply(double alpha, const double* x, double beta, double* y, char trans, int sweeps, ix n) const {
     if (kind == SM_NOPREC) { axpby(n, alpha, x, beta, y); return 0; }
+    if (iterative()) {
+      // only 'N' (mld_d_gs_solver_apply_vect.f90:81-87); the sweep loop of
+      // :133-175 with init 'Z' then 'Y' and an empty ND: every smoother sweep
+      // is one GS sweep (solver sweeps = 1) on x, starting from zero
+      if (trans != 'N') return -3;
+      if (sweeps < 0) return -2;
+      std::vector<double> ty(n, 0.0);
+      for (int i = 0; i < sweeps; ++i) gs_sweep(x, ty.data(), n);
+      axpby(n, alpha, ty.data(), beta, y);
+      return 0;
+    }
     if (!iterative() && (sweeps == 1 || nnz_nd == 0)) {  // :107-109
       sv_apply(alpha, x, beta, y, trans, n);
       return 0;
@@ -644,6 +700,8 @@ struct Level {
   double omega = 0.0;
   int sweeps_pre = 1, sweeps_post = 1;
   Smoother sm;
+  Smoother sm2;          // post smoother when it differs (FBGS: BWGS), else unused
+  bool has_sm2 = false;
   // apply work (mld_dmlprec_aply.f90:249-275, allocated once here)
   std::vector<double> x2l, y2l, tx, ty;
 };
@@ -666,7 +724,8 @@ struct Prec {
       Level& C = lv[level + 1];
       if (pre) {
         int sw = (trans == 'N') ? L.sweeps_pre : L.sweeps_post;
-        int rc = L.sm.apply(1.0, L.x2l.data(), 0.0, L.y2l.data(), trans, sw, n);  // :646-657
+        const Smoother& spre = (trans == 'N' || !L.has_sm2) ? L.sm : L.sm2;
+        int rc = spre.apply(1.0, L.x2l.data(), 0.0, L.y2l.data(), trans, sw, n);  // :646-657
         if (rc) return rc;
         axpby(n, 1.0, L.x2l.data(), 0.0, L.ty.data());                             // :669
         if (trans == 'N') spmv(L.a, -1.0, L.y2l.data(), 1.0, L.ty.data());         // :674-676
@@ -695,7 +754,8 @@ struct Prec {
         if (trans == 'N') spmv(L.a, -1.0, L.y2l.data(), 1.0, L.ty.data());
         else spmv_t(L.a, -1.0, L.y2l.data(), 1.0, L.ty.data());
         int sw = (trans == 'N') ? L.sweeps_post : L.sweeps_pre;
-        rc = L.sm.apply(1.0, L.ty.data(), 1.0, L.y2l.data(), trans, sw, n);
+        const Smoother& spost = (trans == 'N' && L.has_sm2) ? L.sm2 : L.sm;  // sm2a (:766-770)
+        rc = spost.apply(1.0, L.ty.data(), 1.0, L.y2l.data(), trans, sw, n);
         if (rc) return rc;
       }
     } else {
@@ -711,7 +771,19 @@ struct Prec {
     ix n = lv[0].a.nr;
     if (nlev == 1) {  // one-level preconditioner (:372-417)
       int nsw = std::max(lv[0].sweeps_pre, lv[0].sweeps_post);
-      return lv[0].sm.apply(1.0, x, 0.0, y, trans, nsw, n);
+      if (!lv[0].has_sm2) return lv[0].sm.apply(1.0, x, 0.0, y, trans, nsw, n);
+      // symmetrized pair (:378-412): w1 = x; nsw times w2 = S1 w1, w1 = S2 w2
+      const Smoother& s1 = (trans == 'N') ? lv[0].sm : lv[0].sm2;
+      const Smoother& s2 = (trans == 'N') ? lv[0].sm2 : lv[0].sm;
+      std::vector<double> w1(x, x + n), w2(n);
+      for (int k = 0; k < nsw; ++k) {
+        int rc = s1.apply(1.0, w1.data(), 0.0, w2.data(), trans, 1, n);
+        if (rc) return rc;
+        rc = s2.apply(1.0, w2.data(), 0.0, w1.data(), trans, 1, n);
+        if (rc) return rc;
+      }
+      axpby(n, 1.0, w1.data(), 0.0, y);
+      return 0;
     }
     double beta = 0.0;
     axpby(n, 1.0, x, 0.0, lv[0].x2l.data());  // :283
@@ -743,6 +815,9 @@ struct Prec {
     lv.resize(1);
     lv[0].a = a;
     int newsz = 0;
+    // MAX_LEVS 1: the one-level preconditioners of init ('JACOBI', 'FBGS', ...):
+    // a one-entry level array is kept as it is (:162-172)
+    if (prm.max_levs == 1) nplevs = 1;
     for (int i = 2; i <= nplevs; ++i) {  // array_build_loop (:251-380), 1-based levels
       const Csr& af = lv[i - 2].a;
       std::vector<int> ilaggr;
@@ -781,11 +856,19 @@ struct Prec {
     for (int l = 0; l < nlev; ++l) {
       Level& L = lv[l];
       bool coarse = (l == nlev - 1) && nlev > 1;
-      L.sm.kind = coarse ? prm.c_smoother : prm.smoother;
+      const int kind = coarse ? prm.c_smoother : prm.smoother;
+      // FBGS (mld_d_base_onelev_seti.F90:128-132): GS before, BWGS after
+      L.sm.kind = (kind == SM_FBGS) ? SM_GS : kind;
+      L.has_sm2 = (kind == SM_FBGS);
       L.sweeps_pre = coarse ? prm.c_sweeps : prm.sweeps_pre;
       L.sweeps_post = coarse ? prm.c_sweeps : prm.sweeps_post;
       int rc = L.sm.build(L.a);
       if (rc) return 1000 + rc;
+      if (L.has_sm2) {
+        L.sm2.kind = SM_BWGS;
+        rc = L.sm2.build(L.a);
+        if (rc) return 1000 + rc;
+      }
       ix n = L.a.nr;
       L.x2l.assign(n, 0.0); L.y2l.assign(n, 0.0); L.tx.assign(n, 0.0); L.ty.assign(n, 0.0);
     }
@@ -988,6 +1071,13 @@ int orc_prec_apply(void* h, const double* x, double* y, char trans) { return ((P
 int orc_prec_smooth(void* h, int lev, double alpha, const double* x, double beta, double* y, char trans, int sweeps) {
   Level& L = ((Prec*)h)->lv[lev];
   return L.sm.apply(alpha, x, beta, y, trans, sweeps, L.a.nr);
+}
+// which = 1: the post smoother (sm2a) where the level has a separate one
+int orc_prec_smooth_w(void* h, int lev, int which, double alpha, const double* x, double beta, double* y, char trans,
+                      int sweeps) {
+  Level& L = ((Prec*)h)->lv[lev];
+  const Smoother& sm = (which == 1 && L.has_sm2) ? L.sm2 : L.sm;
+  return sm.apply(alpha, x, beta, y, trans, sweeps, L.a.nr);
 }
 int orc_krylov(void* h, void* a, int method, const double* b, double* x, double eps, int itmax, int* iters,
                double* err) {

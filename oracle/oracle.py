@@ -21,6 +21,7 @@ _lib = None
 
 # Smoother codes: mld_base_prec_type.F90:166-176 (0..2); 100+ are additions.
 SM_NOPREC, SM_JAC, SM_BJAC, SM_L1JAC, SM_DENSELU = 0, 1, 2, 100, 101
+SM_FBGS, SM_GS, SM_BWGS = 4, 102, 103
 CYC_V, CYC_W = 3, 4
 KRYLOV = {"CG": 0, "FCG": 1, "BICGSTAB": 2}
 
@@ -94,6 +95,7 @@ def lib():
         L.orc_prec_level_dv.argtypes = [vp, C.c_int, dp]
         L.orc_prec_apply.argtypes = [vp, dp, dp, C.c_char]
         L.orc_prec_smooth.argtypes = [vp, C.c_int, C.c_double, dp, C.c_double, dp, C.c_char, C.c_int]
+        L.orc_prec_smooth_w.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, C.c_double, dp, C.c_char, C.c_int]
         L.orc_krylov.argtypes = [vp, vp, C.c_int, dp, dp, C.c_double, C.c_int, ip, dp]
         _lib = L
     return _lib
@@ -277,10 +279,10 @@ class Prec:
             raise RuntimeError(f"oracle apply failed: {rc}")
         return y
 
-    def smooth(self, lev, x, alpha=1.0, beta=0.0, y=None, trans="N", sweeps=1):
+    def smooth(self, lev, x, alpha=1.0, beta=0.0, y=None, trans="N", sweeps=1, which=0):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, dtype=np.float64).copy()
-        rc = lib().orc_prec_smooth(self.h, lev, alpha, _dp(x), beta, _dp(y), trans.encode(), sweeps)
+        rc = lib().orc_prec_smooth_w(self.h, lev, which, alpha, _dp(x), beta, _dp(y), trans.encode(), sweeps)
         if rc:
             raise RuntimeError(f"oracle smooth failed: {rc}")
         return y

@@ -136,10 +136,11 @@ def test_fv7_grad_kernel(eng, idim):
 
     rng = np.random.default_rng(idim)
     th, x, lam = (rng.standard_normal(idim ** 3) for _ in range(3))
-    g = eng.fv7_grad_theta(idim, dev(th), dev(x), dev(lam))
-    torch.cuda.synchronize()
     gr = fv7_grad_ref(th, x, lam)
-    assert np.abs(g.cpu().numpy() - gr).max() <= 1e-13 * np.abs(gr).max()
+    for work in (True, None):  # exp(theta) precomputed / formed in place
+        g = eng.fv7_grad_theta(idim, dev(th), dev(x), dev(lam), work=work)
+        torch.cuda.synchronize()
+        assert np.abs(g.cpu().numpy() - gr).max() <= 1e-13 * np.abs(gr).max()
 
 
 @pytest.mark.gpu

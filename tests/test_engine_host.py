@@ -40,11 +40,11 @@ def same(m1, m2):
             and (m1.data == m2.data).all())
 
 
-SMC = {"JACOBI": 1, "L1-JACOBI": 100, "BJAC": 2}
+SMC = {"JACOBI": 1, "L1-JACOBI": 100, "BJAC": 2, "FBGS": 4}
 
 
 @pytest.mark.parametrize("idim", [7, 20, 40])
-@pytest.mark.parametrize("smoother", ["JACOBI", "L1-JACOBI", "BJAC"])
+@pytest.mark.parametrize("smoother", ["JACOBI", "L1-JACOBI", "BJAC", "FBGS"])
 def test_host_build_bitexact_vs_oracle(eng, orc, idim, smoother):
     a = orc.gen_pde3d(idim)
     p = config1(eng, smoother, 2).build(a.to_scipy())
@@ -61,9 +61,26 @@ def test_host_build_bitexact_vs_oracle(eng, orc, idim, smoother):
         if info["nnz_nd"] > 0:
             assert same(p.level_csr(l, "ND"), po.level_csr(l, "ND").to_scipy())
         assert (p.dv(l) == po.dv(l)).all()
-        if info["solver"] == 7:
+        if info["solver"] in (6, 7, 14):  # GS, ILU(0), BWGS
             assert same(p.level_csr(l, "L"), po.level_csr(l, "L").to_scipy())
             assert same(p.level_csr(l, "U"), po.level_csr(l, "U").to_scipy())
+        if smoother == "FBGS" and l < p.nlev - 1:
+            assert (info["solver"], info["solver_post"]) == (6, 14)
+            assert (p.dv(l, which=1) == po.dv(l)).all()
+
+
+def test_one_level_fbgs_build(eng, orc):
+    """init('FBGS'): one level, GS before and BWGS after, the split of A as the
+    oracle's one-level build (MAX_LEVS 1) makes it."""
+    a = orc.gen_pde3d(9)
+    p = eng.MldPrec(device=-1).init("FBGS").build(a.to_scipy())
+    po = orc.Prec(orc.config1_params(smoother=orc.SM_FBGS, max_levs=1)).build(a)
+    assert p.nlev == po.nlev == 1
+    info = p.level_info(0)
+    assert (info["solver"], info["solver_post"]) == (6, 14)
+    for w in ("L", "U"):
+        assert same(p.level_csr(0, w), po.level_csr(0, w).to_scipy())
+    assert (p.dv(0) == po.dv(0)).all()
 
 
 def test_host_build_golden_matrix(eng, orc):
@@ -110,7 +127,8 @@ def test_error_codes(eng):
     with pytest.raises(E) as ex:
         p.init("NOT-A-PREC")
     assert ex.value.info == 2
-    p.init("ML")  # default smoother FBGS: not provided by this engine
+    p.init("ML")
+    p.set("SMOOTHER_TYPE", "AS")  # additive Schwarz: not provided by this engine
     from mld2p4_amd import problems
 
     A = problems.pde3d_poisson(6)

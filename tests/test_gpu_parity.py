@@ -35,7 +35,7 @@ def engine_prec(eng, A, smoother="JACOBI", sweeps=1, coarse="BJAC", csweeps=4, m
     return p.build(A)
 
 
-SM = {"JACOBI": 1, "BJAC": 2, "L1-JACOBI": 100}
+SM = {"JACOBI": 1, "BJAC": 2, "L1-JACOBI": 100, "FBGS": 4}
 
 
 def oracle_prec(orc, a, smoother="JACOBI", sweeps=1, coarse="BJAC", csweeps=4, mcs=200, **kw):
@@ -55,7 +55,7 @@ def relmax(y, yo):
 
 @pytest.mark.parametrize("idim", [6, 12, 24])
 @pytest.mark.parametrize("smoother,sweeps", [("JACOBI", 1), ("JACOBI", 2), ("L1-JACOBI", 2), ("L1-JACOBI", 3),
-                                             ("BJAC", 1)])
+                                             ("BJAC", 1), ("FBGS", 1), ("FBGS", 2)])
 def test_apply_fp64(eng, orc, idim, smoother, sweeps):
     _say(f"[apply fp64] idim={idim} smoother={smoother} sweeps={sweeps}")
     a = orc.gen_pde3d(idim)
@@ -140,6 +140,66 @@ def test_ilu_level_scheduled_bitexact(eng, orc, idim, beta):
             p.level_smooth(lev, dev(x), y, beta=beta, sweeps=1, trans=trans)
             yo = po.smooth(lev, x, alpha=1.0, beta=float(beta), y=y0, sweeps=1, trans=trans)
             assert (y.cpu().numpy() == yo).all(), (lev, trans)
+
+
+@pytest.mark.parametrize("idim", [14, 30])
+def test_gauss_seidel_level_bitexact(eng, orc, idim):
+    """GS (forward) and BWGS (backward) smoothers of the default ML
+    preconditioner, 1 and 3 sweeps, beta 0 and 1: level-scheduled
+    substitutions (one workgroup for idim 14, grid launches and runs for
+    idim 30) identical to the oracle's sequential sweeps; 'T' is refused."""
+    _say(f"[GS level] idim={idim}")
+    a = orc.gen_pde3d(idim)
+    A = a.to_scipy()
+    p = engine_prec(eng, A, "FBGS", 1)
+    po = oracle_prec(orc, a, "FBGS", 1)
+    rng = np.random.default_rng(21)
+    n = A.shape[0]
+    x = rng.standard_normal(n)
+    y0 = rng.standard_normal(n)
+    for which in (0, 1):
+        for sweeps in (1, 3):
+            for beta in (0, 1):
+                y = dev(y0.copy())
+                p.level_smooth(0, dev(x), y, beta=beta, sweeps=sweeps, which=which)
+                yo = po.smooth(0, x, alpha=1.0, beta=float(beta), y=y0, sweeps=sweeps, which=which)
+                assert (y.cpu().numpy() == yo).all(), (which, sweeps, beta)
+    with pytest.raises(eng.MldError):
+        p.level_smooth(0, dev(x), dev(y0.copy()), trans="T")
+
+
+@pytest.mark.parametrize("nsw", [1, 2])
+def test_one_level_fbgs_apply(eng, orc, nsw):
+    """init('FBGS'): the symmetrized one-level pair, bit-identical to the
+    oracle's; 'T' is refused as by the GS solver."""
+    _say(f"[one-level FBGS] nsw={nsw}")
+    a = orc.gen_pde3d(24)
+    A = a.to_scipy()
+    p = eng.MldPrec(device=0).init("FBGS")
+    p.set("SMOOTHER_SWEEPS", nsw)
+    p.build(A)
+    po = orc.Prec(orc.config1_params(smoother=orc.SM_FBGS, sweeps_pre=nsw, sweeps_post=nsw, max_levs=1)).build(a)
+    x = np.random.default_rng(22).standard_normal(A.shape[0])
+    y = p.apply(dev(x)).cpu().numpy()
+    assert (y == po.apply(x)).all()
+    with pytest.raises(eng.MldError):
+        p.apply(dev(x), trans="T")
+
+
+def test_krylov_default_ml(eng, orc):
+    """The reference's default ML preconditioner (FBGS smoother, coarse ILU(0)
+    through block Jacobi) in CG: the same iteration count as the oracle."""
+    _say("[krylov default ML] idim=30")
+    a = orc.gen_pde3d(30)
+    A = a.to_scipy()
+    p = engine_prec(eng, A, "FBGS", 1)
+    po = oracle_prec(orc, a, "FBGS", 1)
+    b = np.ones(A.shape[0])
+    x, it, err = eng.psb_krylov("CG", A, p, dev(b), eps=1e-8)
+    xo, ito, erro = po.krylov("CG", b, eps=1e-8)
+    assert err <= 1e-8 and abs(it - ito) <= 1
+    r = b - A @ x.cpu().numpy()
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-8
 
 
 @pytest.mark.parametrize("method", ["CG", "FCG", "BICGSTAB"])

@@ -277,6 +277,80 @@ def test_vcycle_linear_symmetric_transpose(orc, sm):
     assert np.allclose(p.apply(u, trans="T"), Mu, rtol=1e-11, atol=1e-12 * np.abs(Mu).max())
 
 
+# --------------------------------------------------- hybrid Gauss-Seidel
+def test_gauss_seidel_sweeps(orc):
+    """GS: one sweep from zero is forward substitution with tril(A) (scipy's
+    triangular solver); BWGS: backward substitution with triu(A); a second
+    sweep is (D+L)^-1 (x - U y1) (dense solve); 'T' is refused as in the
+    reference solver."""
+    a = orc.gen_pde3d(5)
+    A = a.to_scipy()
+    prm = orc.config1_params(smoother=orc.SM_FBGS, sweeps_pre=1, sweeps_post=1, min_coarse_size=20)
+    p = orc.Prec(prm).build(a)
+    x = np.random.default_rng(12).standard_normal(A.shape[0])
+    Lo = sp.tril(A, 0, format="csr")
+    Up = sp.triu(A, 0, format="csr")
+    y_gs = p.smooth(0, x, which=0, sweeps=1)
+    y_bw = p.smooth(0, x, which=1, sweeps=1)
+    assert np.allclose(y_gs, spla.spsolve_triangular(Lo, x, lower=True), rtol=1e-13, atol=0)
+    assert np.allclose(y_bw, spla.spsolve_triangular(Up, x, lower=False), rtol=1e-13, atol=0)
+    Ad = A.toarray()
+    Ls, Us = np.tril(Ad), np.triu(Ad, 1)
+    y1 = np.linalg.solve(Ls, x)
+    y2 = np.linalg.solve(Ls, x - Us @ y1)
+    assert np.allclose(p.smooth(0, x, which=0, sweeps=2), y2, rtol=1e-12, atol=1e-14 * np.abs(y2).max())
+    # y = alpha S x + beta y
+    y0 = np.random.default_rng(13).standard_normal(A.shape[0])
+    assert np.allclose(p.smooth(0, x, alpha=1.0, beta=1.0, y=y0, which=1, sweeps=1), y_bw + y0, rtol=1e-15)
+    with pytest.raises(RuntimeError):
+        p.smooth(0, x, trans="T")
+
+
+def test_fbgs_two_level_error_propagation_and_symmetry(orc):
+    """V(1,1) with forward GS before and backward GS after an exact coarse
+    solve: I - M A = (I - (D+U)^-1 A)(I - P Ac^-1 P^T A)(I - (D+L)^-1 A); the
+    resulting M is symmetric for symmetric A."""
+    a = orc.gen_pde3d(4)
+    A = a.to_scipy().toarray()
+    prm = orc.config1_params(smoother=orc.SM_FBGS, min_coarse_size=1, max_levs=2, c_smoother=orc.SM_DENSELU,
+                             c_sweeps=1)
+    p = orc.Prec(prm).build(a)
+    P = p.level_csr(1, "P").to_scipy().toarray()
+    n = A.shape[0]
+    Spre = np.eye(n) - np.linalg.solve(np.tril(A), A)
+    Spost = np.eye(n) - np.linalg.solve(np.triu(A), A)
+    C = np.eye(n) - P @ np.linalg.solve(P.T @ A @ P, P.T @ A)
+    M = (np.eye(n) - Spost @ C @ Spre) @ np.linalg.inv(A)
+    x = np.random.default_rng(14).standard_normal(n)
+    assert np.allclose(p.apply(x), M @ x, rtol=1e-9, atol=1e-14 * np.abs(M @ x).max())
+    rng = np.random.default_rng(15)
+    a3 = orc.gen_pde3d(10)
+    p3 = orc.Prec(orc.config1_params(smoother=orc.SM_FBGS, sweeps_pre=2, sweeps_post=2)).build(a3)
+    u, v = rng.standard_normal((2, a3.shape[0]))
+    assert abs(u @ p3.apply(v) - v @ p3.apply(u)) <= 1e-12 * abs(u @ p3.apply(v))
+
+
+@pytest.mark.parametrize("nsw", [1, 2])
+def test_fbgs_one_level_symmetrized_pair(orc, nsw):
+    """One-level FBGS (MAX_LEVS 1): nsw times w2 = (D+L)^-1 w1, w1 = (D+U)^-1 w2
+    from w1 = x, each a single sweep from zero; for 'T' the two solvers
+    swap, and GS refuses 'T'."""
+    a = orc.gen_pde3d(5)
+    A = a.to_scipy()
+    prm = orc.config1_params(smoother=orc.SM_FBGS, sweeps_pre=nsw, sweeps_post=1, max_levs=1)
+    p = orc.Prec(prm).build(a)
+    assert p.nlev == 1
+    x = np.random.default_rng(16).standard_normal(A.shape[0])
+    Lo = sp.tril(A, 0, format="csr")
+    Up = sp.triu(A, 0, format="csr")
+    w = x.copy()
+    for _ in range(nsw):
+        w = spla.spsolve_triangular(Up, spla.spsolve_triangular(Lo, w, lower=True), lower=False)
+    assert np.allclose(p.apply(x), w, rtol=1e-12, atol=1e-14 * np.abs(w).max())
+    with pytest.raises(RuntimeError):
+        p.apply(x, trans="T")
+
+
 # ----------------------------------------------------------------- Krylov
 @pytest.mark.parametrize("method", ["CG", "FCG", "BICGSTAB"])
 def test_krylov_small_exact(orc, method):
