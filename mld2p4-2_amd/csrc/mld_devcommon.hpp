@@ -78,6 +78,9 @@ struct DLevel {
   bool has_sm2 = false;
   int sweeps_pre = 1, sweeps_post = 1, ml_cycle = VCYCLE;
   V *x2l = nullptr, *y2l = nullptr, *tx = nullptr, *ty = nullptr, *t1 = nullptr, *t2 = nullptr;
+  // K-cycle inner iteration at this level: w, v, v1, d0, d1 (the iterate
+  // accumulates in y2l)
+  V *kw = nullptr, *kv = nullptr, *kv1 = nullptr, *kd0 = nullptr, *kd1 = nullptr;
   double* wlu = nullptr;
 };
 

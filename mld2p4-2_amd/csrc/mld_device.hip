@@ -57,6 +57,15 @@ class DevPrecT : public DevPrec {
       D.t1 = ar_.alloc<V>(n); D.t2 = ar_.alloc<V>(n);
       if (H.smd.solver == DENSE_LU || (D.has_sm2 && H.smd2.solver == DENSE_LU)) D.wlu = ar_.alloc<double>(n);
     }
+    // K-cycle: the inner iteration runs on levels 1 .. nlev-2 (0-based)
+    for (int l = 1; l + 1 < nlev; ++l) {
+      const int c = p.precv[l - 1].parms.ml_cycle;
+      if (c != KCYCLE && c != KCYCLESYM) continue;
+      DLevel<V>& D = L_[l];
+      const size_t n = (size_t)D.n;
+      D.kw = ar_.alloc<V>(n); D.kv = ar_.alloc<V>(n); D.kv1 = ar_.alloc<V>(n);
+      D.kd0 = ar_.alloc<V>(n); D.kd1 = ar_.alloc<V>(n);
+    }
     // Krylov matrix, summed in FP64: level 0's own copy in FP64; in the FP32
     // form, level 0's FP32 copy when every value of A is exact in FP32 (the
     // products are then bit-identical to FP64 storage), else an FP64 copy.
@@ -156,7 +165,7 @@ class DevPrecT : public DevPrec {
     V* y0save = L0.y2l;
     L0.x2l = const_cast<V*>(in_vec(x, L0.x2l, L0.n));
     L0.y2l = out_vec(y, L0.y2l);
-    int rc = inner_mult(0, trans);
+    int rc = inner_ml_aply(0, trans);
     if (!out_done_) finish_out(y, L0.y2l, L0.n);
     L0.x2l = x0save;
     L0.y2l = y0save;
@@ -303,8 +312,8 @@ class DevPrecT : public DevPrec {
   double* partial_ = nullptr;
   Red red(int slot) { return Red{partial_, scal_ + slot}; }
   // one-block sum of the partials of a streaming reduction over n0_ entries
-  void finalize(int K, int slot) {
-    const int nb = red_blocks(n0_);
+  void finalize(int K, int slot, int nb = -1) {
+    if (nb < 0) nb = red_blocks(n0_);
     if (K == 1) finalize_kernel<1><<<1, FIN_T, 0, s_>>>(nb, partial_, scal_ + slot);
     else finalize_kernel<2><<<1, FIN_T, 0, s_>>>(nb, partial_, scal_ + slot);
     HIPCHECK(hipGetLastError());
@@ -562,13 +571,11 @@ class DevPrecT : public DevPrec {
   //   GS:   y <- (D + L)^-1 (x - U y),   BWGS: y <- (D + U)^-1 (x - L y)
   // (mld_d_gs_solver_apply_vect.f90:150-156, mld_d_bwgs_solver_apply_vect.f90
   // :151-155).  The first sweep's product with the zero iterate is skipped
-  // (x - 0 = x exactly).  Only 'N', as in the reference solvers.
+  // (x - 0 = x exactly).  Only 'N', as in the reference solvers.  The 'Z'
+  // application precedes the loop there, so sweeps = 0 also gives one sweep.
   int gs_apply(DLevel<V>& D, DSmoother<V>& S, const V* x, int beta, V* y, int sweeps, char trans) {
     if (trans != 'N') return E_UNSUPPORTED;
-    if (sweeps == 0) {
-      if (!beta) fill_kernel<V><<<nblk(D.n), BS, 0, s_>>>(D.n, y, (V)0);
-      return 0;
-    }
+    sweeps = std::max(1, sweeps);
     const bool fw = (S.solver == GS);
     const DMat<V>& O = fw ? S.u : S.l;
     V* t = D.tx;
@@ -712,6 +719,134 @@ class DevPrecT : public DevPrec {
     spmv_launch(a, GatherX<V, V>{yv}, EpiRes<V, V>{b, r}, s_);
   }
 
+  // inner_ml_aply (mld_dmlprec_aply.f90:382-464): dispatch on the level's cycle
+  int inner_ml_aply(int level, char trans) {
+    switch (L_[level].ml_cycle) {
+      case ADD_ML: return inner_add(level, trans);
+      case MULT_ML: case VCYCLE: case WCYCLE: return inner_mult(level, trans);
+      case KCYCLE: case KCYCLESYM: return inner_k_cycle(level, trans);
+      default: return E_UNSUPPORTED;
+    }
+  }
+
+  // mld_d_inner_add (mld_dmlprec_aply.f90:467-588): smoothing of x2l itself
+  // (the sm / sm2a pair max(pre, post) times from y2l = x2l where sm2a
+  // exists, in that order also for 'T'), then y2l += P (cycle on R x2l).
+  int inner_add(int level, char trans) {
+    const int nlev = (int)L_.size();
+    DLevel<V>& D = L_[level];
+    int rc;
+    if (D.has_sm2) {
+      add_kernel<V, V, false><<<nblk(D.n), BS, 0, s_>>>(D.n, D.x2l, D.y2l);
+      const int sw = std::max(D.sweeps_pre, D.sweeps_post);
+      for (int k = 0; k < sw; ++k) {  // the smoothers' own scratch is tx, t1, t2
+        rc = smooth(D, D.sm, D.y2l, 0, D.ty, 1, trans);
+        if (rc) return rc;
+        rc = smooth(D, D.sm2, D.ty, 0, D.y2l, 1, trans);
+        if (rc) return rc;
+      }
+    } else {
+      rc = smooth(D, D.sm, D.x2l, 0, D.y2l, D.sweeps_pre, trans);
+      if (rc) return rc;
+    }
+    if (level < nlev - 1) {
+      DLevel<V>& C = L_[level + 1];
+      spmv_launch(C.r, GatherX<V, V>{D.x2l}, EpiSet<V, V>{C.x2l}, s_);
+      rc = inner_ml_aply(level + 1, trans);
+      if (rc) return rc;
+      spmv_launch(C.p, GatherX<V, V>{C.y2l}, EpiAdd<V>{D.y2l}, s_);
+    }
+    return 0;
+  }
+
+  // mld_d_inner_k_cycle (mld_dmlprec_aply.f90:811-1014): pre- and
+  // post-smoothing without the pre/post switches; where the next level is not
+  // the coarsest, its correction comes from inneritkcycle.
+  int inner_k_cycle(int level, char trans) {
+    const int nlev = (int)L_.size();
+    DLevel<V>& D = L_[level];
+    if (level == nlev - 1) return smooth(D, D.sm, D.x2l, 0, D.y2l, D.sweeps_pre, trans);
+    const bool tn = (trans == 'N');
+    DSmoother<V>& s2 = D.has_sm2 ? D.sm2 : D.sm;
+    int rc = tn ? smooth(D, D.sm, D.x2l, 0, D.y2l, D.sweeps_pre, trans)
+                : smooth(D, s2, D.x2l, 0, D.y2l, D.sweeps_post, trans);
+    if (rc) return rc;
+    DLevel<V>& C = L_[level + 1];
+    residual(D, D.x2l, D.y2l, D.ty, trans);
+    spmv_launch(C.r, GatherX<V, V>{D.ty}, EpiSet<V, V>{C.x2l}, s_);
+    if (level <= nlev - 3) rc = inneritkcycle(level + 1, trans, D.ml_cycle == KCYCLESYM);
+    else rc = inner_ml_aply(level + 1, trans);
+    if (rc) return rc;
+    spmv_launch(C.p, GatherX<V, V>{C.y2l}, EpiAdd<V>{D.y2l}, s_);
+    residual(D, D.x2l, D.y2l, D.ty, trans);
+    return tn ? smooth(D, s2, D.ty, 1, D.y2l, D.sweeps_post, trans)
+              : smooth(D, D.sm, D.ty, 1, D.y2l, D.sweeps_pre, trans);
+  }
+
+  // mld_dinneritkcycle (mld_dmlprec_aply.f90:1017-1184): at most two steps
+  // of FCG (fcg) or GCR from zero on A_l y = x2l, preconditioned by this
+  // level's cycle; one step when ||w|| <= 0.25 ||x2l||.  The dots are
+  // reduced on the device and read by the host (the step count and the
+  // coefficients are host decisions, as in the reference); the products
+  // with A_l are untransposed for 'T' too (:1095, :1137).
+  int inneritkcycle(int level, char trans, bool fcg) {
+    DLevel<V>& D = L_[level];
+    const int n = D.n;
+    const int nb = nblk(n);
+    V *w = D.kw, *v = D.kv, *v1 = D.kv1, *d0 = D.kd0, *d1 = D.kd1;
+    if (!w) return E_UNSUPPORTED;
+    add_kernel<V, V, false><<<nb, BS, 0, s_>>>(n, D.x2l, w);  // w = x2l
+    fill_kernel<V><<<nb, BS, 0, s_>>>(n, D.y2l, (V)0);
+    int rc = inner_ml_aply(level, trans);
+    if (rc) return rc;
+    // the scalar slots are shared with the inner iterations of deeper levels,
+    // so every dot of this level is formed after its cycle returns
+    ldot(n, w, w, nullptr, nullptr, S_K0);  // delta0^2
+    add_kernel<V, V, false><<<nb, BS, 0, s_>>>(n, D.y2l, d0);
+    spmv_launch(D.a, GatherX<V, V>{d0}, EpiSet<V, V>{v}, s_);
+    if (fcg) ldot(n, d0, w, d0, v, S_K1);  // delta_old, tau
+    else ldot(n, v, w, v, v, S_K1);
+    read_scal(S_K0);
+    const double delta0 = std::sqrt(hscal_[S_K0]);
+    const double delta_old = hscal_[S_K1], tau = hscal_[S_K1 + 1];
+    double alpha = delta_old / tau;
+    axpby_kernel<V><<<nb, BS, 0, s_>>>(n, (V)(-alpha), v, (V)1, w);  // w -= alpha v
+    ldot(n, w, w, nullptr, nullptr, S_K0);
+    const double l2 = std::sqrt(read_scal(S_K0));
+    if (l2 <= 0.25 * delta0) {
+      axpby_kernel<V><<<nb, BS, 0, s_>>>(n, (V)alpha, d0, (V)0, D.y2l);  // y = alpha d0
+    } else {
+      add_kernel<V, V, false><<<nb, BS, 0, s_>>>(n, w, D.x2l);  // x2l = w
+      rc = inner_ml_aply(level, trans);
+      if (rc) return rc;
+      add_kernel<V, V, false><<<nb, BS, 0, s_>>>(n, D.y2l, d1);
+      spmv_launch(D.a, GatherX<V, V>{d1}, EpiSet<V, V>{v1}, s_);
+      if (fcg) {
+        ldot(n, d1, v, d1, v1, S_K1);  // tau1, tau2
+        ldot(n, d1, w, nullptr, nullptr, S_K0);  // tau3
+      } else {
+        ldot(n, v1, v, v1, v1, S_K1);
+        ldot(n, v1, w, nullptr, nullptr, S_K0);
+      }
+      read_scal(S_K0);
+      const double tau1 = hscal_[S_K1], tau2 = hscal_[S_K1 + 1], tau3 = hscal_[S_K0];
+      const double tau4 = tau2 - (tau1 * tau1) / tau;
+      alpha = alpha - (tau1 * tau3) / (tau * tau4);
+      const double alpha1 = tau3 / tau4;
+      axpby_kernel<V><<<nb, BS, 0, s_>>>(n, (V)alpha, d0, (V)0, D.y2l);   // y = alpha d0
+      axpby_kernel<V><<<nb, BS, 0, s_>>>(n, (V)alpha1, d1, (V)1, D.y2l);  // y += alpha1 d1
+    }
+    HIPCHECK(hipGetLastError());
+    return 0;
+  }
+  // dot products of level vectors (one or two pairs) into scalar slots
+  void ldot(int n, const V* a, const V* b, const V* c, const V* d, int slot) {
+    const int nb = red_blocks(n);
+    if (c) dot_kernel<V, 2><<<nb, BS, 0, s_>>>(n, a, b, c, d, red(slot));
+    else dot_kernel<V, 1><<<nb, BS, 0, s_>>>(n, a, b, nullptr, nullptr, red(slot));
+    finalize(c ? 2 : 1, slot, nb);
+  }
+
   // mld_d_inner_mult (mld_dmlprec_aply.f90:590-809)
   int inner_mult(int level, char trans) {
     const int nlev = (int)L_.size();
@@ -733,13 +868,13 @@ class DevPrecT : public DevPrec {
         fill_kernel<V><<<nblk(D.n), BS, 0, s_>>>(D.n, D.y2l, (V)0);
         spmv_launch(C.r, GatherX<V, V>{D.x2l}, EpiSet<V, V>{C.x2l}, s_);
       }
-      rc = inner_mult(level + 1, trans);
+      rc = inner_ml_aply(level + 1, trans);
       if (rc) return rc;
       spmv_launch(C.p, GatherX<V, V>{C.y2l}, EpiAdd<V>{D.y2l}, s_);
       if (D.ml_cycle == WCYCLE) {
         residual(D, D.x2l, D.y2l, D.ty, trans);
         spmv_launch(C.r, GatherX<V, V>{D.ty}, EpiSet<V, V>{C.x2l}, s_);
-        rc = inner_mult(level + 1, trans);
+        rc = inner_ml_aply(level + 1, trans);
         if (rc) return rc;
         spmv_launch(C.p, GatherX<V, V>{C.y2l}, EpiAdd<V>{D.y2l}, s_);
       }
@@ -768,7 +903,7 @@ class DevPrecT : public DevPrec {
         // x2l = x - A y
         spmv_launch(L0.a, GatherX<V, V>{ysum}, EpiRes<V, V>{xres_, xin}, s_);
       }
-      int rc = inner_mult(0, trans);
+      int rc = inner_ml_aply(0, trans);
       if (rc) return rc;
       if (isw == 1) add_kernel<V, V, false><<<nblk(n), BS, 0, s_>>>(n, L0.y2l, ysum);
       else add_kernel<V, V, true><<<nblk(n), BS, 0, s_>>>(n, L0.y2l, ysum);

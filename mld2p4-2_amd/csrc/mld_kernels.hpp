@@ -432,7 +432,7 @@ __global__ __launch_bounds__(BS) void dot_kernel(int n, const V* __restrict__ x0
 // ------------------------------------------------------------ Krylov steps
 // Scalar slots in device memory.
 enum { S_RHO0 = 0, S_RHO1 = 1, S_SIGMA = 2, S_RR = 3, S_PR = 4, S_GAMMA = 5, S_DELTA = 6, S_TMP0 = 8,
-       S_TMP1 = 9, S_OMEGA = 10, S_ALPHA = 11, S_NSCAL = 16 };
+       S_TMP1 = 9, S_OMEGA = 10, S_ALPHA = 11, S_K0 = 12, S_K1 = 13, S_NSCAL = 16 };
 
 // CG: p = z (first) or p = z + (rho_new/rho_old) p
 __global__ __launch_bounds__(BS) void cg_pupdate(int n, const double* __restrict__ z, double* __restrict__ p,

@@ -648,14 +648,15 @@ static int build_union(const std::vector<const Prec*>& S, std::vector<HULevel>& 
             err = "the batched engines provide point/l1-Jacobi smoothers above the coarsest level";
             return E_UNSUPPORTED;
           }
+        const int cyc = (L.parms.ml_cycle == MULT_ML) ? VCYCLE : L.parms.ml_cycle;  // MULT is the V-cycle
         if (first) {
           H.sweeps_pre = L.parms.sweeps_pre;
           H.sweeps_post = L.parms.sweeps_post;
-          H.ml_cycle = L.parms.ml_cycle;
+          H.ml_cycle = cyc;
           H.has2 = has2;
           first = false;
         } else if (H.sweeps_pre != L.parms.sweeps_pre || H.sweeps_post != L.parms.sweeps_post ||
-                   H.ml_cycle != L.parms.ml_cycle || H.has2 != has2) {
+                   H.ml_cycle != cyc || H.has2 != has2) {
           err = "systems of a group must share sweeps, cycle and smoother layout per level";
           return E_UNSUPPORTED;
         }

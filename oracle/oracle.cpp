@@ -66,7 +66,9 @@ enum {
   SM_GS = 102,     // jacobi smoother + forward Gauss-Seidel solver (mld_d_gs_solver)
   SM_BWGS = 103    // jacobi smoother + backward Gauss-Seidel solver (mld_d_bwgs_solver)
 };
-enum { CYC_V = 3, CYC_W = 4 };  // mld_vcycle_ml_, mld_wcycle_ml_
+// mld_add_ml_, mld_mult_ml_, mld_vcycle_ml_, mld_wcycle_ml_, mld_kcycle_ml_,
+// mld_kcyclesym_ml_ (mld_base_prec_type.F90)
+enum { CYC_ADD = 1, CYC_MULT = 2, CYC_V = 3, CYC_W = 4, CYC_K = 5, CYC_KSYM = 6 };
 
 }  // namespace
 
@@ -80,7 +82,7 @@ struct orc_params {
   double aggr_omega_val; // used when aggr_omega_alg == 999
   int aggr_filter;       // 0 = no filter, 1 = filter
   int aggr_ord;          // 0 = natural, 1 = descending degree
-  int ml_cycle;          // 3 = V, 4 = W
+  int ml_cycle;          // 1 ADD, 2 MULT (= V), 3 V, 4 W, 5 K (GCR inner), 6 K symmetric (FCG inner)
   int outer_sweeps;      // OUTER_SWEEPS
   int smoother;          // smoother at levels 1..nlev-1
   int sweeps_pre;
@@ -666,11 +668,12 @@ struct Smoother {
     if (iterative()) {
       // only 'N' (mld_d_gs_solver_apply_vect.f90:81-87); the sweep loop of
       // :133-175 with init 'Z' then 'Y' and an empty ND: every smoother sweep
-      // is one GS sweep (solver sweeps = 1) on x, starting from zero
+      // is one GS sweep (solver sweeps = 1) on x, starting from zero; the
+      // 'Z' application comes first, so sweeps = 0 also gives one sweep
       if (trans != 'N') return -3;
       if (sweeps < 0) return -2;
       std::vector<double> ty(n, 0.0);
-      for (int i = 0; i < sweeps; ++i) gs_sweep(x, ty.data(), n);
+      for (int i = 0; i < std::max(1, sweeps); ++i) gs_sweep(x, ty.data(), n);
       axpby(n, alpha, ty.data(), beta, y);
       return 0;
     }
@@ -704,6 +707,8 @@ struct Level {
   bool has_sm2 = false;
   // apply work (mld_dmlprec_aply.f90:249-275, allocated once here)
   std::vector<double> x2l, y2l, tx, ty;
+  // K-cycle inner iteration at this level (mld_dinneritkcycle): w, v, v1, x, d(0:1)
+  std::vector<double> kw, kv, kv1, kx, kd0, kd1;
 };
 
 struct Prec {
@@ -712,6 +717,124 @@ struct Prec {
   int casize = 0;
 
   void zero_y_if_needed(int l) { std::fill(lv[l].y2l.begin(), lv[l].y2l.end(), 0.0); }
+
+  // inner_ml_aply (mld_dmlprec_aply.f90:382-464): dispatch on the level's cycle.
+  int inner_ml_aply(int level, char trans) {
+    switch (prm.ml_cycle) {
+      case CYC_ADD: return inner_add(level, trans);
+      case CYC_MULT: case CYC_V: case CYC_W: return inner_mult(level, trans);
+      case CYC_K: case CYC_KSYM: return inner_k_cycle(level, trans);
+      default: return -4;
+    }
+  }
+
+  // mld_d_inner_add (mld_dmlprec_aply.f90:467-588): smoother on x2l (the
+  // sm/sm2a pair max(pre, post) times when sm2a exists, from y2l = x2l), then
+  // y2l += P (cycle on R x2l).
+  int inner_add(int level, char trans) {
+    int nlev = (int)lv.size();
+    Level& L = lv[level];
+    ix n = L.a.nr;
+    if (L.has_sm2) {
+      axpby(n, 1.0, L.x2l.data(), 0.0, L.y2l.data());
+      int sw = std::max(L.sweeps_pre, L.sweeps_post);
+      for (int k = 0; k < sw; ++k) {
+        int rc = L.sm.apply(1.0, L.y2l.data(), 0.0, L.tx.data(), trans, 1, n);
+        if (rc) return rc;
+        rc = L.sm2.apply(1.0, L.tx.data(), 0.0, L.y2l.data(), trans, 1, n);
+        if (rc) return rc;
+      }
+    } else {
+      int rc = L.sm.apply(1.0, L.x2l.data(), 0.0, L.y2l.data(), trans, L.sweeps_pre, n);
+      if (rc) return rc;
+    }
+    if (level < nlev - 1) {
+      Level& C = lv[level + 1];
+      spmv(C.r, 1.0, L.x2l.data(), 0.0, C.x2l.data());
+      int rc = inner_ml_aply(level + 1, trans);
+      if (rc) return rc;
+      spmv(C.p, 1.0, C.y2l.data(), 1.0, L.y2l.data());
+    }
+    return 0;
+  }
+
+  // mld_d_inner_k_cycle (mld_dmlprec_aply.f90:811-1014): smoothing with no
+  // pre/post switches, and on levels whose next level is not the coarsest
+  // the next level's correction comes from two steps of a flexible Krylov
+  // method preconditioned by that level's own cycle.
+  int inner_k_cycle(int level, char trans) {
+    int nlev = (int)lv.size();
+    Level& L = lv[level];
+    ix n = L.a.nr;
+    if (level == nlev - 1)
+      return L.sm.apply(1.0, L.x2l.data(), 0.0, L.y2l.data(), trans, L.sweeps_pre, n);  // :867-873
+    const Smoother& s2 = L.has_sm2 ? L.sm2 : L.sm;
+    int rc = (trans == 'N') ? L.sm.apply(1.0, L.x2l.data(), 0.0, L.y2l.data(), trans, L.sweeps_pre, n)
+                            : s2.apply(1.0, L.x2l.data(), 0.0, L.y2l.data(), trans, L.sweeps_post, n);  // :877-893
+    if (rc) return rc;
+    axpby(n, 1.0, L.x2l.data(), 0.0, L.ty.data());  // :903-912
+    if (trans == 'N') spmv(L.a, -1.0, L.y2l.data(), 1.0, L.ty.data());
+    else spmv_t(L.a, -1.0, L.y2l.data(), 1.0, L.ty.data());
+    Level& C = lv[level + 1];
+    spmv(C.r, 1.0, L.ty.data(), 0.0, C.x2l.data());  // :918-920
+    if (level <= nlev - 3) rc = inneritkcycle(level + 1, trans, prm.ml_cycle == CYC_KSYM);  // :929-942
+    else rc = inner_ml_aply(level + 1, trans);
+    if (rc) return rc;
+    spmv(C.p, 1.0, C.y2l.data(), 1.0, L.y2l.data());  // :953-955
+    axpby(n, 1.0, L.x2l.data(), 0.0, L.ty.data());    // :966-972
+    if (trans == 'N') spmv(L.a, -1.0, L.y2l.data(), 1.0, L.ty.data());
+    else spmv_t(L.a, -1.0, L.y2l.data(), 1.0, L.ty.data());
+    return (trans == 'N') ? s2.apply(1.0, L.ty.data(), 1.0, L.y2l.data(), trans, L.sweeps_post, n)
+                          : L.sm.apply(1.0, L.ty.data(), 1.0, L.y2l.data(), trans, L.sweeps_pre, n);  // :982-993
+  }
+
+  // mld_dinneritkcycle (mld_dmlprec_aply.f90:1017-1184): at most two steps of
+  // FCG (fcg) or GCR from x = 0 on A_l x = x2l, preconditioned by
+  // inner_ml_aply(level); stops after one step when ||w|| <= 0.25 ||x2l||.
+  // The products with A_l are untransposed for 'T' as well (:1095, :1137).
+  int inneritkcycle(int level, char trans, bool fcg) {
+    Level& L = lv[level];
+    ix n = L.a.nr;
+    if (L.kw.size() != (size_t)n) {
+      L.kw.assign(n, 0.0); L.kv.assign(n, 0.0); L.kv1.assign(n, 0.0);
+      L.kx.assign(n, 0.0); L.kd0.assign(n, 0.0); L.kd1.assign(n, 0.0);
+    }
+    double* w = L.kw.data(); double* v = L.kv.data(); double* v1 = L.kv1.data();
+    double* x = L.kx.data(); double* d0 = L.kd0.data(); double* d1 = L.kd1.data();
+    std::fill(L.kx.begin(), L.kx.end(), 0.0);
+    axpby(n, 1.0, L.x2l.data(), 0.0, w);
+    const double delta0 = nrm2(n, w);
+    std::fill(L.y2l.begin(), L.y2l.end(), 0.0);
+    int rc = inner_ml_aply(level, trans);
+    if (rc) return rc;
+    axpby(n, 1.0, L.y2l.data(), 0.0, d0);
+    spmv(L.a, 1.0, d0, 0.0, v);
+    double delta_old, tau;
+    if (fcg) { delta_old = dot(n, d0, w); tau = dot(n, d0, v); }
+    else { delta_old = dot(n, v, w); tau = dot(n, v, v); }
+    double alpha = delta_old / tau;
+    axpby(n, -alpha, v, 1.0, w);
+    const double l2 = nrm2(n, w);
+    if (l2 <= 0.25 * delta0) {
+      axpby(n, alpha, d0, 1.0, x);
+    } else {
+      axpby(n, 1.0, w, 0.0, L.x2l.data());
+      rc = inner_ml_aply(level, trans);
+      if (rc) return rc;
+      axpby(n, 1.0, L.y2l.data(), 0.0, d1);
+      spmv(L.a, 1.0, d1, 0.0, v1);
+      double tau1, tau2, tau3;
+      if (fcg) { tau1 = dot(n, d1, v); tau2 = dot(n, d1, v1); tau3 = dot(n, d1, w); }
+      else { tau1 = dot(n, v1, v); tau2 = dot(n, v1, v1); tau3 = dot(n, v1, w); }
+      const double tau4 = tau2 - (tau1 * tau1) / tau;
+      alpha = alpha - (tau1 * tau3) / (tau * tau4);
+      axpby(n, alpha, d0, 1.0, x);
+      alpha = tau3 / tau4;
+      axpby(n, alpha, d1, 1.0, x);
+    }
+    axpby(n, 1.0, x, 0.0, L.y2l.data());
+    return 0;
+  }
 
   // mld_d_inner_mult (mld_dmlprec_aply.f90:590-809); level 0-based.
   int inner_mult(int level, char trans) {
@@ -737,7 +860,7 @@ struct Prec {
         std::fill(L.y2l.begin(), L.y2l.end(), 0.0);
         spmv(C.r, 1.0, L.x2l.data(), 0.0, C.x2l.data());                          // :692-694
       }
-      int rc = inner_mult(level + 1, trans);                                       // :702
+      int rc = inner_ml_aply(level + 1, trans);                                    // :702
       if (rc) return rc;
       spmv(C.p, 1.0, C.y2l.data(), 1.0, L.y2l.data());                             // :707-709
       if (prm.ml_cycle == CYC_W) {                                                 // :716-745
@@ -745,7 +868,7 @@ struct Prec {
         if (trans == 'N') spmv(L.a, -1.0, L.y2l.data(), 1.0, L.ty.data());
         else spmv_t(L.a, -1.0, L.y2l.data(), 1.0, L.ty.data());
         spmv(C.r, 1.0, L.ty.data(), 0.0, C.x2l.data());
-        rc = inner_mult(level + 1, trans);
+        rc = inner_ml_aply(level + 1, trans);
         if (rc) return rc;
         spmv(C.p, 1.0, C.y2l.data(), 1.0, L.y2l.data());
       }
@@ -788,14 +911,14 @@ struct Prec {
     double beta = 0.0;
     axpby(n, 1.0, x, 0.0, lv[0].x2l.data());  // :283
     for (int isw = 1; isw <= prm.outer_sweeps - 1; ++isw) {  // :285-308
-      int rc = inner_mult(0, trans);
+      int rc = inner_ml_aply(0, trans);
       if (rc) return rc;
       axpby(n, 1.0, lv[0].y2l.data(), beta, y);
       beta = 1.0;
       axpby(n, 1.0, x, 0.0, lv[0].x2l.data());
       spmv(lv[0].a, -1.0, y, 1.0, lv[0].x2l.data());
     }
-    int rc = inner_mult(0, trans);
+    int rc = inner_ml_aply(0, trans);
     if (rc) return rc;
     axpby(n, 1.0, lv[0].y2l.data(), beta, y);  // :327
     return 0;
@@ -1078,6 +1201,19 @@ int orc_prec_smooth_w(void* h, int lev, int which, double alpha, const double* x
   Level& L = ((Prec*)h)->lv[lev];
   const Smoother& sm = (which == 1 && L.has_sm2) ? L.sm2 : L.sm;
   return sm.apply(alpha, x, beta, y, trans, sweeps, L.a.nr);
+}
+// The cycle from level lev down (inner_ml_aply(lev)) on x: y = B_lev x.
+int orc_prec_level_cycle(void* h, int lev, const double* x, double* y, char trans) {
+  Prec& P = *(Prec*)h;
+  if (lev < 0 || lev >= (int)P.lv.size()) return -2;
+  Level& L = P.lv[lev];
+  const ix n = L.a.nr;
+  axpby(n, 1.0, x, 0.0, L.x2l.data());
+  std::fill(L.y2l.begin(), L.y2l.end(), 0.0);
+  int rc = P.inner_ml_aply(lev, trans);
+  if (rc) return rc;
+  axpby(n, 1.0, L.y2l.data(), 0.0, y);
+  return 0;
 }
 int orc_krylov(void* h, void* a, int method, const double* b, double* x, double eps, int itmax, int* iters,
                double* err) {

@@ -22,7 +22,7 @@ _lib = None
 # Smoother codes: mld_base_prec_type.F90:166-176 (0..2); 100+ are additions.
 SM_NOPREC, SM_JAC, SM_BJAC, SM_L1JAC, SM_DENSELU = 0, 1, 2, 100, 101
 SM_FBGS, SM_GS, SM_BWGS = 4, 102, 103
-CYC_V, CYC_W = 3, 4
+CYC_ADD, CYC_MULT, CYC_V, CYC_W, CYC_K, CYC_KSYM = 1, 2, 3, 4, 5, 6
 KRYLOV = {"CG": 0, "FCG": 1, "BICGSTAB": 2}
 
 
@@ -96,6 +96,7 @@ def lib():
         L.orc_prec_apply.argtypes = [vp, dp, dp, C.c_char]
         L.orc_prec_smooth.argtypes = [vp, C.c_int, C.c_double, dp, C.c_double, dp, C.c_char, C.c_int]
         L.orc_prec_smooth_w.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, C.c_double, dp, C.c_char, C.c_int]
+        L.orc_prec_level_cycle.argtypes = [vp, C.c_int, dp, dp, C.c_char]
         L.orc_krylov.argtypes = [vp, vp, C.c_int, dp, dp, C.c_double, C.c_int, ip, dp]
         _lib = L
     return _lib
@@ -277,6 +278,15 @@ class Prec:
         rc = lib().orc_prec_apply(self.h, _dp(x), _dp(y), trans.encode())
         if rc:
             raise RuntimeError(f"oracle apply failed: {rc}")
+        return y
+
+    def level_cycle(self, lev, x, trans="N"):
+        """The cycle from level lev down applied to x (inner_ml_aply)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        rc = lib().orc_prec_level_cycle(self.h, lev, _dp(x), _dp(y), trans.encode())
+        if rc:
+            raise RuntimeError(f"oracle level cycle failed: {rc}")
         return y
 
     def smooth(self, lev, x, alpha=1.0, beta=0.0, y=None, trans="N", sweeps=1, which=0):

@@ -35,7 +35,8 @@ def engine_prec(eng, A, smoother="JACOBI", sweeps=1, coarse="BJAC", csweeps=4, m
     return p.build(A)
 
 
-SM = {"JACOBI": 1, "BJAC": 2, "L1-JACOBI": 100, "FBGS": 4}
+SM = {"NOPREC": 0, "JACOBI": 1, "BJAC": 2, "L1-JACOBI": 100, "FBGS": 4}
+CYC = {"ADD": 1, "MULT": 2, "VCYCLE": 3, "WCYCLE": 4, "KCYCLE": 5, "KCYCLESYM": 6}
 
 
 def oracle_prec(orc, a, smoother="JACOBI", sweeps=1, coarse="BJAC", csweeps=4, mcs=200, **kw):
@@ -200,6 +201,52 @@ def test_krylov_default_ml(eng, orc):
     assert err <= 1e-8 and abs(it - ito) <= 1
     r = b - A @ x.cpu().numpy()
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-8
+
+
+@pytest.mark.parametrize("cycle,smoother,sweeps,idim,mcs", [
+    ("ADD", "JACOBI", 1, 16, 40), ("ADD", "FBGS", 1, 16, 40), ("MULT", "L1-JACOBI", 2, 16, 40),
+    ("KCYCLE", "JACOBI", 2, 24, 40), ("KCYCLESYM", "JACOBI", 2, 24, 40),
+    ("KCYCLE", "NOPREC", 1, 12, 60), ("KCYCLESYM", "NOPREC", 1, 12, 60), ("KCYCLESYM", "FBGS", 1, 30, 40)])
+def test_cycles_vs_oracle(eng, orc, cycle, smoother, sweeps, idim, mcs):
+    """ADD, MULT and K-cycles (GCR and FCG inner iterations; the NOPREC
+    smoother makes the inner iteration take its second step) against the
+    oracle.  The K-cycle's inner dots are reduced in blocks on the device and
+    sequentially in the oracle, so agreement is to rounding, not bit-exact."""
+    _say(f"[cycle] {cycle} {smoother} sweeps={sweeps} idim={idim}")
+    a = orc.gen_pde3d(idim)
+    A = a.to_scipy()
+    p = engine_prec(eng, A, smoother, sweeps, mcs=mcs, extra=(("ML_CYCLE", cycle),))
+    po = oracle_prec(orc, a, smoother, sweeps, mcs=mcs, ml_cycle=CYC[cycle])
+    assert p.nlev == po.nlev
+    if cycle.startswith("K"):
+        assert p.nlev >= 3
+    rng = np.random.default_rng(idim + 7)
+    for trial in range(2):
+        x = rng.standard_normal(A.shape[0])
+        y = p.apply(dev(x)).cpu().numpy()
+        assert relmax(y, po.apply(x)) <= 1e-11, trial
+    if smoother in ("JACOBI", "NOPREC"):
+        x = rng.standard_normal(A.shape[0])
+        y = p.apply(dev(x), trans="T").cpu().numpy()
+        assert relmax(y, po.apply(x, trans="T")) <= 1e-11
+
+
+def test_kcycle_mixed_and_krylov(eng, orc):
+    """FCG with the symmetric K-cycle (the reference pairs them): FP64 and
+    FP32-mixed hierarchies reach the tolerance in the oracle's iteration
+    count (+-1)."""
+    _say("[K-cycle krylov] idim=32")
+    a = orc.gen_pde3d(32)
+    A = a.to_scipy()
+    po = oracle_prec(orc, a, "L1-JACOBI", 1, mcs=40, ml_cycle=CYC["KCYCLESYM"])
+    b = np.ones(A.shape[0])
+    xo, ito, erro = po.krylov("FCG", b, eps=1e-8)
+    for prec in ("double", "mixed"):
+        p = engine_prec(eng, A, "L1-JACOBI", 1, mcs=40, precision=prec, extra=(("ML_CYCLE", "KCYCLESYM"),))
+        x, it, err = eng.psb_krylov("FCG", A, p, dev(b), eps=1e-8)
+        assert err <= 1e-8 and abs(it - ito) <= 1, (prec, it, ito)
+        r = b - A @ x.cpu().numpy()
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-8
 
 
 @pytest.mark.parametrize("method", ["CG", "FCG", "BICGSTAB"])

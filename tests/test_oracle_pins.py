@@ -351,6 +351,94 @@ def test_fbgs_one_level_symmetrized_pair(orc, nsw):
         p.apply(x, trans="T")
 
 
+# ------------------------------------------------------- ADD and K cycles
+@pytest.mark.parametrize("smoother", ["JACOBI", "FBGS"])
+def test_add_cycle_closed_form(orc, smoother):
+    """Additive two-level cycle with an exact coarse solve:
+    M = S + P A_c^-1 R, S = D^-1 (one Jacobi sweep) or (D+U)^-1 (D+L)^-1 (the
+    GS / BWGS pair of FBGS, applied in turn)."""
+    a = orc.gen_pde3d(6)
+    A = a.to_scipy().toarray()
+    sm = orc.SM_JAC if smoother == "JACOBI" else orc.SM_FBGS
+    prm = orc.config1_params(smoother=sm, min_coarse_size=1, max_levs=2, c_smoother=orc.SM_DENSELU, c_sweeps=1,
+                             ml_cycle=orc.CYC_ADD)
+    p = orc.Prec(prm).build(a)
+    assert p.nlev == 2
+    P = p.level_csr(1, "P").to_scipy().toarray()
+    R = p.level_csr(1, "R").to_scipy().toarray()
+    Ac = p.level_csr(1, "A").to_scipy().toarray()
+    if smoother == "JACOBI":
+        S = np.diag(1.0 / np.diag(A))
+    else:
+        S = np.linalg.solve(np.triu(A), np.linalg.inv(np.tril(A)))
+    M = S + P @ np.linalg.solve(Ac, R)
+    x = np.random.default_rng(31).standard_normal(A.shape[0])
+    assert np.allclose(p.apply(x), M @ x, rtol=1e-10, atol=1e-13 * np.abs(M @ x).max())
+
+
+def test_kcycle_two_levels_is_vcycle(orc):
+    """With two levels the K-cycle has no inner iteration: pre-smoothing,
+    coarse correction, post-smoothing, i.e. the V-cycle (and MULT is the
+    V-cycle), bit for bit."""
+    a = orc.gen_pde3d(8)
+    x = np.random.default_rng(32).standard_normal(a.shape[0])
+    ys = []
+    for cyc in (orc.CYC_V, orc.CYC_MULT, orc.CYC_K, orc.CYC_KSYM):
+        prm = orc.config1_params(smoother=orc.SM_JAC, sweeps_pre=2, sweeps_post=2, min_coarse_size=60, max_levs=2,
+                                 ml_cycle=cyc)
+        p = orc.Prec(prm).build(a)
+        assert p.nlev == 2
+        ys.append(p.apply(x))
+    assert all((y == ys[0]).all() for y in ys[1:])
+
+
+@pytest.mark.parametrize("cyc", ["K", "KSYM"])
+@pytest.mark.parametrize("case", ["one_step", "two_steps"])
+def test_kcycle_inner_iteration_is_krylov_projection(orc, cyc, case):
+    """Three levels: the level-1 correction of the K-cycle is the K-cycle
+    iterate of at most two steps preconditioned by B_1 (the level-1 cycle),
+    here formed independently as a projection onto span{d0, d1}, d0 = B_1 r,
+    d1 = B_1 (r - alpha A_1 d0): the A_1-orthogonal (Galerkin) projection for
+    the symmetric variant (FCG), the least-squares residual minimiser for
+    GCR; one step (x = alpha d0) when ||r - alpha A_1 d0|| <= ||r|| / 4.
+    A weak B_1 (no smoother) forces the second step."""
+    a = orc.gen_pde3d(10 if case == "one_step" else 12)
+    ml = orc.CYC_K if cyc == "K" else orc.CYC_KSYM
+    sm = orc.SM_JAC if case == "one_step" else orc.SM_NOPREC
+    prm = orc.config1_params(smoother=sm, min_coarse_size=20 if case == "one_step" else 60, ml_cycle=ml)
+    p = orc.Prec(prm).build(a)
+    assert p.nlev == 3
+    A0 = a.to_scipy()
+    A1 = p.level_csr(1, "A").to_scipy()
+    P1 = p.level_csr(1, "P").to_scipy()
+    R1 = p.level_csr(1, "R").to_scipy()
+    B1 = np.column_stack([p.level_cycle(1, e) for e in np.eye(A1.shape[0])])
+    rng = np.random.default_rng(33)
+    for _ in range(3):
+        x = rng.standard_normal(A0.shape[0])
+        y = p.smooth(0, x)
+        r = R1 @ (x - A0 @ y)
+        d0 = B1 @ r
+        v = A1 @ d0
+        alpha = (d0 @ r) / (d0 @ v) if cyc == "KSYM" else (v @ r) / (v @ v)
+        w = r - alpha * v
+        one = np.linalg.norm(w) <= 0.25 * np.linalg.norm(r)
+        assert one == (case == "one_step")
+        if one:
+            y1 = alpha * d0
+        else:
+            D = np.column_stack([d0, B1 @ w])
+            if cyc == "KSYM":
+                c = np.linalg.solve(D.T @ (A1 @ D), D.T @ r)
+            else:
+                c = np.linalg.lstsq(A1 @ D, r, rcond=None)[0]
+            y1 = D @ c
+        y = y + P1 @ y1
+        y = p.smooth(0, x - A0 @ y, beta=1.0, y=y)
+        yo = p.apply(x)
+        assert np.abs(yo - y).max() <= 1e-12 * np.abs(yo).max()
+
+
 # ----------------------------------------------------------------- Krylov
 @pytest.mark.parametrize("method", ["CG", "FCG", "BICGSTAB"])
 def test_krylov_small_exact(orc, method):
