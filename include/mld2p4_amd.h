@@ -179,6 +179,15 @@ int mldhip_get_omega(mldhip_prec* p, int lev, double* omega);
 int mldhip_get_complexity(mldhip_prec* p, double* cplx);
 int mldhip_get_device_bytes(mldhip_prec* p, int64_t* bytes);
 int mldhip_get_error(mldhip_prec* p, char* buf, int len);
+/* prec%dump (mlprec/mld_d_prec_type.f90:698-729, per level
+ * mld_d_base_onelev_dump.f90:38-131): MatrixMarket files of levels
+ * istart..iend (1-based; <= 0: from level 2, or 1 when there is one level,
+ * to the last), named <prefix>_p000_lLLL_{ac,r,p}.mtx and, for the
+ * smoothers, <prefix>_sm[2a]_p000_lLLL_{nd,diag,lower,upper}.mtx.  flags:
+ * 1 = ac (A of levels >= 2), 2 = rp (R and P), 4 = smoother (ND),
+ * 8 = solver (diagonal / triangular factors).  prefix NULL or "": "dump_lev_d";
+ * head (may be NULL): a comment line of every file. */
+int mldhip_dump(mldhip_prec* p, int istart, int iend, const char* prefix, const char* head, int flags);
 
 /* ---- single-kernel entry points (parity tests of the device kernels) ---- */
 /* y = M x with the device copy of a level operator (what 0 = A, 1 = P,

@@ -74,6 +74,7 @@ def lib():
             "mldhip_get_level_dv": [vp, C.c_int, C.c_int, dp],
             "mldhip_get_omega": [vp, C.c_int, dp],
             "mldhip_get_complexity": [vp, dp],
+            "mldhip_dump": [vp, C.c_int, C.c_int, cp, cp, C.c_int],
             "mldhip_get_device_bytes": [vp, i64p],
             "mldhip_get_error": [vp, C.c_char_p, C.c_int],
             "mldhip_level_spmv": [vp, C.c_int, C.c_int, vp, vp],
@@ -314,6 +315,16 @@ class MldPrec:
         v = C.c_double()
         self._check(lib().mldhip_get_complexity(self.h, C.byref(v)), "get_complexity")
         return v.value
+
+    def dump(self, prefix=None, istart=None, iend=None, head=None, ac=False, rp=False, smoother=False,
+             solver=False):
+        """prec%dump (mlprec/mld_d_prec_type.f90:698-729): MatrixMarket files
+        <prefix>_p000_lLLL_{ac,r,p}.mtx and <prefix>_sm[2a]_p000_lLLL_
+        {nd,diag,lower,upper}.mtx for levels istart..iend (1-based)."""
+        flags = (1 if ac else 0) | (2 if rp else 0) | (4 if smoother else 0) | (8 if solver else 0)
+        self._check(lib().mldhip_dump(self.h, istart or 0, iend or 0, (prefix or "").encode(),
+                                      (head or "").encode(), flags), "dump")
+        return self
 
     def device_bytes(self):
         v = C.c_int64()

@@ -325,6 +325,15 @@ int mldhip_get_complexity(mldhip_prec* p, double* c) {
   return 0;
 }
 
+int mldhip_dump(mldhip_prec* p, int istart, int iend, const char* prefix, const char* head, int flags) {
+  if (!p) return E_BAD_ARG;
+  if (!p->host.hier_built) return fail(p, 3112, "dump: hierarchy not built");
+  std::string err;
+  int rc = mld::dump_prec(p->host, istart, iend, prefix ? prefix : "", head ? head : "", flags, err);
+  if (rc) return fail(p, rc, err);
+  return 0;
+}
+
 int mldhip_get_device_bytes(mldhip_prec* p, int64_t* bytes) {
   if (!p || !bytes) return E_BAD_ARG;
   *bytes = p->dev ? (int64_t)p->dev->device_bytes() : 0;

@@ -138,5 +138,8 @@ int aggrmat_smth_asb(const HCsr& a, const std::vector<int>& ilaggr, int naggr, L
 int ilu0_factor(const HCsr& a, HCsr& l, HCsr& u, std::vector<double>& d);
 void level_schedule(const HCsr& t, bool lower, std::vector<int>& ptr, std::vector<int>& rows);
 double operator_complexity(const Prec& p);
+// prec%dump: flags 1 = ac, 2 = rp, 4 = smoother, 8 = solver; istart/iend <= 0: defaults
+int dump_prec(const Prec& p, int istart, int iend, const std::string& prefix, const std::string& head, int flags,
+              std::string& err);
 
 }  // namespace mld

@@ -8,6 +8,7 @@
 program mld_d_pde3d
   use iso_c_binding
   use mld_d_prec_mod
+  use mld_data_input
   implicit none
 
   type(psb_dspmat_type) :: a
@@ -21,35 +22,35 @@ program mld_d_pde3d
   real(psb_dpk_) :: eps, thr, thr2, mncrratio, athres, cthres, err, t0, t1, tprec, tsolve, rn, bn
   real(psb_dpk_), allocatable :: athresv(:)
 
-  call read_data_s(afmt); call read_data_i(idim)
-  call read_data_s(kmethd); call read_data_i(istopc); call read_data_i(itmax); call read_data_i(itrace)
-  call read_data_i(irst); call read_data_r(eps)
-  call read_data_s(descr); call read_data_s(ptype)
-  call read_data_s(smther); call read_data_i(jsweeps); call read_data_i(novr); call read_data_s(restr)
-  call read_data_s(prol); call read_data_s(solve); call read_data_i(fill); call read_data_r(thr)
-  call read_data_s(smther2); call read_data_i(jsweeps2); call read_data_i(novr2); call read_data_s(restr2)
-  call read_data_s(prol2); call read_data_s(solve2); call read_data_i(fill2); call read_data_r(thr2)
-  call read_data_s(mlcycle); call read_data_i(outer_sweeps); call read_data_i(maxlevs); call read_data_i(csize)
-  call read_data_s(aggr_prol); call read_data_s(par_aggr_alg); call read_data_s(aggr_ord); call read_data_s(aggr_filter)
-  call read_data_r(mncrratio); call read_data_i(thrvsz)
+  call read_data(afmt, 5); call read_data(idim, 5)
+  call read_data(kmethd, 5); call read_data(istopc, 5); call read_data(itmax, 5); call read_data(itrace, 5)
+  call read_data(irst, 5); call read_data(eps, 5)
+  call read_data(descr, 5); call read_data(ptype, 5)
+  call read_data(smther, 5); call read_data(jsweeps, 5); call read_data(novr, 5); call read_data(restr, 5)
+  call read_data(prol, 5); call read_data(solve, 5); call read_data(fill, 5); call read_data(thr, 5)
+  call read_data(smther2, 5); call read_data(jsweeps2, 5); call read_data(novr2, 5); call read_data(restr2, 5)
+  call read_data(prol2, 5); call read_data(solve2, 5); call read_data(fill2, 5); call read_data(thr2, 5)
+  call read_data(mlcycle, 5); call read_data(outer_sweeps, 5); call read_data(maxlevs, 5); call read_data(csize, 5)
+  call read_data(aggr_prol, 5); call read_data(par_aggr_alg, 5); call read_data(aggr_ord, 5); call read_data(aggr_filter, 5)
+  call read_data(mncrratio, 5); call read_data(thrvsz, 5)
   allocate(athresv(max(1, thrvsz)))
   if (thrvsz > 0) then
-    call read_data_rv(athresv, thrvsz)
+    call read_data(athresv(1:thrvsz), 5)
   else
-    call read_data_s(restr)   ! thresholds line, ignored
+    call read_data(restr, 5)   ! thresholds line, ignored
   end if
-  call read_data_r(athres)
-  call read_data_s(csolve); call read_data_s(csbsolve); call read_data_s(cmat); call read_data_i(cfill)
-  call read_data_r(cthres); call read_data_i(cjswp)
+  call read_data(athres, 5)
+  call read_data(csolve, 5); call read_data(csbsolve, 5); call read_data(cmat, 5); call read_data(cfill, 5)
+  call read_data(cthres, 5); call read_data(cjswp, 5)
   prec_kind = 'DOUBLE'
-  call read_data_opt(prec_kind)
+  call read_data_opt(prec_kind, 5)
 
-  call cpu_time(t0)
+  t0 = psb_wtime()
   call gen_pde3d(idim, a, b)
   desc_a%n = a%m
   allocate(x(a%m), r(a%m))
   x = 0.0_psb_dpk_
-  call cpu_time(t1)
+  t1 = psb_wtime()
   write(*, '("Overall matrix creation time : ",es12.5)') t1 - t0
 
   ! device from MLD_HIP_DEVICE (default 0; -1 builds on the host only)
@@ -110,10 +111,10 @@ program mld_d_pde3d
   end select
   call prec%set_precision(trim(prec_kind), info)
 
-  call cpu_time(t0)
+  t0 = psb_wtime()
   call prec%hierarchy_build(a, desc_a, info)
   if (info == 0) call prec%smoothers_build(a, desc_a, info)
-  call cpu_time(t1)
+  t1 = psb_wtime()
   if (info /= 0) then
     write(0, *) 'preconditioner build failed, info=', info
     stop 2
@@ -121,10 +122,10 @@ program mld_d_pde3d
   tprec = t1 - t0
   call prec%descr()
 
-  call cpu_time(t0)
+  t0 = psb_wtime()
   call psb_krylov(kmethd, a, prec, b, x, eps, desc_a, info, itmax=itmax, iter=iter, err=err, &
        & itrace=itrace, irst=irst, istop=istopc)
-  call cpu_time(t1)
+  t1 = psb_wtime()
   tsolve = t1 - t0
   if (info /= 0) then
     write(0, *) 'solve failed, info=', info
@@ -144,69 +145,6 @@ program mld_d_pde3d
   call prec%free(info)
 
 contains
-
-  ! read_data (tests/pdegen/data_input.f90:41-56): skip '%' comment lines,
-  ! take the first blank-delimited token, drop a trailing '!' comment.
-  subroutine next_token(tok)
-    character(len=*), intent(out) :: tok
-    character(len=256) :: line
-    integer :: ios, k
-    do
-      read(*, '(a)', iostat=ios) line
-      if (ios /= 0) then
-        tok = ' '
-        return
-      end if
-      line = adjustl(line)
-      if (len_trim(line) == 0) cycle
-      if (line(1:1) == '%') cycle
-      exit
-    end do
-    k = index(line, '!')
-    if (k > 0) line = line(1:k-1)
-    read(line, '(a)') tok
-    k = index(trim(tok), ' ')
-    if (k > 0) tok = tok(1:k-1)
-  end subroutine next_token
-  subroutine read_data_s(s)
-    character(len=*), intent(out) :: s
-    call next_token(s)
-  end subroutine read_data_s
-  subroutine read_data_opt(s)
-    character(len=*), intent(inout) :: s
-    character(len=40) :: t
-    call next_token(t)
-    if (len_trim(t) > 0) s = t
-  end subroutine read_data_opt
-  subroutine read_data_i(v)
-    integer(psb_ipk_), intent(out) :: v
-    character(len=64) :: t
-    call next_token(t)
-    read(t, *) v
-  end subroutine read_data_i
-  subroutine read_data_r(v)
-    real(psb_dpk_), intent(out) :: v
-    character(len=64) :: t
-    integer :: k
-    call next_token(t)
-    k = index(t, 'd'); if (k > 0) t(k:k) = 'e'
-    k = index(t, 'D'); if (k > 0) t(k:k) = 'e'
-    read(t, *) v
-  end subroutine read_data_r
-  subroutine read_data_rv(v, n)
-    real(psb_dpk_), intent(out) :: v(:)
-    integer(psb_ipk_), intent(in) :: n
-    character(len=256) :: line
-    integer :: ios
-    do
-      read(*, '(a)', iostat=ios) line
-      if (ios /= 0) return
-      line = adjustl(line)
-      if (len_trim(line) == 0 .or. line(1:1) == '%') cycle
-      exit
-    end do
-    read(line, *) v(1:n)
-  end subroutine read_data_rv
 
   ! mld_d_gen_pde3d with a1=a2=a3=1, b=c=0; b = 1
   subroutine gen_pde3d(idim, a, b)

@@ -57,6 +57,7 @@ module mld_d_prec_mod
     procedure, pass(prec) :: descr => mld_dprecdescr
     procedure, pass(prec) :: get_nlevs => mld_dprec_get_nlevs
     procedure, pass(prec) :: get_complexity => mld_dprec_get_complexity
+    procedure, pass(prec) :: dump => mld_dprec_dump
   end type mld_dprec_type
 
 contains
@@ -155,6 +156,32 @@ contains
       a%irp(i+1) = w + 1
     end do
   end subroutine psb_spasb_coo
+
+  ! wall-clock seconds (psb_wtime)
+  real(psb_dpk_) function psb_wtime()
+    integer(c_int64_t) :: cnt, rate
+    call system_clock(cnt, rate)
+    psb_wtime = real(cnt, psb_dpk_)/real(rate, psb_dpk_)
+  end function psb_wtime
+
+  ! drop stored zeros (a%clean_zeros), rows keep their order
+  subroutine psb_clean_zeros(a)
+    type(psb_dspmat_type), intent(inout) :: a
+    integer(psb_ipk_) :: i
+    integer(psb_epk_) :: k, w, s
+    w = 0
+    do i = 1, a%m
+      s = a%irp(i)
+      a%irp(i) = w + 1
+      do k = s, a%irp(i+1) - 1
+        if (a%val(k) /= 0.0_psb_dpk_) then
+          w = w + 1
+          a%ja(w) = a%ja(k); a%val(w) = a%val(k)
+        end if
+      end do
+    end do
+    a%irp(a%m+1) = w + 1
+  end subroutine psb_clean_zeros
 
   ! ---------------------------------------------------------------- error text
   subroutine report(prec, info, where)
@@ -360,6 +387,39 @@ contains
     info = mldhip_get_complexity(prec%h, c)
     mld_dprec_get_complexity = c
   end function mld_dprec_get_complexity
+
+  ! prec%dump (mlprec/mld_d_prec_type.f90:698-729): MatrixMarket files of
+  ! levels istart..iend in the reference's names (<prefix>_p000_lLLL_ac.mtx,
+  ! _r, _p; <prefix>_sm[2a]_p000_lLLL_nd/_diag/_lower/_upper)
+  subroutine mld_dprec_dump(prec, info, istart, iend, prefix, head, ac, rp, smoother, solver, global_num)
+    class(mld_dprec_type), intent(in) :: prec
+    integer(psb_ipk_), intent(out) :: info
+    integer(psb_ipk_), intent(in), optional :: istart, iend
+    character(len=*), intent(in), optional :: prefix, head
+    logical, optional, intent(in) :: ac, rp, smoother, solver, global_num
+    integer(c_int) :: il1, iln, flags
+    character(len=256) :: pf, hd
+    il1 = 0; iln = 0; flags = 0; pf = ' '; hd = ' '
+    if (present(istart)) il1 = istart
+    if (present(iend)) iln = iend
+    if (present(prefix)) pf = prefix
+    if (present(head)) hd = head
+    if (present(ac)) then
+      if (ac) flags = ior(flags, 1)
+    end if
+    if (present(rp)) then
+      if (rp) flags = ior(flags, 2)
+    end if
+    if (present(smoother)) then
+      if (smoother) flags = ior(flags, 4)
+    end if
+    if (present(solver)) then
+      if (solver) flags = ior(flags, 8)
+    end if
+    ! global_num: one process, so global and local numbering coincide
+    info = mldhip_dump(prec%h, il1, iln, cstr(pf), cstr(hd), flags)
+    call report(prec, info, 'dump')
+  end subroutine mld_dprec_dump
 
   ! prec%descr (mlprec/impl/mld_dfile_prec_descr.f90:118-180), abridged
   subroutine mld_dprecdescr(prec, iout)

@@ -153,6 +153,13 @@ module mld_hip_cbind
       integer(c_int), value :: kind
       integer(c_int) :: e
     end function hipMemcpy
+    function mldhip_dump(p, istart, iend, prefix, head, flags) bind(c, name='mldhip_dump') result(info)
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: p
+      integer(c_int), value :: istart, iend, flags
+      character(kind=c_char), dimension(*) :: prefix, head
+      integer(c_int) :: info
+    end function mldhip_dump
     function hipDeviceSynchronize() bind(c, name='hipDeviceSynchronize') result(e)
       import :: c_int
       integer(c_int) :: e

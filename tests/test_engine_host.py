@@ -221,3 +221,50 @@ def test_diffusion_generator_properties(eng):
     assert np.isclose(A[i, i + idim * idim], -2 * k[i] * k[i + idim * idim] / (k[i] + k[i + idim * idim]) * idim ** 2,
                       rtol=1e-14)
     assert b.min() >= -1.0 and b.max() <= 1.0
+
+
+def test_dump_matrix_market(eng, orc, tmp_path):
+    """prec%dump: file names of mld_d_base_onelev_dump / the smoother and solver
+    dumps, contents equal to the level data (17 significant digits: exact);
+    GS writes tril(A) with the diagonal and BWGS triu(A) with the diagonal, as
+    their builds store them."""
+    import scipy.io
+    import scipy.sparse as sp
+
+    a = orc.gen_pde3d(12)
+    A = a.to_scipy()
+    p = config1(eng, "FBGS", 1).build(A)
+    assert p.nlev >= 3
+    prefix = str(tmp_path / "mld")
+    p.dump(prefix, istart=1, ac=True, rp=True, smoother=True, solver=True, head="test dump")
+
+    def rd(name):
+        m = scipy.io.mmread(f"{prefix}{name}")
+        return sp.csr_matrix(m) if sp.issparse(m) else np.asarray(m).ravel()
+
+    def eq(m1, m2):
+        d = (sp.csr_matrix(m1) - sp.csr_matrix(m2))
+        return m1.shape == m2.shape and (d.count_nonzero() == 0)
+
+    assert not os.path.exists(f"{prefix}_p000_l001_ac.mtx")  # level 1 holds no ac
+    for lev in range(2, p.nlev + 1):
+        l = lev - 1
+        assert eq(rd(f"_p000_l{lev:03d}_ac.mtx"), p.level_csr(l, "A"))
+        assert eq(rd(f"_p000_l{lev:03d}_r.mtx"), p.level_csr(l, "R"))
+        assert eq(rd(f"_p000_l{lev:03d}_p.mtx"), p.level_csr(l, "P"))
+    for lev in range(1, p.nlev):
+        Al = p.level_csr(lev - 1, "A")
+        assert eq(rd(f"_sm_p{0:03d}_l{lev:03d}_lower.mtx"), sp.tril(Al, 0))
+        assert eq(rd(f"_sm_p000_l{lev:03d}_upper.mtx"), sp.triu(Al, 1))
+        assert eq(rd(f"_sm2a_p000_l{lev:03d}_lower.mtx"), sp.tril(Al, -1))
+        assert eq(rd(f"_sm2a_p000_l{lev:03d}_upper.mtx"), sp.triu(Al, 0))
+        assert rd(f"_sm_p000_l{lev:03d}_nd.mtx").nnz == 0
+    lc = p.nlev
+    assert eq(rd(f"_sm_p000_l{lc:03d}_lower.mtx"), p.level_csr(lc - 1, "L"))
+    assert eq(rd(f"_sm_p000_l{lc:03d}_upper.mtx"), p.level_csr(lc - 1, "U"))
+    assert (rd(f"_sm_p000_l{lc:03d}_diag.mtx") == p.dv(lc - 1)).all()
+    assert "% test dump" in open(f"{prefix}_p000_l002_ac.mtx").read(200)
+    # default range: levels 2..nlev, level data only when asked
+    q = str(tmp_path / "d")
+    p.dump(q, ac=True)
+    assert os.path.exists(f"{q}_p000_l002_ac.mtx") and not os.path.exists(f"{q}_p000_l002_r.mtx")

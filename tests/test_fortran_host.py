@@ -64,3 +64,72 @@ def test_fortran_driver_gpu_mixed(exe):
     r = run_driver(0, txt)
     assert r.returncode == 0, r.stdout + r.stderr
     assert field(r.stdout, "True relative residual") <= 1.01e-6
+
+
+# ------------------------------------------------------------ file-reading driver
+FILEREAD = os.path.join(ROOT, "mld2p4-2_amd", "_build", "fortran", "mld_d_fileread")
+DFS = os.path.join(ROOT, "mld2p4-2_amd", "fortran", "runs", "mld_dfs.inp")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def run_fileread(device, deck_text=None):
+    env = dict(os.environ, MLD_HIP_DEVICE=str(device))
+    txt = deck_text if deck_text is not None else open(DFS).read()
+    return subprocess.run([FILEREAD], input=txt, capture_output=True, text=True, env=env, timeout=300, cwd=GOLD)
+
+
+def dfs_oracle(orc):
+    """The deck's configuration in the oracle: FBGS 1/1, four outer sweeps,
+    filtered smoothed aggregation with the default threshold and coarse size,
+    one ILU(0) sweep on the coarsest level."""
+    a = orc.Csr.from_scipy(orc.read_mtx(os.path.join(GOLD, "mld_mat.mtx")))
+    prm = orc.Params(smoother=orc.SM_FBGS, sweeps_pre=1, sweeps_post=1, outer_sweeps=4, aggr_filter=1,
+                     c_smoother=orc.SM_BJAC, c_sweeps=1)
+    return a, orc.Prec(prm).build(a)
+
+
+def test_fileread_driver_host_build_and_dump(orc, tmp_path):
+    """mm_mat_read / mm_array_read of the reference's golden files, the deck of
+    tests/fileread/runs/mld_dfs.inp (coarse UMF replaced by ILU(0)), and
+    prec%dump: level sizes and the dumped level-2 matrix equal the oracle's."""
+    if not os.path.exists(FILEREAD):
+        pytest.fail(f"Fortran driver not built: {FILEREAD} (run __graft_entry__.build())")
+    import scipy.io
+
+    prefix = str(tmp_path / "h")
+    txt = open(DFS).read().replace("NONE                        ! prec%dump", prefix + " ! prec%dump")
+    r = run_fileread(-1, txt)
+    assert r.returncode == 3, r.stdout + r.stderr  # host-only handle: no device solve
+    a, po = dfs_oracle(orc)
+    sizes = [int(s) for s in re.findall(r"level \d+: n=(\d+)", r.stdout)]
+    assert sizes == [po.level_csr(l, "A").shape[0] for l in range(po.nlev)]
+    ac = scipy.io.mmread(prefix + "_p000_l002_ac.mtx").tocsr()
+    ref = po.level_csr(1, "A").to_scipy()
+    assert ac.shape == ref.shape and abs(ac - ref).max() == 0
+    assert os.path.exists(prefix + "_sm2a_p000_l001_upper.mtx")
+
+
+@pytest.mark.gpu
+def test_fileread_driver_gpu_golden(orc):
+    """The golden system of the reference (mld_mat / mld_rhs / mld_sol): CG to
+    1e-6 in the oracle's iteration count, and the solution's distance to the
+    reference solution equal to the oracle's to rounding."""
+    print("[fortran fileread] golden matrix, FBGS V-cycle, 4 outer sweeps", flush=True)
+    if not os.path.exists(FILEREAD):
+        pytest.fail(f"Fortran driver not built: {FILEREAD} (run __graft_entry__.build())")
+    import scipy.io
+
+    r = run_fileread(0)
+    assert r.returncode == 0, r.stdout + r.stderr
+    it = field(r.stdout, "Iterations to convergence")
+    err = field(r.stdout, "Relative error estimate on exit")
+    a, po = dfs_oracle(orc)
+    b = np.asarray(scipy.io.mmread(os.path.join(GOLD, "mld_rhs.mtx"))).ravel()
+    xs = np.asarray(scipy.io.mmread(os.path.join(GOLD, "mld_sol.mtx"))).ravel()
+    xo, ito, erro = po.krylov("CG", b, eps=1e-6, itmax=500)
+    assert err <= 1e-6 and abs(it - ito) <= 1
+    lines = r.stdout.splitlines()
+    k = [i for i, s in enumerate(lines) if "(2-norm)" in s][0]
+    rel = float(lines[k + 1].split()[2])
+    rel_o = np.linalg.norm(xo - xs) / np.linalg.norm(xs)
+    assert abs(rel - rel_o) <= 1e-3 * rel_o + 1e-12, (rel, rel_o)
