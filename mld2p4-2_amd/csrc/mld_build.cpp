@@ -14,11 +14,19 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "mld_host.hpp"
 
 namespace mld {
+
+double wall_seconds() { return omp_get_wtime(); }
+void phase_log(const char* what, int level, double seconds) {
+  static const bool on = std::getenv("MLD_VERBOSE") != nullptr;
+  if (on) std::fprintf(stderr, "[mld] level %d %-22s %9.3f ms\n", level, what, seconds * 1e3);
+}
 
 static std::vector<double> diag_of(const HCsr& a) {
   std::vector<double> d(a.nr, 0.0);
@@ -242,11 +250,19 @@ int aggrmat_smth_asb(const HCsr& a, const std::vector<int>& ilaggr, int naggr, L
   for (int i = 0; i <= n; ++i) pt.irp[i] = i;
   pt.ja.assign(ilaggr.begin(), ilaggr.end());
   pt.val.assign(n, 1.0);
+  double t0 = wall_seconds();
   spgemm(s, pt, p);        // P  = (I - omega D^-1 A) Ptilde
+  double t1 = wall_seconds();
+  phase_log("P = S Ptilde", -1, t1 - t0);
   HCsr ap;
   spgemm(a, p, ap);        // AP = A P
+  double t2 = wall_seconds();
+  phase_log("A P", -1, t2 - t1);
   transpose(p, r);         // R  = P^T
+  double t3 = wall_seconds();
+  phase_log("R = P^T", -1, t3 - t2);
   spgemm(r, ap, ac);       // Ac = R (A P)
+  phase_log("Ac = R (A P)", -1, wall_seconds() - t3);
   return 0;
 }
 
@@ -507,8 +523,10 @@ int hierarchy_build(Prec& p, int n, const int64_t* irp, const int* ja, const dou
     std::vector<int> ilaggr;
     int naggr = 0;
     if (cur.parms.par_aggr_alg != DEC_AGGR) return E_UNSUPPORTED;
+    double t0 = wall_seconds();
     int rc = dec_map_bld(cur.parms.aggr_ord, cur.parms.aggr_thresh, prev.a, ilaggr, naggr);
     if (rc) return rc;
+    phase_log("aggregation", i, wall_seconds() - t0);
     double sizeratio = (double)naggr;
     sizeratio = (i == 2) ? (double)n / sizeratio : (double)prev.naggr / sizeratio;
     cur.szratio = sizeratio;
@@ -546,9 +564,13 @@ int hierarchy_build(Prec& p, int n, const int64_t* irp, const int* ja, const dou
 int smoothers_build(Prec& p) {
   if (!p.initialized) return E_NOT_INIT;
   if (!p.hier_built) return E_NOT_BUILT;
+  int lev = 0;
   for (auto& lv : p.precv) {
+    ++lev;
     if (!lv.sm.allocated) return E_NO_SOLVER;
+    double t0 = wall_seconds();
     int rc = build_smoother(lv.a, lv.sm, lv.smd);
+    phase_log("smoother build", lev, wall_seconds() - t0);
     if (rc) return rc;
     if (lv.sm2a.allocated) {
       rc = build_smoother(lv.a, lv.sm2a, lv.smd2);

@@ -126,21 +126,29 @@ constexpr int dia_rpt() { return K > 0 ? 2 : 1; }
 // slices; CSR with a power-of-two number of lanes per row (matched to the
 // mean row length) for small matrices.
 // ---------------------------------------------------------------------------
-struct FmtChoice { int fmt, lanes, w; };
+struct FmtChoice {
+  int fmt, lanes, w;
+  std::vector<int> offs;  // DIA: the diagonal offsets (ascending)
+};
 
 // Diagonal offsets of h when it has at most 8 distinct ones (ascending), else
 // empty.
+// Columns ascend within a row, so the offsets of a row ascend too: one merge
+// walk per row against the sorted set found so far.
 static std::vector<int> dia_offsets(const HCsr& h) {
   std::vector<int> offs;
-  for (int i = 0; i < h.nr; ++i)
+  for (int i = 0; i < h.nr; ++i) {
+    size_t p = 0;
     for (int64_t k = h.irp[i]; k < h.irp[i + 1]; ++k) {
       const int o = h.ja[k] - i;
-      if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
-        offs.push_back(o);
+      while (p < offs.size() && offs[p] < o) ++p;
+      if (p == offs.size() || offs[p] != o) {
+        offs.insert(offs.begin() + p, o);
         if (offs.size() > 8) return {};
       }
+      ++p;
     }
-  std::sort(offs.begin(), offs.end());
+  }
   return offs;
 }
 
@@ -160,8 +168,11 @@ static FmtChoice choose_format(const HCsr& h, int force) {
   c.lanes = lanes;
   if (force == 1 || maxrow == 0) return c;
   if (force == 0 || force == 6) {
-    const std::vector<int> offs = dia_offsets(h);
-    if (dia_eligible(h, offs)) { c.fmt = FMT_DIA; c.w = (int)offs.size(); return c; }
+    std::vector<int> offs = dia_offsets(h);
+    if (dia_eligible(h, offs)) {
+      c.fmt = FMT_DIA; c.w = (int)offs.size(); c.offs = std::move(offs);
+      return c;
+    }
   }
   if (force == 2 && maxrow <= 64) { c.fmt = FMT_ELL; return c; }
   if (force >= 3 && force <= 5) { c.fmt = FMT_SELL; return c; }
@@ -177,57 +188,73 @@ static FmtChoice choose_format(const HCsr& h, int force) {
 // force 4: sigma 256, force 5: no sorting.
 static int sell_sigma(int force) { return force == 4 ? 256 : (force == 5 ? 1 : 4096); }
 
+// An operator uploaded once as CSR (int32 row pointers, FP64 values); every
+// candidate layout is then built from it on the device.
+struct DevSrc {
+  int nr = 0, nc = 0;
+  int64_t nnz = 0;
+  int maxrow = 0;
+  int* irp = nullptr;
+  int* ja = nullptr;
+  double* val = nullptr;
+};
+
+static DevSrc make_src(const HCsr& h, DevArena& tmp, hipStream_t s) {
+  DevSrc d;
+  d.nr = h.nr; d.nc = h.nc; d.nnz = h.nnz();
+  if (d.nnz >= ((int64_t)1 << 31)) throw DevError("operator exceeds 2^31 entries");
+  for (int i = 0; i < h.nr; ++i) d.maxrow = std::max(d.maxrow, (int)(h.irp[i + 1] - h.irp[i]));
+  int64_t* irp64 = tmp.alloc<int64_t>((size_t)h.nr + 1);
+  if (!h.irp.empty()) HIPCHECK(hipMemcpyAsync(irp64, h.irp.data(), ((size_t)h.nr + 1) * sizeof(int64_t),
+                                              hipMemcpyHostToDevice, s));
+  else HIPCHECK(hipMemsetAsync(irp64, 0, ((size_t)h.nr + 1) * sizeof(int64_t), s));
+  d.irp = tmp.alloc<int>((size_t)h.nr + 1);
+  irp_to32_kernel<<<nblk(h.nr + 1), BS, 0, s>>>(h.nr + 1, irp64, d.irp);
+  d.ja = tmp.upload(h.ja.data(), h.ja.size(), s);
+  d.val = tmp.upload(h.val.data(), h.val.size(), s);
+  HIPCHECK(hipGetLastError());
+  return d;
+}
+
+// Layout c of the operator h (uploaded as src) in arena ar.
 template <typename V>
-static DMat<V> upload_mat(const HCsr& h, DevArena& ar, hipStream_t s, int force, int lanes_override = 0) {
+static DMat<V> build_mat(const HCsr& h, const DevSrc& src, DevArena& ar, hipStream_t s, FmtChoice c, int force) {
   DMat<V> m;
   m.nr = h.nr; m.nc = h.nc; m.nnz = h.nnz();
-  FmtChoice c = choose_format(h, force);
-  if (lanes_override > 0) { c.fmt = FMT_CSR; c.lanes = lanes_override; }
   m.fmt = c.fmt; m.lanes = c.lanes; m.w = c.w;
   {
     const int per_lane = (c.w + c.lanes - 1) / std::max(1, c.lanes);
     m.iters = per_lane <= 2 ? 2 : (per_lane <= 4 ? 4 : 8);
   }
   if (c.fmt == FMT_DIA) {
-    const std::vector<int> offs = dia_offsets(h);
+    const std::vector<int> offs = c.offs.empty() ? dia_offsets(h) : c.offs;
     m.w = (int)offs.size();
     for (int d = 0; d < 8; ++d) m.off.o[d] = d < m.w ? offs[d] : 0;
-    std::vector<V> val((size_t)h.nr * m.w, (V)0);
-    for (int i = 0; i < h.nr; ++i)
-      for (int64_t k = h.irp[i]; k < h.irp[i + 1]; ++k) {
-        const int d = (int)(std::lower_bound(offs.begin(), offs.end(), h.ja[k] - i) - offs.begin());
-        val[(size_t)d * h.nr + i] = (V)h.val[k];
-      }
-    m.val = ar.upload(val.data(), val.size(), s);
-    HIPCHECK(hipStreamSynchronize(s));
+    m.val = ar.alloc<V>((size_t)h.nr * m.w);
+    dia_fill_kernel<V><<<nblk(h.nr), BS, 0, s>>>(h.nr, m.w, m.off, src.irp, src.ja, src.val, m.val);
   } else if (c.fmt == FMT_ELL) {
     m.ld = h.nr;
-    std::vector<int> ja((size_t)m.ld * m.w);
-    std::vector<V> val((size_t)m.ld * m.w);
-    for (int i = 0; i < h.nr; ++i) {
-      int k = 0;
-      for (int64_t q = h.irp[i]; q < h.irp[i + 1]; ++q, ++k) {
-        ja[(size_t)k * m.ld + i] = h.ja[q];
-        val[(size_t)k * m.ld + i] = (V)h.val[q];
-      }
-      for (; k < m.w; ++k) {  // padding: zero value on a valid column
-        ja[(size_t)k * m.ld + i] = std::min(i, h.nc - 1);
-        val[(size_t)k * m.ld + i] = (V)0;
-      }
-    }
-    m.ja = ar.upload(ja.data(), ja.size(), s);
-    m.val = ar.upload(val.data(), val.size(), s);
-    HIPCHECK(hipStreamSynchronize(s));
+    m.ja = ar.alloc<int>((size_t)m.ld * m.w);
+    m.val = ar.alloc<V>((size_t)m.ld * m.w);
+    ell_fill_kernel<V><<<nblk(h.nr), BS, 0, s>>>(h.nr, h.nc, m.w, src.irp, src.ja, src.val, m.ja, m.val);
   } else if (c.fmt == FMT_SELL) {
     const int nr = h.nr;
     const int nsl = (nr + SELL_C - 1) / SELL_C;
     std::vector<int> perm(nr);
-    for (int i = 0; i < nr; ++i) perm[i] = i;
     auto len = [&](int i) { return (int)(h.irp[i + 1] - h.irp[i]); };
     const int sigma = sell_sigma(force);
-    for (int w0 = 0; w0 < nr && sigma > 1; w0 += sigma) {
-      const int w1 = std::min(nr, w0 + sigma);
-      std::stable_sort(perm.begin() + w0, perm.begin() + w1, [&](int a, int b) { return len(a) > len(b); });
+    if (sigma <= 1) {
+      for (int i = 0; i < nr; ++i) perm[i] = i;
+    } else {
+      // stable counting sort by decreasing length inside each window
+      std::vector<int> cnt((size_t)src.maxrow + 2);
+      for (int w0 = 0; w0 < nr; w0 += sigma) {
+        const int w1 = std::min(nr, w0 + sigma);
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int i = w0; i < w1; ++i) cnt[src.maxrow - len(i) + 1]++;
+        for (size_t b = 1; b < cnt.size(); ++b) cnt[b] += cnt[b - 1];
+        for (int i = w0; i < w1; ++i) perm[w0 + cnt[src.maxrow - len(i)]++] = i;
+      }
     }
     std::vector<int> sptr(nsl + 1, 0);
     for (int sl = 0; sl < nsl; ++sl) {
@@ -241,41 +268,36 @@ static DMat<V> upload_mat(const HCsr& h, DevArena& ar, hipStream_t s, int force,
       sptr[sl + 1] = (int)next;
     }
     const size_t tot = (size_t)sptr[nsl];
-    std::vector<int> ja(tot);
-    std::vector<V> val(tot);
-    for (int sl = 0; sl < nsl; ++sl) {
-      const int w = (sptr[sl + 1] - sptr[sl]) / SELL_C;
-      for (int l = 0; l < SELL_C; ++l) {
-        const int p = sl * SELL_C + l;
-        const int row = (p < nr) ? perm[p] : -1;
-        int k = 0;
-        if (row >= 0)
-          for (int64_t q = h.irp[row]; q < h.irp[row + 1]; ++q, ++k) {
-            ja[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = h.ja[q];
-            val[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = (V)h.val[q];
-          }
-        for (; k < w; ++k) {  // padding: zero value on a valid column
-          ja[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = (row >= 0) ? std::min(row, h.nc - 1) : 0;
-          val[(size_t)sptr[sl] + (size_t)k * SELL_C + l] = (V)0;
-        }
-      }
-    }
     m.sptr = ar.upload(sptr.data(), sptr.size(), s);
     m.perm = ar.upload(perm.data(), perm.size(), s);
-    m.ja = ar.upload(ja.data(), ja.size(), s);
-    m.val = ar.upload(val.data(), val.size(), s);
-    HIPCHECK(hipStreamSynchronize(s));
+    m.ja = ar.alloc<int>(tot);
+    m.val = ar.alloc<V>(tot);
+    sell_fill_kernel<V><<<nblk((int64_t)nsl * SELL_C), BS, 0, s>>>(nr, h.nc, nsl * SELL_C, m.perm, m.sptr, src.irp,
+                                                                  src.ja, src.val, m.ja, m.val);
   } else {
-    std::vector<int> irp(h.nr + 1);
-    for (int i = 0; i <= h.nr; ++i) irp[i] = (int)h.irp[i];
-    std::vector<V> val(h.val.size());
-    for (size_t q = 0; q < val.size(); ++q) val[q] = (V)h.val[q];
-    m.irp = ar.upload(irp.data(), irp.size(), s);
-    m.ja = ar.upload(h.ja.data(), h.ja.size(), s);
-    m.val = ar.upload(val.data(), val.size(), s);
-    HIPCHECK(hipStreamSynchronize(s));
+    m.irp = ar.alloc<int>((size_t)h.nr + 1);
+    m.ja = ar.alloc<int>(h.ja.size());
+    m.val = ar.alloc<V>(h.val.size());
+    HIPCHECK(hipMemcpyAsync(m.irp, src.irp, ((size_t)h.nr + 1) * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (!h.ja.empty())
+      HIPCHECK(hipMemcpyAsync(m.ja, src.ja, h.ja.size() * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (!h.val.empty())
+      cvt_val_kernel<V><<<(int)((h.val.size() + BS - 1) / BS), BS, 0, s>>>((int64_t)h.val.size(), src.val, m.val);
   }
+  HIPCHECK(hipGetLastError());
+  // the host's arrays (sptr, perm) must outlive the asynchronous copies
+  HIPCHECK(hipStreamSynchronize(s));
   return m;
+}
+
+template <typename V>
+static DMat<V> upload_mat(const HCsr& h, DevArena& ar, hipStream_t s, int force, int lanes_override = 0) {
+  FmtChoice c = choose_format(h, force);
+  if (lanes_override > 0) { c.fmt = FMT_CSR; c.lanes = lanes_override; }
+  DevArena tmp;
+  const DevSrc src = make_src(h, tmp, s);
+  DMat<V> m = build_mat<V>(h, src, ar, s, c, force);
+  return m;  // tmp is released after build_mat's synchronisation
 }
 
 static HCsr host_transpose(const HCsr& a) {
@@ -415,8 +437,12 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
   hipEvent_t e0, e1;
   HIPCHECK(hipEventCreate(&e0));
   HIPCHECK(hipEventCreate(&e1));
+  DevArena tmp;
+  const DevSrc src = make_src(h, tmp, s);
   int bforce = 6, blanes = 0;
-  if (!dia_eligible(h, dia_offsets(h))) {  // DIA reads strictly fewer bytes than any other layout
+  std::vector<int> offs = dia_offsets(h);
+  const bool dia = dia_eligible(h, offs);
+  if (!dia) {  // DIA reads strictly fewer bytes than any other layout
     int maxrow = 0;
     for (int i = 0; i < h.nr; ++i) maxrow = std::max(maxrow, (int)(h.irp[i + 1] - h.irp[i]));
     const double avg = (double)h.nnz() / std::max(1, h.nr);
@@ -435,15 +461,19 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
     double best = 1e300;
     int bi = 0;
     for (size_t c = 0; c < cands.size() && cands.size() > 1; ++c) {
-      DevArena tmp;
-      DMat<V> m = upload_mat<V>(h, tmp, s, cands[c].force, cands[c].lanes);
+      DevArena ctmp;
+      FmtChoice fc = choose_format(h, cands[c].force);
+      if (cands[c].lanes > 0) { fc.fmt = FMT_CSR; fc.lanes = cands[c].lanes; }
+      DMat<V> m = build_mat<V>(h, src, ctmp, s, fc, cands[c].force);
       const double t = time_spmv(m, x, y, s, e0, e1) * (cands[c].ordered ? 1.0 : 1.03);
       if (t < best) { best = t; bi = (int)c; }
     }
     bforce = cands[bi].force;
     blanes = cands[bi].lanes;
   }
-  DMat<V> m = upload_mat<V>(h, ar, s, bforce, blanes);
+  FmtChoice fb = dia ? FmtChoice{FMT_DIA, 1, (int)offs.size(), offs} : choose_format(h, bforce);
+  if (blanes > 0) { fb.fmt = FMT_CSR; fb.lanes = blanes; }
+  DMat<V> m = build_mat<V>(h, src, ar, s, fb, bforce);
   // row-block order of the chosen layout: 8192-row chunks per XCD (default),
   // identity, or one contiguous range per XCD; the default wins ties within 2 %
   // (MLD_XMAP=0/1/2 fixes it instead)

@@ -40,11 +40,16 @@ class DevPrecT : public DevPrec {
       const Level& H = p.precv[l];
       DLevel<V>& D = L_[l];
       D.n = H.a.nr;
+      double t0 = wall_seconds();
       D.a = upload_tuned<V>(H.a, ar_, s_, p.dev_format);
+      phase_log("upload A", l + 1, wall_seconds() - t0);
       if (l > 0) {
+        t0 = wall_seconds();
         D.p = upload_tuned<V>(H.p, ar_, s_, p.dev_format);
         D.r = upload_tuned<V>(H.r, ar_, s_, p.dev_format);
+        phase_log("upload P, R", l + 1, wall_seconds() - t0);
       }
+      t0 = wall_seconds();
       D.sweeps_pre = H.parms.sweeps_pre;
       D.sweeps_post = H.parms.sweeps_post;
       D.ml_cycle = H.parms.ml_cycle;
@@ -56,6 +61,7 @@ class DevPrecT : public DevPrec {
       D.tx = ar_.alloc<V>(n); D.ty = ar_.alloc<V>(n);
       D.t1 = ar_.alloc<V>(n); D.t2 = ar_.alloc<V>(n);
       if (H.smd.solver == DENSE_LU || (D.has_sm2 && H.smd2.solver == DENSE_LU)) D.wlu = ar_.alloc<double>(n);
+      phase_log("upload smoothers", l + 1, wall_seconds() - t0);
     }
     // K-cycle: the inner iteration runs on levels 1 .. nlev-2 (0-based)
     for (int l = 1; l + 1 < nlev; ++l) {

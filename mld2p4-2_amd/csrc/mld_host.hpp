@@ -119,6 +119,11 @@ struct Prec {
   std::string last_error;
 };
 
+// Phase timing on standard error when MLD_VERBOSE is set (the reference
+// prints its build phases at debug_level >= psb_debug_outer_).
+double wall_seconds();
+void phase_log(const char* what, int level, double seconds);
+
 // configuration (mld_dprecinit / mld_dprecset / mld_d_base_onelev_*set*)
 int prec_init(Prec& p, const std::string& ptype);
 int prec_set_int(Prec& p, const std::string& what, int val, int ilev, int ilmax, const std::string& pos);

@@ -672,6 +672,79 @@ __global__ __launch_bounds__(1024) void dense_lu_small_kernel(int n, const doubl
   }
 }
 
+// ------------------------------------------------- layout construction
+// Device-side conversion of an uploaded CSR operator (int32 row pointers,
+// int32 columns, FP64 values) into the layouts of the products; each writes
+// exactly the arrays the sequential host conversion would.
+__global__ __launch_bounds__(BS) void irp_to32_kernel(int n1, const int64_t* __restrict__ a, int* __restrict__ b) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i < n1) b[i] = (int)a[i];
+}
+template <typename V>
+__global__ __launch_bounds__(BS) void cvt_val_kernel(int64_t n, const double* __restrict__ a, V* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n) b[i] = (V)a[i];
+}
+// ELL, column-major with leading dimension ld = nr: padding is a zero value
+// on column min(i, nc-1)
+template <typename V>
+__global__ __launch_bounds__(BS) void ell_fill_kernel(int nr, int nc, int w, const int* __restrict__ irp,
+                                                      const int* __restrict__ ja, const double* __restrict__ val,
+                                                      int* __restrict__ oja, V* __restrict__ oval) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= nr) return;
+  const int k0 = irp[i], len = irp[i + 1] - k0;
+  for (int k = 0; k < w; ++k) {
+    const int64_t q = (int64_t)k * nr + i;
+    if (k < len) {
+      oja[q] = ja[k0 + k];
+      oval[q] = (V)val[k0 + k];
+    } else {
+      oja[q] = min(i, nc - 1);
+      oval[q] = (V)0;
+    }
+  }
+}
+// SELL-64: position p of slice p/64 holds row perm[p] (or padding past nr)
+template <typename V>
+__global__ __launch_bounds__(BS) void sell_fill_kernel(int nr, int nc, int npos, const int* __restrict__ perm,
+                                                       const int* __restrict__ sptr, const int* __restrict__ irp,
+                                                       const int* __restrict__ ja, const double* __restrict__ val,
+                                                       int* __restrict__ oja, V* __restrict__ oval) {
+  const int p = blockIdx.x * BS + threadIdx.x;
+  if (p >= npos) return;
+  const int sl = p / 64, l = p % 64;
+  const int row = (p < nr) ? perm[p] : -1;
+  const int base = sptr[sl], w = (sptr[sl + 1] - base) / 64;
+  const int k0 = (row >= 0) ? irp[row] : 0;
+  const int len = (row >= 0) ? irp[row + 1] - k0 : 0;
+  for (int k = 0; k < w; ++k) {
+    const int64_t q = (int64_t)base + (int64_t)k * 64 + l;
+    if (k < len) {
+      oja[q] = ja[k0 + k];
+      oval[q] = (V)val[k0 + k];
+    } else {
+      oja[q] = (row >= 0) ? min(row, nc - 1) : 0;
+      oval[q] = (V)0;
+    }
+  }
+}
+// DIA: val[d * nr + i] = a(i, i + off[d]), zero where row i has no such entry
+template <typename V>
+__global__ __launch_bounds__(BS) void dia_fill_kernel(int nr, int w, DiaOff off, const int* __restrict__ irp,
+                                                      const int* __restrict__ ja, const double* __restrict__ val,
+                                                      V* __restrict__ oval) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= nr) return;
+  for (int d = 0; d < w; ++d) oval[(int64_t)d * nr + i] = (V)0;
+  for (int k = irp[i]; k < irp[i + 1]; ++k) {
+    const int o = ja[k] - i;
+    int d = 0;
+    while (d < w - 1 && off.o[d] != o) ++d;
+    oval[(int64_t)d * nr + i] = (V)val[k];
+  }
+}
+
 }  // namespace
 }  // namespace k
 }  // namespace mld
