@@ -77,8 +77,11 @@ int dec_map_bld(int iorder, double theta, const HCsr& a, std::vector<int>& ilagg
       if (strong[k]) agg[a.ja[k]] = cnt;
     agg[i] = cnt;
   }
-  // phase 2 (decisions on the phase-1 snapshot)
+  // phase 2 (decisions on the phase-1 snapshot; a node joins the aggregate
+  // of a node aggregated in phase 1, whose id phase 2 never changes, so the
+  // rows are independent)
   std::vector<int> snap(agg);
+#pragma omp parallel for schedule(static)
   for (int t = 0; t < nr; ++t) {
     int i = order[t];
     if (agg[i] != FREE) continue;
@@ -101,21 +104,34 @@ int dec_map_bld(int iorder, double theta, const HCsr& a, std::vector<int>& ilagg
     agg[i] = cnt;
   }
   ilaggr.resize(nr);
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
   for (int i = 0; i < nr; ++i) {
-    if (agg[i] <= 0) return E_INTERNAL;
+    if (agg[i] <= 0) bad = 1;
     ilaggr[i] = agg[i] - 1;
   }
+  if (bad) return E_INTERNAL;
   naggr = cnt;
   return 0;
 }
 
+// Read-only view of a CSR matrix (lets S share A's row pointers and columns).
+struct CsrView {
+  int nr, nc;
+  const int64_t* irp;
+  const int* ja;
+  const double* val;
+};
+static CsrView view(const HCsr& a) { return CsrView{a.nr, a.nc, a.irp.data(), a.ja.data(), a.val.data()}; }
+
 // Row-parallel Gustavson product C = A*B; every C_ij is the sum over A's
-// stored row i, in that order, of a_ik*b_kj.  Columns of C ascending.
-static void spgemm(const HCsr& A, const HCsr& B, HCsr& C) {
+// stored row i, in that order, of a_ik*b_kj, accumulated from 0.0 (each sum
+// is the sequential loop's, whatever the thread count).  Columns of C
+// ascending.  A counting pass sizes C, the second pass fills it in place.
+static void spgemm(const CsrView& A, const CsrView& B, HCsr& C) {
   const int nr = A.nr, nc = B.nc;
   C.nr = nr; C.nc = nc;
   C.irp.assign((size_t)nr + 1, 0);
-  std::vector<int64_t> cnt(nr);
 #pragma omp parallel
   {
     std::vector<int> mark(nc, -1);
@@ -123,16 +139,16 @@ static void spgemm(const HCsr& A, const HCsr& B, HCsr& C) {
     for (int i = 0; i < nr; ++i) {
       int64_t c = 0;
       for (int64_t ka = A.irp[i]; ka < A.irp[i + 1]; ++ka) {
-        int k = A.ja[ka];
+        const int k = A.ja[ka];
         for (int64_t kb = B.irp[k]; kb < B.irp[k + 1]; ++kb) {
-          int j = B.ja[kb];
+          const int j = B.ja[kb];
           if (mark[j] != i) { mark[j] = i; ++c; }
         }
       }
-      cnt[i] = c;
+      C.irp[i + 1] = c;
     }
   }
-  for (int i = 0; i < nr; ++i) C.irp[i + 1] = C.irp[i] + cnt[i];
+  for (int i = 0; i < nr; ++i) C.irp[i + 1] += C.irp[i];
   C.ja.resize(C.irp[nr]);
   C.val.resize(C.irp[nr]);
 #pragma omp parallel
@@ -144,34 +160,63 @@ static void spgemm(const HCsr& A, const HCsr& B, HCsr& C) {
     for (int i = 0; i < nr; ++i) {
       cols.clear();
       for (int64_t ka = A.irp[i]; ka < A.irp[i + 1]; ++ka) {
-        int k = A.ja[ka];
-        double ak = A.val[ka];
+        const int k = A.ja[ka];
+        const double ak = A.val[ka];
         for (int64_t kb = B.irp[k]; kb < B.irp[k + 1]; ++kb) {
-          int j = B.ja[kb];
+          const int j = B.ja[kb];
           if (mark[j] != i) { mark[j] = i; acc[j] = 0.0; cols.push_back(j); }
           acc[j] = acc[j] + ak * B.val[kb];
         }
       }
       std::sort(cols.begin(), cols.end());
-      int64_t p = C.irp[i];
-      for (int j : cols) { C.ja[p] = j; C.val[p] = acc[j]; ++p; }
+      int64_t q = C.irp[i];
+      for (int j : cols) { C.ja[q] = j; C.val[q] = acc[j]; ++q; }
     }
   }
 }
 
-// Transpose; each output row lists its columns in ascending order.
+// Transpose; each output row lists its columns in ascending order.  Thread t
+// takes a contiguous block of rows and counts its entries per column; the
+// entries of block t then go after those of blocks 0..t-1 in every output
+// row, which is the sequential placement's order.
 static void transpose(const HCsr& A, HCsr& T) {
   T.nr = A.nc; T.nc = A.nr;
+  const int64_t nz = A.nnz();
+  const int nc = A.nc;
+  const int nt = std::max(1, std::min(omp_get_max_threads(), (int)(nz / 65536) + 1));
+  std::vector<int> r0(nt + 1);
+  for (int t = 0; t <= nt; ++t) r0[t] = (int)((int64_t)A.nr * t / nt);
+  std::vector<int64_t> cnt((size_t)nt * nc, 0);  // cnt[t * nc + j]
+#pragma omp parallel num_threads(nt)
+  {
+    const int t = omp_get_thread_num();
+    int64_t* c = cnt.data() + (size_t)t * nc;
+    for (int i = r0[t]; i < r0[t + 1]; ++i)
+      for (int64_t k = A.irp[i]; k < A.irp[i + 1]; ++k) c[A.ja[k]]++;
+  }
+  // start of block t in output row j: sum over rows < j, then blocks < t
   T.irp.assign((size_t)T.nr + 1, 0);
-  for (int64_t k = 0; k < A.nnz(); ++k) T.irp[A.ja[k] + 1]++;
-  for (int i = 0; i < T.nr; ++i) T.irp[i + 1] += T.irp[i];
-  T.ja.resize(A.nnz()); T.val.resize(A.nnz());
-  std::vector<int64_t> pos(T.irp.begin(), T.irp.end() - 1);
-  for (int i = 0; i < A.nr; ++i)
-    for (int64_t k = A.irp[i]; k < A.irp[i + 1]; ++k) {
-      int64_t q = pos[A.ja[k]]++;
-      T.ja[q] = i; T.val[q] = A.val[k];
+  int64_t run = 0;
+  for (int j = 0; j < nc; ++j) {
+    T.irp[j] = run;
+    for (int t = 0; t < nt; ++t) {
+      const int64_t v = cnt[(size_t)t * nc + j];
+      cnt[(size_t)t * nc + j] = run;
+      run += v;
     }
+  }
+  T.irp[nc] = run;
+  T.ja.resize(nz); T.val.resize(nz);
+#pragma omp parallel num_threads(nt)
+  {
+    const int t = omp_get_thread_num();
+    int64_t* pos = cnt.data() + (size_t)t * nc;
+    for (int i = r0[t]; i < r0[t + 1]; ++i)
+      for (int64_t k = A.irp[i]; k < A.irp[i + 1]; ++k) {
+        const int64_t q = pos[A.ja[k]]++;
+        T.ja[q] = i; T.val[q] = A.val[k];
+      }
+  }
 }
 
 int aggrmat_smth_asb(const HCsr& a, const std::vector<int>& ilaggr, int naggr, LevelParms& parms, HCsr& ac,
@@ -210,16 +255,16 @@ int aggrmat_smth_asb(const HCsr& a, const std::vector<int>& ilaggr, int naggr, L
     }
   }
   for (int i = 0; i < n; ++i) dinv[i] = (dinv[i] != 0.0) ? 1.0 / dinv[i] : 1.0;
-  // D^-1 A row scaling and the max-norm estimate ||D^-1 A||_inf
-  HCsr da;
-  da.nr = a.nr; da.nc = a.nc; da.irp = a.irp; da.ja = a.ja; da.val.resize(a.nnz());
+  // D^-1 A row scaling (values only: the structure stays A's) and the
+  // max-norm estimate ||D^-1 A||_inf
+  std::vector<double> daval(a.nnz());
   double anorm = 0.0;
 #pragma omp parallel for schedule(static) reduction(max : anorm)
   for (int i = 0; i < n; ++i) {
     double rs = 0.0;
     for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
-      da.val[k] = a.val[k] * dinv[i];
-      rs = rs + std::fabs(da.val[k]);
+      daval[k] = a.val[k] * dinv[i];
+      rs = rs + std::fabs(daval[k]);
     }
     anorm = std::max(anorm, rs);
   }
@@ -233,16 +278,19 @@ int aggrmat_smth_asb(const HCsr& a, const std::vector<int>& ilaggr, int naggr, L
   } else {
     return E_BAD_ARG;
   }
+  CsrView sv;
   if (filter) {
     for (int i = 0; i < n; ++i)
       for (int64_t k = s.irp[i]; k < s.irp[i + 1]; ++k) s.val[k] = s.val[k] * dinv[i];
+    sv = view(s);
   } else {
-    s = std::move(da);
+    sv = CsrView{a.nr, a.nc, a.irp.data(), a.ja.data(), daval.data()};
   }
+  double* svv = const_cast<double*>(sv.val);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i)
-    for (int64_t k = s.irp[i]; k < s.irp[i + 1]; ++k)
-      s.val[k] = (s.ja[k] == i) ? 1.0 - omega * s.val[k] : -(omega * s.val[k]);
+    for (int64_t k = sv.irp[i]; k < sv.irp[i + 1]; ++k)
+      svv[k] = (sv.ja[k] == i) ? 1.0 - omega * svv[k] : -(omega * svv[k]);
   // tentative prolongator: row i has a single 1.0 in column ilaggr[i]
   HCsr pt;
   pt.nr = n; pt.nc = naggr;
@@ -251,17 +299,17 @@ int aggrmat_smth_asb(const HCsr& a, const std::vector<int>& ilaggr, int naggr, L
   pt.ja.assign(ilaggr.begin(), ilaggr.end());
   pt.val.assign(n, 1.0);
   double t0 = wall_seconds();
-  spgemm(s, pt, p);        // P  = (I - omega D^-1 A) Ptilde
+  spgemm(sv, view(pt), p);  // P  = (I - omega D^-1 A) Ptilde
   double t1 = wall_seconds();
   phase_log("P = S Ptilde", -1, t1 - t0);
   HCsr ap;
-  spgemm(a, p, ap);        // AP = A P
+  spgemm(view(a), view(p), ap);  // AP = A P
   double t2 = wall_seconds();
   phase_log("A P", -1, t2 - t1);
-  transpose(p, r);         // R  = P^T
+  transpose(p, r);          // R  = P^T
   double t3 = wall_seconds();
   phase_log("R = P^T", -1, t3 - t2);
-  spgemm(r, ap, ac);       // Ac = R (A P)
+  spgemm(view(r), view(ap), ac);  // Ac = R (A P)
   phase_log("Ac = R (A P)", -1, wall_seconds() - t3);
   return 0;
 }
@@ -478,15 +526,27 @@ int hierarchy_build(Prec& p, int n, const int64_t* irp, const int* ja, const dou
   p.hier_built = p.smoothers_built = false;
   std::vector<Level>& v = p.precv;
   Level& l0 = v[0];
+  const double tc = wall_seconds();
   l0.a.clear();
   l0.a.nr = l0.a.nc = n;
-  l0.a.irp.assign(irp, irp + n + 1);
-  int64_t nz = l0.a.irp[n];
-  l0.a.ja.assign(ja, ja + nz);
-  l0.a.val.assign(val, val + nz);
-  for (int i = 0; i < n; ++i)
-    for (int64_t k = irp[i] + 1; k < irp[i + 1]; ++k)
-      if (ja[k] <= ja[k - 1]) { p.last_error = "CSR rows must have strictly ascending columns"; return E_BAD_ARG; }
+  const int64_t nz = irp[n];
+  l0.a.irp.resize((size_t)n + 1);
+  l0.a.ja.resize(nz);
+  l0.a.val.resize(nz);
+  int bad = 0;
+  // copy (the caller keeps its arrays) and check the rows, in parallel
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int i = 0; i < n; ++i) {
+    l0.a.irp[i + 1] = irp[i + 1];
+    for (int64_t k = irp[i]; k < irp[i + 1]; ++k) {
+      l0.a.ja[k] = ja[k];
+      l0.a.val[k] = val[k];
+      if (k > irp[i] && ja[k] <= ja[k - 1]) bad = 1;
+    }
+  }
+  l0.a.irp[0] = irp[0];
+  phase_log("copy of A", 1, wall_seconds() - tc);
+  if (bad) { p.last_error = "CSR rows must have strictly ascending columns"; return E_BAD_ARG; }
   if ((int)v.size() == 1) { p.hier_built = true; return 0; }
   // default coarse size: 40 * int(N^(1/3)) (mld_d_hierarchy_bld.f90:187-197)
   int casize = p.min_coarse_size;

@@ -5,6 +5,8 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <memory>
+#include <utility>
 #include <vector>
 
 namespace mld {
@@ -49,11 +51,30 @@ enum : int {
   E_INTERNAL = -40,
 };
 
+// Allocator whose resize() leaves new elements uninitialised: the large CSR
+// arrays are then first written by the threads that fill them (no
+// single-threaded zero fill of gigabytes).
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  using value_type = T;
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  template <class U>
+  struct rebind { using other = NoInitAlloc<U>; };
+  template <class U>
+  void construct(U* p) noexcept { ::new ((void*)p) U; }
+  template <class U, class... Args>
+  void construct(U* p, Args&&... args) { ::new ((void*)p) U(std::forward<Args>(args)...); }
+};
+template <class T>
+using bigvec = std::vector<T, NoInitAlloc<T>>;
+
 struct HCsr {
   int nr = 0, nc = 0;
-  std::vector<int64_t> irp;  // nr+1, 0-based
-  std::vector<int> ja;
-  std::vector<double> val;
+  bigvec<int64_t> irp;  // nr+1, 0-based
+  bigvec<int> ja;
+  bigvec<double> val;
   int64_t nnz() const { return irp.empty() ? 0 : irp[nr]; }
   void clear() { nr = nc = 0; irp.clear(); ja.clear(); val.clear(); }
 };
