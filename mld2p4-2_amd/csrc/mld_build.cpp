@@ -13,6 +13,8 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <functional>
+#include <queue>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -359,6 +361,79 @@ int ilu0_factor(const HCsr& a, HCsr& l, HCsr& u, std::vector<double>& d) {
   return 0;
 }
 
+// ILU(k) and MILU(k) by rows with levels of fill (mld_diluk_fact.f90:268-960):
+// row i is copied into a dense work row with level 0 on A's pattern
+// (iluk_copyin, :485-566); its columns are visited in ascending order (a
+// min-heap, new fill columns joining it), and every visited column k < i with
+// a nonzero value and level <= fill eliminates: l_ik = row_k * d_k, row_j -=
+// row_k * u_kj (U rows stored scaled by their inverse pivot), lev_j =
+// min(lev_j, lev_k + lev(u_kj) + 1) (iluk_fact, :637-722).  The visited columns
+// are then split (iluk_copyout, :812-960): level <= fill kept in L or U,
+// others dropped (MILU: added to the pivot, in the visiting order), d_i =
+// 1/pivot, row i of U scaled by d_i.  Same storage as ilu0_factor.
+int iluk_factor(const HCsr& a, int fill, bool milu, HCsr& l, HCsr& u, std::vector<double>& d) {
+  const int m = a.nr;
+  l.clear(); u.clear();
+  l.nr = l.nc = u.nr = u.nc = m;
+  l.irp.assign((size_t)m + 1, 0); u.irp.assign((size_t)m + 1, 0);
+  d.assign(m, 0.0);
+  std::vector<int> ulev;                          // level of every stored U entry
+  std::vector<double> row(m, 0.0);
+  std::vector<int> lev(m, -(m + 1));              // < 0: column not in the row
+  std::priority_queue<int, std::vector<int>, std::greater<int>> heap;
+  std::vector<int> idx;
+  for (int i = 0; i < m; ++i) {
+    for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
+      const int j = a.ja[k];
+      if (j < 0 || j >= m) continue;
+      row[j] = a.val[k];
+      lev[j] = 0;
+      heap.push(j);
+    }
+    idx.clear();
+    int last = -1;
+    while (!heap.empty()) {
+      const int k = heap.top();
+      heap.pop();
+      if (k == last) continue;
+      last = k;
+      idx.push_back(k);
+      if (row[k] != 0.0 && lev[k] <= fill && k < i) {
+        const double rk = row[k];
+        row[k] = row[k] * d[k];
+        const int lk = lev[k];
+        for (int64_t jj = u.irp[k]; jj < u.irp[k + 1]; ++jj) {
+          const int j = u.ja[jj];
+          if (lev[j] < 0) { heap.push(j); lev[j] = -lev[j]; }
+          row[j] = row[j] - rk * u.val[jj];
+          lev[j] = std::min(lev[j], lk + ulev[jj] + 1);
+        }
+      }
+    }
+    double piv = 0.0;
+    for (int j : idx) {
+      if (j < i) {
+        if (lev[j] <= fill) { l.ja.push_back(j); l.val.push_back(row[j]); }
+        else if (milu) piv = piv + row[j];
+      } else if (j == i) {
+        piv = piv + row[j];
+      } else {
+        if (lev[j] <= fill) { u.ja.push_back(j); u.val.push_back(row[j]); ulev.push_back(lev[j]); }
+        else if (milu) piv = piv + row[j];
+      }
+      row[j] = 0.0;
+      lev[j] = -(m + 1);
+    }
+    l.irp[i + 1] = (int64_t)l.ja.size();
+    u.irp[i + 1] = (int64_t)u.ja.size();
+    if (std::fabs(piv) < 2.220446049250313e-16) return E_PIVOT;
+    piv = 1.0 / piv;
+    d[i] = piv;
+    for (int64_t kk = u.irp[i]; kk < u.irp[i + 1]; ++kk) u.val[kk] = piv * u.val[kk];
+  }
+  return 0;
+}
+
 // Level sets of a triangular solve: rows in the same level depend only on
 // rows of earlier levels.  Rows within a level are kept in ascending order.
 void level_schedule(const HCsr& t, bool lower, std::vector<int>& ptr, std::vector<int>& rows) {
@@ -420,8 +495,16 @@ static int build_smoother(const HCsr& a, const SmootherSpec& spec, SmootherData&
   sd.solver = spec.solver;
   const int n = a.nr;
   if (spec.smoother == NOPREC) return 0;
-  if (spec.smoother != JAC) return E_UNSUPPORTED;  // AS: not provided
+  // Additive Schwarz on one process: no overlap rows (mld_d_as_smoother_bld.f90:78,
+  // extension only when novr > 0 and np > 1), restriction and prolongation are
+  // identities and ND holds the columns beyond the local rows, so it is empty
+  // (:147-151); every sweep then applies the solver to x itself
+  // (mld_d_as_smoother_apply_vect.f90:108-200): the data of the block-Jacobi
+  // smoother with an empty ND, whatever the solver.
+  const bool as = (spec.smoother == AS);
+  if (spec.smoother != JAC && !as) return E_UNSUPPORTED;
   if (spec.solver < 0) return E_NO_SOLVER;
+  sd.smoother = JAC;
   sd.nd.nr = sd.nd.nc = n;
   sd.nd.irp.assign((size_t)n + 1, 0);
   switch (spec.solver) {
@@ -431,6 +514,22 @@ static int build_smoother(const HCsr& a, const SmootherSpec& spec, SmootherData&
     case L1_DIAG: {
       const bool l1 = (spec.solver == L1_DIAG);
       std::vector<double> d = diag_of(a);
+      if (as) {  // ND empty: only the (l1-)diagonal
+        sd.dv.resize(n);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+          double dd = d[i];
+          if (l1) {
+            double s = 0.0;
+            for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k)
+              if (a.ja[k] != i) s = s + std::fabs(a.val[k]);
+            dd = dd + s;
+          }
+          sd.dv[i] = (dd == 0.0) ? 1.0 : 1.0 / dd;
+        }
+        sd.nnz_nd = 0;
+        return 0;
+      }
       // ND: A without its diagonal (clip_diag); for l1-Jacobi A - D_l1,
       // i.e. the diagonal entry replaced by -sum_{j!=i}|a_ij|.
       std::vector<int64_t> cnt(n);
@@ -472,10 +571,16 @@ static int build_smoother(const HCsr& a, const SmootherSpec& spec, SmootherData&
       sd.nnz_nd = sd.nd.nnz();
       return 0;
     }
-    case ILU_N: {
-      if (spec.fillin != 0) return E_UNSUPPORTED;  // ILU(k>0) not provided
+    case ILU_N:
+    case MILU_N: {
+      // mld_d_ilu_solver_bld.f90:126-160: ILU(0) by its own routine, ILU(k>0)
+      // and MILU(k) (k = 0 included) by the level-of-fill routine
+      if (spec.fillin < 0) return E_BAD_ARG;
       sd.nnz_nd = 0;                               // off-block part: empty on one device
-      int rc = ilu0_factor(a, sd.l, sd.u, sd.dv);
+      sd.solver = ILU_N;                           // the same factors' apply
+      int rc = (spec.solver == ILU_N && spec.fillin == 0)
+                   ? ilu0_factor(a, sd.l, sd.u, sd.dv)
+                   : iluk_factor(a, spec.fillin, spec.solver == MILU_N, sd.l, sd.u, sd.dv);
       if (rc) return rc;
       level_schedule(sd.l, true, sd.lsched_ptr, sd.lsched_rows);
       level_schedule(sd.u, false, sd.usched_ptr, sd.usched_rows);

@@ -162,6 +162,7 @@ int dec_map_bld(int iorder, double theta, const HCsr& a, std::vector<int>& ilagg
 int aggrmat_smth_asb(const HCsr& a, const std::vector<int>& ilaggr, int naggr, LevelParms& parms, HCsr& ac,
                      HCsr& p, HCsr& r);
 int ilu0_factor(const HCsr& a, HCsr& l, HCsr& u, std::vector<double>& d);
+int iluk_factor(const HCsr& a, int fill, bool milu, HCsr& l, HCsr& u, std::vector<double>& d);
 void level_schedule(const HCsr& t, bool lower, std::vector<int>& ptr, std::vector<int>& rows);
 double operator_complexity(const Prec& p);
 // prec%dump: flags 1 = ac, 2 = rp, 4 = smoother, 8 = solver; istart/iend <= 0: defaults

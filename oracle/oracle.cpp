@@ -38,6 +38,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <set>
 #include <vector>
 
 namespace {
@@ -60,6 +61,9 @@ enum {
   SM_NOPREC = 0,   // mld_noprec_
   SM_JAC = 1,      // mld_jac_   (jacobi smoother + diag solver)
   SM_BJAC = 2,     // mld_bjac_  (jacobi smoother + ILU(0) solver)
+  SM_AS = 3,       // mld_as_    (additive Schwarz + ILU(0) solver; on one process the
+                   //             block-Jacobi smoother: mld_d_as_smoother_bld.f90:78,147-151,
+                   //             mld_d_as_smoother_apply_vect.f90:108-200)
   SM_FBGS = 4,     // mld_fbgs_  (pre: jacobi smoother + GS solver, post: + BWGS solver)
   SM_L1JAC = 100,  // l1-Jacobi (Baker, Falgout, Kolev, Yang, SISC 33(5) 2011)
   SM_DENSELU = 101, // dense LU of the coarsest matrix (coarse solver only)
@@ -89,6 +93,10 @@ struct orc_params {
   int sweeps_post;
   int c_smoother;        // smoother at the coarsest level
   int c_sweeps;          // COARSE_SWEEPS
+  int sub_fill;          // SUB_FILLIN (ILU levels)
+  int sub_milu;          // SUB_SOLVE MILU
+  int c_fill;            // COARSE_FILLIN
+  int c_milu;            // COARSE_SUBSOLVE MILU
 };
 }
 
@@ -454,6 +462,69 @@ int ilu0_fact(const Csr& a, Csr& l, Csr& u, std::vector<double>& d, bool milu = 
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// ILU(k) and MILU(k): mlprec/impl/mld_diluk_fact.f90 (mld_diluk_factint
+// :268-419).  Row i in a dense work row, A's entries at level 0 (iluk_copyin
+// :485-566); columns visited once each in ascending order, fill columns
+// joining the set as they appear (the heap of iluk_fact :637-722): a visited
+// column k < i with a nonzero value and level <= fill gives l_ik = w_k d_k
+// and w_j -= w_k u_kj, lev_j = min(lev_j, lev_k + lev(u_kj) + 1) over row k of
+// U.  iluk_copyout (:812-960): level <= fill kept, else dropped (MILU: added
+// to the pivot, in visiting order); d_i = 1/pivot; row i of U times d_i.
+// ---------------------------------------------------------------------------
+int iluk_fact(const Csr& a, int fill, bool milu, Csr& l, Csr& u, std::vector<double>& d) {
+  ix m = a.nr;
+  l.nr = l.nc = u.nr = u.nc = m;
+  l.irp.assign(m + 1, 0); u.irp.assign(m + 1, 0);
+  l.ja.clear(); l.val.clear(); u.ja.clear(); u.val.clear();
+  d.assign(m, 0.0);
+  std::vector<long> ulev;
+  std::vector<double> w(m, 0.0);
+  std::vector<long> lev(m, 0);
+  std::vector<char> in(m, 0);
+  for (ix i = 0; i < m; ++i) {
+    std::set<ix> cols;
+    for (ix q = a.irp[i]; q < a.irp[i + 1]; ++q) {
+      ix k = a.ja[q];
+      if (k < 0 || k >= m) continue;
+      w[k] = a.val[q]; lev[k] = 0; in[k] = 1;
+      cols.insert(k);
+    }
+    for (auto it = cols.begin(); it != cols.end(); ++it) {
+      ix k = *it;
+      if (!(w[k] != 0.0 && lev[k] <= fill && k < i)) continue;
+      double rk = w[k];
+      w[k] = w[k] * d[k];
+      for (ix jj = u.irp[k]; jj < u.irp[k + 1]; ++jj) {
+        ix j = u.ja[jj];
+        if (!in[j]) { in[j] = 1; lev[j] = m + 1; cols.insert(j); }
+        w[j] = w[j] - rk * u.val[jj];
+        lev[j] = std::min(lev[j], lev[k] + ulev[jj] + 1);
+      }
+    }
+    double dia = 0.0;
+    for (ix j : cols) {
+      if (j < i) {
+        if (lev[j] <= fill) { l.ja.push_back((int)j); l.val.push_back(w[j]); }
+        else if (milu) dia = dia + w[j];
+      } else if (j == i) {
+        dia = dia + w[j];
+      } else {
+        if (lev[j] <= fill) { u.ja.push_back((int)j); u.val.push_back(w[j]); ulev.push_back(lev[j]); }
+        else if (milu) dia = dia + w[j];
+      }
+      w[j] = 0.0; in[j] = 0;
+    }
+    l.irp[i + 1] = (ix)l.ja.size();
+    u.irp[i + 1] = (ix)u.ja.size();
+    if (std::fabs(dia) < 2.220446049250313e-16) return (int)(i + 1);
+    dia = 1.0 / dia;
+    d[i] = dia;
+    for (ix q = u.irp[i]; q < u.irp[i + 1]; ++q) u.val[q] = dia * u.val[q];
+  }
+  return 0;
+}
+
 // psb_spsm on a unit triangular CSR factor, PSBLAS order assumed:
 //   lower : t_i = x_i - sum_{stored order} l_ij t_j,  i ascending
 //   upper : t_i = x_i - sum_{stored order} u_ij t_j,  i descending
@@ -537,6 +608,8 @@ struct DenseLU {
 // ---------------------------------------------------------------------------
 struct Smoother {
   int kind = SM_JAC;
+  int fill = 0;            // ILU: fill level (SUB_FILLIN / COARSE_FILLIN)
+  bool milu = false;       // ILU: MILU
   Csr nd;                  // ND (mld_d_jac_smoother_bld.f90:73-79)
   ix nnz_nd = 0;
   std::vector<double> dv;  // diag solver: 1/diag (0 -> 1); ILU: 1/pivot
@@ -587,9 +660,11 @@ struct Smoother {
       return 0;
     }
     if (kind == SM_BJAC) {
-      // csclip jmin = nrow+1: empty on one process
+      // csclip jmin = nrow+1: empty on one process; ILU(0) by its own
+      // routine, ILU(k>0) and MILU(k) by iluk (mld_d_ilu_solver_bld.f90:126-160)
       nd.nr = nd.nc = n; nd.irp.assign(n + 1, 0); nnz_nd = 0;
-      return ilu0_fact(a, l, u, dv);
+      if (fill == 0 && !milu) return ilu0_fact(a, l, u, dv);
+      return iluk_fact(a, fill, milu, l, u, dv);
     }
     if (kind == SM_DENSELU) {
       nd.nr = nd.nc = n; nd.irp.assign(n + 1, 0); nnz_nd = 0;
@@ -979,9 +1054,12 @@ struct Prec {
     for (int l = 0; l < nlev; ++l) {
       Level& L = lv[l];
       bool coarse = (l == nlev - 1) && nlev > 1;
-      const int kind = coarse ? prm.c_smoother : prm.smoother;
+      int kind = coarse ? prm.c_smoother : prm.smoother;
+      if (kind == SM_AS) kind = SM_BJAC;
       // FBGS (mld_d_base_onelev_seti.F90:128-132): GS before, BWGS after
       L.sm.kind = (kind == SM_FBGS) ? SM_GS : kind;
+      L.sm.fill = coarse ? prm.c_fill : prm.sub_fill;
+      L.sm.milu = (coarse ? prm.c_milu : prm.sub_milu) != 0;
       L.has_sm2 = (kind == SM_FBGS);
       L.sweeps_pre = coarse ? prm.c_sweeps : prm.sweeps_pre;
       L.sweeps_post = coarse ? prm.c_sweeps : prm.sweeps_post;
@@ -1152,6 +1230,15 @@ int orc_ilu0(void* m, void** l, void** u, double* d) {
   Csr* L = new Csr; Csr* U = new Csr;
   std::vector<double> dv;
   int rc = ilu0_fact(*(Csr*)m, *L, *U, dv);
+  std::copy(dv.begin(), dv.end(), d);
+  *l = L; *u = U;
+  return rc;
+}
+// ILU(k) / MILU(k) of m; as orc_ilu0.
+int orc_iluk(void* m, int fill, int milu, void** l, void** u, double* d) {
+  Csr* L = new Csr; Csr* U = new Csr;
+  std::vector<double> dv;
+  int rc = iluk_fact(*(Csr*)m, fill, milu != 0, *L, *U, dv);
   std::copy(dv.begin(), dv.end(), d);
   *l = L; *u = U;
   return rc;

@@ -22,6 +22,7 @@ _lib = None
 # Smoother codes: mld_base_prec_type.F90:166-176 (0..2); 100+ are additions.
 SM_NOPREC, SM_JAC, SM_BJAC, SM_L1JAC, SM_DENSELU = 0, 1, 2, 100, 101
 SM_FBGS, SM_GS, SM_BWGS = 4, 102, 103
+SM_AS = 3
 CYC_ADD, CYC_MULT, CYC_V, CYC_W, CYC_K, CYC_KSYM = 1, 2, 3, 4, 5, 6
 KRYLOV = {"CG": 0, "FCG": 1, "BICGSTAB": 2}
 
@@ -43,6 +44,10 @@ class OrcParams(C.Structure):
         ("sweeps_post", C.c_int),
         ("c_smoother", C.c_int),
         ("c_sweeps", C.c_int),
+        ("sub_fill", C.c_int),
+        ("sub_milu", C.c_int),
+        ("c_fill", C.c_int),
+        ("c_milu", C.c_int),
     ]
 
 
@@ -80,6 +85,7 @@ def lib():
         L.orc_dec_map_bld.argtypes = [vp, C.c_int, C.c_double, ip, ip]
         L.orc_ilu0.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), dp]
         L.orc_trsv.argtypes = [vp, C.c_int, C.c_int, dp, dp]
+        L.orc_iluk.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp), C.POINTER(vp), dp]
         L.orc_prec_new.restype = vp
         L.orc_prec_new.argtypes = [C.POINTER(OrcParams)]
         L.orc_prec_free.argtypes = [vp]
@@ -191,6 +197,14 @@ def ilu0(a: Csr):
     return Csr(lh.value), Csr(uh.value), d, rc
 
 
+def iluk(a: Csr, fill, milu=False):
+    n = a.shape[0]
+    d = np.zeros(n)
+    lh, uh = C.c_void_p(), C.c_void_p()
+    rc = lib().orc_iluk(a.h, int(fill), int(milu), C.byref(lh), C.byref(uh), _dp(d))
+    return Csr(lh.value), Csr(uh.value), d, rc
+
+
 def trsv(t: Csr, x, lower=True, trans=False):
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.zeros_like(x)
@@ -215,6 +229,10 @@ class Params:
     sweeps_post: int = 1
     c_smoother: int = SM_BJAC
     c_sweeps: int = 1
+    sub_fill: int = 0
+    sub_milu: int = 0
+    c_fill: int = 0
+    c_milu: int = 0
 
     def to_c(self):
         return OrcParams(**{f: getattr(self, f) for f, _ in OrcParams._fields_})

@@ -128,7 +128,8 @@ def test_error_codes(eng):
         p.init("NOT-A-PREC")
     assert ex.value.info == 2
     p.init("ML")
-    p.set("SMOOTHER_TYPE", "AS")  # additive Schwarz: not provided by this engine
+    p.set("SMOOTHER_TYPE", "BJAC")
+    p.set("SUB_SOLVE", "ILUT")  # ILU with threshold: not provided by this engine
     from mld2p4_amd import problems
 
     A = problems.pde3d_poisson(6)
@@ -268,3 +269,58 @@ def test_dump_matrix_market(eng, orc, tmp_path):
     q = str(tmp_path / "d")
     p.dump(q, ac=True)
     assert os.path.exists(f"{q}_p000_l002_ac.mtx") and not os.path.exists(f"{q}_p000_l002_r.mtx")
+
+
+def test_additive_schwarz_one_process(eng, orc):
+    """AS on one process is the block-Jacobi smoother with an empty ND
+    (no overlap rows, identity restriction/prolongation): same ILU(0)
+    factors as BJAC; with SUB_SOLVE DIAG only the inverse diagonal, no ND
+    (unlike the Jacobi smoother).  SUB_SOLVE JACOBI names no solver of the
+    reference's list (mld_d_base_onelev_seti.F90:148-192) and keeps ILU(0)."""
+    a = orc.gen_pde3d(10)
+    A = a.to_scipy()
+    p_as = config1(eng, "AS", 1).build(A)
+    p_bj = config1(eng, "BJAC", 1).build(A)
+    po = orc.Prec(orc.config1_params(smoother=orc.SM_AS)).build(a)
+    assert p_as.nlev == p_bj.nlev == po.nlev
+    for l in range(p_as.nlev):
+        ia, ib = p_as.level_info(l), p_bj.level_info(l)
+        assert ia["solver"] == ib["solver"] == 7 and ia["nnz_nd"] == 0
+        for w in ("L", "U"):
+            assert same(p_as.level_csr(l, w), p_bj.level_csr(l, w))
+            assert same(p_as.level_csr(l, w), po.level_csr(l, w).to_scipy())
+        assert (p_as.dv(l) == p_bj.dv(l)).all()
+    q = eng.MldPrec(device=-1).init("AS")
+    q.set("SUB_SOLVE", "DIAG")
+    q.build(A)
+    info = q.level_info(0)
+    assert info["nnz_nd"] == 0 and info["solver"] == 5
+    assert (q.dv(0) == 1.0 / A.diagonal()).all()
+    r = eng.MldPrec(device=-1).init("AS")
+    r.set("SUB_SOLVE", "JACOBI")
+    assert r.build(A).level_info(0)["solver"] == 7
+
+
+@pytest.mark.parametrize("solve,fill", [("ILU", 1), ("ILU", 2), ("MILU", 0), ("MILU", 1)])
+def test_iluk_milu_factors_vs_oracle(eng, orc, solve, fill):
+    """BJAC smoother with ILU(k) or MILU(k) (SUB_SOLVE / SUB_FILLIN): factors
+    and inverse pivots bit-identical to the oracle's."""
+    a = orc.gen_pde3d(12)
+    p = config1(eng, "BJAC", 1)
+    p.set("SUB_SOLVE", solve)
+    p.set("SUB_FILLIN", fill)
+    p.build(a.to_scipy())
+    po = orc.Prec(orc.config1_params(smoother=orc.SM_BJAC, sub_fill=fill, sub_milu=int(solve == "MILU"))).build(a)
+    assert p.nlev == po.nlev
+    for l in range(p.nlev):
+        for w in ("L", "U"):
+            assert same(p.level_csr(l, w), po.level_csr(l, w).to_scipy()), (l, w)
+        assert (p.dv(l) == po.dv(l)).all()
+    if fill > 0:
+        assert p.level_csr(0, "L").nnz > sp_tril_nnz(a.to_scipy())
+
+
+def sp_tril_nnz(A):
+    import scipy.sparse as sp
+
+    return sp.tril(A, -1).nnz

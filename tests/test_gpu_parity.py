@@ -187,6 +187,49 @@ def test_one_level_fbgs_apply(eng, orc, nsw):
         p.apply(dev(x), trans="T")
 
 
+@pytest.mark.parametrize("solve,fill", [("ILU", 2), ("MILU", 1)])
+def test_iluk_smoother_bitexact(eng, orc, solve, fill):
+    """ILU(k) / MILU(k) factors in the level-scheduled solves ('N' and 'T'):
+    bit-identical to the oracle's sequential substitution; the whole
+    preconditioner within the FP64 bar."""
+    _say(f"[ILU(k)] {solve}({fill}) idim=24")
+    a = orc.gen_pde3d(24)
+    A = a.to_scipy()
+    p = engine_prec(eng, A, "BJAC", 1, extra=(("SUB_SOLVE", solve), ("SUB_FILLIN", fill)))
+    po = oracle_prec(orc, a, "BJAC", 1, sub_fill=fill, sub_milu=int(solve == "MILU"))
+    rng = np.random.default_rng(24)
+    n = A.shape[0]
+    x = rng.standard_normal(n)
+    y0 = rng.standard_normal(n)
+    for trans in ("N", "T"):
+        y = dev(y0.copy())
+        p.level_smooth(0, dev(x), y, beta=1, sweeps=1, trans=trans)
+        yo = po.smooth(0, x, alpha=1.0, beta=1.0, y=y0, sweeps=1, trans=trans)
+        assert (y.cpu().numpy() == yo).all(), trans
+    assert relmax(p.apply(dev(x)).cpu().numpy(), po.apply(x)) <= 1e-12
+
+
+def test_additive_schwarz_apply(eng, orc):
+    """AS on one process: the multilevel apply equals BJAC's bit for bit and
+    the oracle's within the FP64 bar; one-level AS with the diagonal solver
+    applies D^-1 once whatever the sweeps."""
+    _say("[AS] idim=20")
+    a = orc.gen_pde3d(20)
+    A = a.to_scipy()
+    p_as = engine_prec(eng, A, "AS", 1)
+    p_bj = engine_prec(eng, A, "BJAC", 1)
+    po = orc.Prec(orc.config1_params(smoother=orc.SM_AS)).build(a)
+    x = np.random.default_rng(23).standard_normal(A.shape[0])
+    y = p_as.apply(dev(x)).cpu().numpy()
+    assert (y == p_bj.apply(dev(x)).cpu().numpy()).all()
+    assert relmax(y, po.apply(x)) <= 1e-12
+    q = eng.MldPrec(device=0).init("AS")
+    q.set("SUB_SOLVE", "DIAG")
+    q.set("SMOOTHER_SWEEPS", 3)
+    q.build(A)
+    assert (q.apply(dev(x)).cpu().numpy() == x * (1.0 / A.diagonal())).all()
+
+
 def test_krylov_default_ml(eng, orc):
     """The reference's default ML preconditioner (FBGS smoother, coarse ILU(0)
     through block Jacobi) in CG: the same iteration count as the oracle."""

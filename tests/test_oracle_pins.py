@@ -198,6 +198,78 @@ def test_ilu0_exact_on_tridiagonal(orc):
     assert np.allclose(A.T @ y, x, rtol=1e-12, atol=1e-12)
 
 
+def poisson2d(nx):
+    T = sp.diags([-np.ones(nx - 1), 4 * np.ones(nx), -np.ones(nx - 1)], [-1, 0, 1])
+    S = sp.diags([-np.ones(nx - 1), -np.ones(nx - 1)], [-1, 1])
+    A = sp.kron(sp.identity(nx), T) + sp.kron(S, sp.identity(nx))
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return A
+
+
+def factors(L, U, d):
+    n = len(d)
+    return L.to_scipy().toarray() + np.eye(n), np.diag(1.0 / d), U.to_scipy().toarray() + np.eye(n)
+
+
+def test_iluk_level_zero_is_ilu0(orc):
+    """ILU(k) with k = 0 and ILU(0) are separate routines in the reference;
+    on ILU(0)'s pattern they perform the same operations in the same order."""
+    for A in (orc.gen_pde3d(5).to_scipy(), orc.read_mtx(os.path.join(GOLD, "mld_mat.mtx"))[:500, :500].tocsr()):
+        a = orc.Csr.from_scipy(A)
+        L0, U0, d0, rc0 = orc.ilu0(a)
+        Lk, Uk, dk, rck = orc.iluk(a, 0)
+        assert rc0 == rck == 0 and (d0 == dk).all()
+        for X, Y in ((L0, Lk), (U0, Uk)):
+            x, y = X.to_scipy(), Y.to_scipy()
+            assert (x.indptr == y.indptr).all() and (x.indices == y.indices).all() and (x.data == y.data).all()
+
+
+def test_iluk_full_fill_is_lu(orc):
+    """With a fill level above any path length nothing is dropped: L D^-1 U = A
+    (the exact LU of a nonsymmetric sparse matrix) and the solves give A^-1 x."""
+    rng = np.random.default_rng(41)
+    A = (poisson2d(6) + sp.diags([0.3 * rng.random(35)], [1], shape=(36, 36))).tocsr()
+    A.sort_indices()
+    L, U, d, rc = orc.iluk(orc.Csr.from_scipy(A), 40)
+    assert rc == 0
+    Lf, Df, Uf = factors(L, U, d)
+    assert np.abs(Lf @ Df @ Uf - A.toarray()).max() <= 1e-13 * np.abs(A).max()
+    x = rng.standard_normal(36)
+    y = orc.trsv(U, d * orc.trsv(L, x, lower=True), lower=False)
+    assert np.allclose(A @ y, x, rtol=1e-12, atol=1e-12)
+
+
+def test_iluk_level_one_pattern(orc):
+    """ILU(1) of the 5-point Laplacian in natural order adds exactly the fill
+    diagonals at offsets -(nx-1) and nx-1 (Saad, Iterative Methods, sec. 10.3.3)."""
+    nx = 7
+    A = poisson2d(nx)
+    L, U, d, rc = orc.iluk(orc.Csr.from_scipy(A), 1)
+    assert rc == 0
+    n = nx * nx
+    pat = set(zip(*sp.tril(A, -1).nonzero())) | set(zip(*sp.triu(A, 1).nonzero()))
+    want = pat | {(i, i - nx + 1) for i in range(n) if i - nx + 1 >= 0 and (i % nx) != nx - 1} \
+               | {(i, i + nx - 1) for i in range(n) if i + nx - 1 < n and (i % nx) != 0}
+    got = set(zip(*L.to_scipy().nonzero())) | set(zip(*U.to_scipy().nonzero()))
+    assert got == want
+
+
+@pytest.mark.parametrize("fill", [0, 1])
+def test_milu_preserves_row_sums(orc, fill):
+    """MILU(k) adds every dropped entry to the pivot, so the factorization
+    keeps A's row sums: (I+L) D^-1 (I+U) 1 = A 1."""
+    A = orc.gen_pde3d(5).to_scipy()
+    L, U, d, rc = orc.iluk(orc.Csr.from_scipy(A), fill, milu=True)
+    assert rc == 0
+    Lf, Df, Uf = factors(L, U, d)
+    e = np.ones(A.shape[0])
+    assert np.allclose(Lf @ (Df @ (Uf @ e)), A @ e, rtol=1e-12, atol=1e-9 * np.abs(A).max())
+    L0, U0, d0, _ = orc.iluk(orc.Csr.from_scipy(A), fill)
+    Lg, Dg, Ug = factors(L0, U0, d0)
+    assert np.abs(Lg @ (Dg @ (Ug @ e)) - A @ e).max() > 1e-6 * np.abs(A).max()  # plain ILU does not
+
+
 # ------------------------------------------------------------- smoothers
 def test_jacobi_closed_forms(orc):
     a = orc.gen_pde3d(5)
