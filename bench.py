@@ -118,6 +118,27 @@ def solve_timed(eng, torch, p, method, b, eps, steps, warmup, itmax=1000):
     return x, times, its, errs
 
 
+def copy_bandwidth(torch, reps=10):
+    """Device-to-device copy of 2 GiB (torch's copy kernel), read + write bytes
+    per second: what a plain streaming kernel reaches on this GPU."""
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / t / 1e9
+
+
 def time_vcycle(torch, p, n, reps=20):
     xr = torch.ones(n, dtype=torch.float64, device="cuda:0")
     y = torch.empty_like(xr)
@@ -546,10 +567,13 @@ def main():
             traffic = json.load(open(tfile)).get(dom)
         except Exception:
             traffic = None
+    copy_gbs = copy_bandwidth(torch)
+    log(f"device copy (2 x 2 GiB per launch): {copy_gbs:.0f} GB/s")
     roofline = {"bound": "hbm", "kernel": f"fine-level {dom} (level 1 of {p.nlev})", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes": int(by), "avg_ms": round(ms, 5),
-                "steps_GBps": {k: round(v[1] / v[0] / 1e6, 1) for k, v in cand.items()}}
+                "steps_GBps": {k: round(v[1] / v[0] / 1e6, 1) for k, v in cand.items()},
+                "copy_GBps": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)}
 
     # ---------------- FP32-mixed path (FP32 hierarchy inside FP64 FCG)
     mixed = None

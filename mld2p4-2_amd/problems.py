@@ -52,6 +52,45 @@ def pde3d_poisson(idim: int):
     return irp, ja, val
 
 
+def pde2d_poisson(idim: int, single_h: bool = True):
+    """5-point matrix of mld_d_gen_pde2d (tests/pdegen/mld_d_pde2d.f90:96-370)
+    with a1 = a2 = 1, b1 = b2 = c = 0: rows x-major (glob_row = (ix-1)*idim +
+    iy), columns (x-1, y-1, centre, y+1, x+1).  The reference forms the mesh
+    size as 1.e0/(idim+2), a default-real (single precision) quotient widened
+    to double (:163); single_h=False uses 1/(idim+2) in double, the mesh size
+    of the reference's golden matrix tests/fileread/runs/mld_mat.mtx."""
+    n = idim * idim
+    h = float(np.float32(1.0) / np.float32(idim + 2)) if single_h else 1.0 / (idim + 2)
+    sq = h * h
+    off = -1.0 / sq - 0.0 / (2.0 * h)
+    cen = 2.0 * (1.0 + 1.0) / sq + 0.0
+    g = np.arange(n, dtype=np.int64)
+    x = g // idim
+    y = g % idim
+    offs = [(-idim, x > 0), (-1, y > 0), (0, None), (1, y < idim - 1), (idim, x < idim - 1)]
+    counts = np.ones(n, dtype=np.int64)
+    for o, m in offs:
+        if m is not None:
+            counts += m
+    irp = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=irp[1:])
+    nnz = int(irp[-1])
+    ja = np.empty(nnz, dtype=np.int32)
+    val = np.empty(nnz, dtype=np.float64)
+    pos = irp[:-1].copy()
+    for o, m in offs:
+        if m is None:
+            ja[pos] = g
+            val[pos] = cen
+            pos += 1
+        else:
+            idx = pos[m]
+            ja[idx] = (g + o)[m]
+            val[idx] = off
+            pos += m
+    return irp, ja, val
+
+
 def box3(g):
     """One pass of the 3x3x3 box filter with edge replication, applied as
     three 3-point means along the axes (separable form of the 27-point mean)."""

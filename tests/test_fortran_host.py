@@ -133,3 +133,69 @@ def test_fileread_driver_gpu_golden(orc):
     rel = float(lines[k + 1].split()[2])
     rel_o = np.linalg.norm(xo - xs) / np.linalg.norm(xs)
     assert abs(rel - rel_o) <= 1e-3 * rel_o + 1e-12, (rel, rel_o)
+
+
+# ------------------------------------------------------------------ 2-D driver
+PDE2D = os.path.join(ROOT, "mld2p4-2_amd", "_build", "fortran", "mld_d_pde2d")
+DECK2D = os.path.join(ROOT, "mld2p4-2_amd", "fortran", "runs", "mld_pde2d_idim64.inp")
+
+
+def test_pde2d_generator_golden(eng):
+    """The 5-point generator of mld_d_pde2d with the mesh size in double
+    precision reproduces the reference's golden matrix mld_mat.mtx (a 64 x 64
+    grid) entry for entry; with the reference's single-precision mesh size
+    only the two values change."""
+    from mld2p4_amd import problems
+
+    A = scipy_io_csr(os.path.join(GOLD, "mld_mat.mtx"))
+    irp, ja, val = problems.pde2d_poisson(64, single_h=False)
+    assert (A.indptr == irp).all() and (A.indices == ja).all() and (A.data == val).all()
+    irp1, ja1, val1 = problems.pde2d_poisson(64)
+    assert (irp1 == irp).all() and (ja1 == ja).all()
+    assert len(np.unique(val1)) == 2 and abs(val1 / val - 1).max() < 1e-6 and (val1 != val).all()
+
+
+def scipy_io_csr(path):
+    import scipy.io
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix(scipy.io.mmread(path))
+    A.sort_indices()
+    return A
+
+
+def test_pde2d_driver_host_build(eng):
+    """mld_d_pde2d on the deck layout of tests/pdegen/runs/mld_pde2d.inp:
+    the hierarchy it builds (host-only handle) has the level sizes of the
+    engine's own build of the same matrix."""
+    if not os.path.exists(PDE2D):
+        pytest.fail(f"Fortran driver not built: {PDE2D} (run __graft_entry__.build())")
+    from mld2p4_amd import problems
+
+    env = dict(os.environ, MLD_HIP_DEVICE="-1")
+    r = subprocess.run([PDE2D], input=open(DECK2D).read(), capture_output=True, text=True, env=env, timeout=300)
+    assert r.returncode == 3, r.stdout + r.stderr
+    sizes = [int(s) for s in re.findall(r"level \d+: n=(\d+)", r.stdout)]
+    p = eng.MldPrec(device=-1).init("ML")
+    for k, v in [("SMOOTHER_TYPE", "JACOBI"), ("SMOOTHER_SWEEPS", 1), ("AGGR_THRESH", 0.0), ("AGGR_PROL", "SMOOTHED"),
+                 ("MIN_COARSE_SIZE", 200), ("COARSE_SOLVE", "BJAC"), ("COARSE_SUBSOLVE", "ILU"), ("COARSE_SWEEPS", 4)]:
+        p.set(k, v)
+    p.build(problems.pde2d_poisson(64))
+    assert sizes == [p.level_info(l)["n"] for l in range(p.nlev)]
+
+
+@pytest.mark.gpu
+def test_pde2d_driver_gpu_solve(orc):
+    print("[fortran driver] 2-D idim=64 CG", flush=True)
+    import scipy.sparse as sp
+    from mld2p4_amd import problems
+
+    r = subprocess.run([PDE2D], input=open(DECK2D).read(), capture_output=True, text=True,
+                       env=dict(os.environ, MLD_HIP_DEVICE="0"), timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    irp, ja, val = problems.pde2d_poisson(64)
+    A = sp.csr_matrix((val, ja, irp), shape=(4096, 4096))
+    po = orc.Prec(orc.config1_params()).build(orc.Csr.from_scipy(A))
+    _, ito, _ = po.krylov("CG", np.ones(4096), eps=1e-6)
+    assert abs(field(r.stdout, "Iterations to convergence") - ito) <= 1
+    assert field(r.stdout, "True relative residual") <= 1.01e-6
