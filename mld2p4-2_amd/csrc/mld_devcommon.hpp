@@ -357,34 +357,48 @@ struct has_pair : std::false_type {};
 template <class E>
 struct has_pair<E, std::void_t<decltype(std::declval<const E&>().aligned2())>> : std::true_type {};
 
-template <int K = 0, typename CT = void, typename V, class G, class E>
-static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red = Red{nullptr, nullptr}) {
-  using C = typename std::conditional<std::is_void<CT>::value, V, CT>::type;
-  if (m.nr == 0) return;
-  if constexpr (!std::is_void<typename NtForm<E>::type>::value) {
-    if (m.fmt == FMT_DIA && m.nr >= NT_MIN_ROWS) {
-      spmv_launch<K, CT>(m, g, NtForm<E>::make(e), s, red);
+// DIA launch (with its final reduction when K > 0): two adjacent rows per
+// thread when the epilogue has the two-row form and the arrays allow it.
+template <int K, typename C, typename V, class G, class E>
+static void dia_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red) {
+  if constexpr (has_pair<E>::value && std::is_same<V, double>::value) {
+    if ((m.nr & 1) == 0 && al2(m.val, sizeof(V)) && e.aligned2()) {
+      const int nbp = (int)(((int64_t)m.nr / 2 + BS - 1) / BS);
+      const int xc = xcd_chunk(m.xmap, nbp, 2 * BS);
+      if (m.w == 7)
+        dia_kernel<C, 7, K, 1, G, E, V, true><<<nbp, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red, xc);
+      else
+        dia_kernel<C, 0, K, 1, G, E, V, true><<<nbp, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red, xc);
+      if constexpr (K > 0) finalize_kernel<K><<<1, FIN_T, 0, s>>>(nbp, red.partial, red.result);
+      HIPCHECK(hipGetLastError());
       return;
     }
   }
   constexpr int RPT = dia_rpt<K>();
-  const int nb = (m.fmt == FMT_DIA) ? (int)(((int64_t)m.nr + BS * RPT - 1) / (BS * RPT)) : spmv_blocks(m);
+  const int nb = (int)(((int64_t)m.nr + BS * RPT - 1) / (BS * RPT));
+  const int xc = xcd_chunk(m.xmap, nb, BS * RPT);
+  if (m.w == 7) dia_kernel<C, 7, K, RPT><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red, xc);
+  else dia_kernel<C, 0, K, RPT><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red, xc);
+  if constexpr (K > 0) finalize_kernel<K><<<1, FIN_T, 0, s>>>(nb, red.partial, red.result);
+  HIPCHECK(hipGetLastError());
+}
+
+template <int K = 0, typename CT = void, typename V, class G, class E>
+static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red = Red{nullptr, nullptr}) {
+  using C = typename std::conditional<std::is_void<CT>::value, V, CT>::type;
+  if (m.nr == 0) return;
   if (m.fmt == FMT_DIA) {
-    if constexpr (K == 0 && has_pair<E>::value && std::is_same<V, double>::value) {
-      if ((m.nr & 1) == 0 && al2(m.val, sizeof(V)) && e.aligned2()) {  // two adjacent rows per thread
-        const int nbp = (int)(((int64_t)m.nr / 2 + BS - 1) / BS);
-        const int xc = xcd_chunk(m.xmap, nbp, 2 * BS);
-        if (m.w == 7)
-          dia_kernel<C, 7, 0, 1, G, E, V, true><<<nbp, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red, xc);
-        else
-          dia_kernel<C, 0, 0, 1, G, E, V, true><<<nbp, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red, xc);
+    if constexpr (!std::is_void<typename NtForm<E>::type>::value) {
+      if (m.nr >= NT_MIN_ROWS) {
+        dia_launch<K, C>(m, g, NtForm<E>::make(e), s, red);
         return;
       }
     }
-    const int xc = xcd_chunk(m.xmap, nb, BS * RPT);
-    if (m.w == 7) dia_kernel<C, 7, K, RPT><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red, xc);
-    else dia_kernel<C, 0, K, RPT><<<nb, BS, 0, s>>>(m.nr, m.nc, m.w, m.off, m.val, g, e, red, xc);
-  } else if (m.fmt == FMT_ELL) {
+    dia_launch<K, C>(m, g, e, s, red);
+    return;
+  }
+  const int nb = spmv_blocks(m);
+  if (m.fmt == FMT_ELL) {
     const int xc = xcd_chunk(m.xmap, nb, BS);
     switch (m.w) {
       case 4: ell_kernel<C, 4, K><<<nb, BS, 0, s>>>(m.nr, m.w, m.ld, m.ja, m.val, g, e, red, xc); break;

@@ -76,7 +76,7 @@ template <typename V, typename YT, bool NT = false>
 struct EpiSet {  // y = A x
   YT* __restrict__ y;
   __device__ __forceinline__ void operator()(int i, V acc, double*) const { stnt<NT>(y + i, (YT)acc); }
-  __device__ __forceinline__ void pair(int i, V a0, V a1) const {
+  __device__ __forceinline__ void pair(int i, V a0, V a1, double*) const {
     stnt<NT>((vec2_t<YT>*)(y + i), vec2_t<YT>{(YT)a0, (YT)a1});
   }
   bool aligned2() const { return al2(y, sizeof(YT)); }
@@ -85,7 +85,7 @@ template <typename V, bool NT = false>
 struct EpiAdd {  // y = y + A x      (prolongation, psb_map_Y2X with beta = 1)
   V* __restrict__ y;
   __device__ __forceinline__ void operator()(int i, V acc, double*) const { stnt<NT>(y + i, y[i] + acc); }
-  __device__ __forceinline__ void pair(int i, V a0, V a1) const {
+  __device__ __forceinline__ void pair(int i, V a0, V a1, double*) const {
     const vec2_t<V> yy = *(const vec2_t<V>*)(y + i);
     stnt<NT>((vec2_t<V>*)(y + i), vec2_t<V>{yy.x + a0, yy.y + a1});
   }
@@ -96,7 +96,7 @@ struct EpiRes {  // r = b - A x     (psb_geaxpby + psb_spmm(-1,A,x,1,r))
   const BT* __restrict__ b;
   V* __restrict__ r;
   __device__ __forceinline__ void operator()(int i, V acc, double*) const { stnt<NT>(r + i, (V)b[i] - acc); }
-  __device__ __forceinline__ void pair(int i, V a0, V a1) const {
+  __device__ __forceinline__ void pair(int i, V a0, V a1, double*) const {
     const vec2_t<BT> bb = *(const vec2_t<BT>*)(b + i);
     stnt<NT>((vec2_t<V>*)(r + i), vec2_t<V>{(V)bb.x - a0, (V)bb.y - a1});
   }
@@ -112,7 +112,7 @@ struct EpiJac {  // t = d*(b - ND y);  y = t  or  y = y + t
     if (ADD) stnt<NT>(y + i, (YT)((V)y[i] + t));
     else stnt<NT>(y + i, (YT)t);
   }
-  __device__ __forceinline__ void pair(int i, V a0, V a1) const {
+  __device__ __forceinline__ void pair(int i, V a0, V a1, double*) const {
     const vec2_t<V> dd = *(const vec2_t<V>*)(d + i);
     const vec2_t<BT> bb = *(const vec2_t<BT>*)(b + i);
     const V t0 = dd.x * ((V)bb.x - a0), t1 = dd.y * ((V)bb.y - a1);
@@ -141,6 +141,22 @@ struct EpiJacOut {
     stnt<NT>(yout + i, (double)yn);
     if (u) dd[0] = u[i] * (double)yn;
   }
+  // two rows; their dot terms are added to dd
+  __device__ __forceinline__ void pair(int i, V a0, V a1, double* dd) const {
+    const vec2_t<V> dv = *(const vec2_t<V>*)(d + i), yv = *(const vec2_t<V>*)(yin + i);
+    const vec2_t<BT> bb = *(const vec2_t<BT>*)(b + i);
+    const V y0 = yv.x + dv.x * ((V)bb.x - a0), y1 = yv.y + dv.y * ((V)bb.y - a1);
+    stnt<NT>((vec2_t<double>*)(yout + i), vec2_t<double>{(double)y0, (double)y1});
+    if (u) {
+      const vec2_t<double> uu = *(const vec2_t<double>*)(u + i);
+      dd[0] += uu.x * (double)y0;
+      dd[0] += uu.y * (double)y1;
+    }
+  }
+  bool aligned2() const {
+    return al2(d, sizeof(V)) && al2(b, sizeof(BT)) && al2(yin, sizeof(V)) && al2(yout, sizeof(double)) &&
+           (!u || al2(u, sizeof(double)));
+  }
 };
 // Krylov SpMV: y = A x and dots u.y (u == nullptr: y.y) and w.y
 template <typename V, bool NT = false>
@@ -154,6 +170,24 @@ struct EpiSetDot {
     d[0] = (u ? (double)u[i] : a) * a;
     if (w) d[1] = (double)w[i] * a;
   }
+  __device__ __forceinline__ void pair(int i, V a0, V a1, double* d) const {
+    stnt<NT>((vec2_t<V>*)(y + i), vec2_t<V>{a0, a1});
+    const double b0 = (double)a0, b1 = (double)a1;
+    if (u) {
+      const vec2_t<V> uu = *(const vec2_t<V>*)(u + i);
+      d[0] += (double)uu.x * b0;
+      d[0] += (double)uu.y * b1;
+    } else {
+      d[0] += b0 * b0;
+      d[0] += b1 * b1;
+    }
+    if (w) {
+      const vec2_t<V> ww = *(const vec2_t<V>*)(w + i);
+      d[1] += (double)ww.x * b0;
+      d[1] += (double)ww.y * b1;
+    }
+  }
+  bool aligned2() const { return al2(y, sizeof(V)) && (!u || al2(u, sizeof(V))) && (!w || al2(w, sizeof(V))); }
 };
 
 // The NT form of an epilogue (same data, non-temporal stores).
@@ -377,6 +411,7 @@ __global__ __launch_bounds__(BS) void dia_kernel(int nr, int nc, int nd, DiaOff 
   const int lb = xcd_block(xchunk);
   if constexpr (PAIR) {
     const int i0 = 2 * (lb * BS + threadIdx.x);
+    double pd[2] = {0.0, 0.0};
     if (i0 < nr) {
       V a0 = 0, a1 = 0;
       const int ndg = (ND > 0) ? ND : nd;
@@ -388,8 +423,9 @@ __global__ __launch_bounds__(BS) void dia_kernel(int nr, int nc, int nd, DiaOff 
         a0 = a0 + (V)vv.x * g(j0);
         a1 = a1 + (V)vv.y * g(j1);
       }
-      e.pair(i0, a0, a1);
+      e.pair(i0, a0, a1, pd);
     }
+    if constexpr (K > 0) block_partials<K>(pd, red.partial, lb);
     return;
   }
   double dd[2] = {0.0, 0.0};
