@@ -205,7 +205,9 @@ int mldhip_level_smooth(mldhip_prec* p, int lev, int which, const double* x, int
 /* Average duration in ms of one cycle step of level lev, timed with HIP
  * events on the handle's stream over reps back-to-back runs.  kind:
  * 0 residual r = x - A y, 1 pre smoother, 2 restriction, 3 prolongation,
- * 4 post smoother, 5 Krylov SpMV with its dot (level 0). */
+ * 4 post smoother, 5 Krylov SpMV with its dot (level 0), 6 the residual
+ * launches of level lev inside reps whole preconditioner applications (each
+ * launch between its own pair of events: the in-cycle duration). */
 int mldhip_time_step(mldhip_prec* p, int lev, int kind, int reps, double* avg_ms);
 /* Device storage of a level operator (what 0 A, 1 P, 2 R, 3 ND):
  * fmt 0 = CSR with `lanes` lanes per row, 1 = ELL of width `width`, 2 = SELL,

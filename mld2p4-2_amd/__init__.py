@@ -346,7 +346,8 @@ class MldPrec:
         self._check(lib().mldhip_level_spmv(self.h, lev, LEVEL_OPS[what], _ptr(x), _ptr(y)), "level_spmv")
         return y
 
-    STEP_KINDS = {"residual": 0, "pre": 1, "restrict": 2, "prolong": 3, "post": 4, "krylov_spmv": 5}
+    STEP_KINDS = {"residual": 0, "pre": 1, "restrict": 2, "prolong": 3, "post": 4, "krylov_spmv": 5,
+                  "residual_in_cycle": 6}
 
     def time_step(self, lev, kind, reps=20):
         """Average ms of one cycle step (HIP events on the handle's stream)."""

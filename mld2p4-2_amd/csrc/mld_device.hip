@@ -106,6 +106,10 @@ class DevPrecT : public DevPrec {
 
   ~DevPrecT() override {
     if (hscal_) (void)hipHostFree(hscal_);
+    for (auto& e : probe_ev_) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
   }
 
   size_t device_bytes() const override { return ar_.bytes; }
@@ -237,6 +241,36 @@ class DevPrecT : public DevPrec {
     if (lev < 0 || lev >= (int)L_.size() || reps <= 0) return E_BAD_LEVEL;
     if ((kind == 2 || kind == 3) && lev == 0) return E_BAD_LEVEL;
     if (kind == 5 && lev != 0) return E_BAD_LEVEL;
+    if (kind == 6) {
+      // the residual launches of level lev inside reps whole applications of
+      // the preconditioner (inputs and caches as in a solve), each bracketed
+      // by events on the handle's stream
+      const size_t want = (size_t)reps * 2 * (size_t)std::max(1, outer_sweeps_) * 4 + 8;
+      while (probe_ev_.size() < want) {
+        hipEvent_t a, b;
+        HIPCHECK(hipEventCreate(&a));
+        HIPCHECK(hipEventCreate(&b));
+        probe_ev_.push_back({a, b});
+      }
+      double* kx = kvec(2);
+      double* kq = kvec(3);
+      fill_kernel<double><<<nblk(n0_), BS, 0, s_>>>(n0_, kx, 1.0);
+      int rc = apply_impl(kx, kq, 'N', nullptr, -1);  // warm-up
+      if (rc) return rc;
+      probe_lev_ = lev;
+      probe_used_ = 0;
+      for (int i = 0; i < reps && !rc; ++i) rc = apply_impl(kx, kq, 'N', nullptr, -1);
+      probe_lev_ = -1;
+      HIPCHECK(hipStreamSynchronize(s_));
+      double tot = 0.0;
+      for (size_t i = 0; i < probe_used_; ++i) {
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, probe_ev_[i].first, probe_ev_[i].second));
+        tot += ms;
+      }
+      *avg_ms = probe_used_ ? tot / (double)probe_used_ : 0.0;
+      return rc ? rc : (probe_used_ ? 0 : E_BAD_LEVEL);
+    }
     DLevel<V>& D = L_[lev];
     // deterministic, finite operands
     fill_kernel<V><<<nblk(D.n), BS, 0, s_>>>(D.n, D.x2l, (V)1);
@@ -722,8 +756,15 @@ class DevPrecT : public DevPrec {
 
   void residual(DLevel<V>& D, const V* b, const V* yv, V* r, char trans) {
     const DMat<V>& a = (trans != 'N') ? D.at : D.a;
+    const bool probe = (probe_lev_ >= 0 && &D == &L_[probe_lev_] && probe_used_ < probe_ev_.size());
+    if (probe) HIPCHECK(hipEventRecord(probe_ev_[probe_used_].first, s_));
     spmv_launch(a, GatherX<V, V>{yv}, EpiRes<V, V>{b, r}, s_);
+    if (probe) HIPCHECK(hipEventRecord(probe_ev_[probe_used_++].second, s_));
   }
+  // in-cycle timing of one level's residual launches (time_step kind 6)
+  int probe_lev_ = -1;
+  size_t probe_used_ = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> probe_ev_;
 
   // inner_ml_aply (mld_dmlprec_aply.f90:382-464): dispatch on the level's cycle
   int inner_ml_aply(int level, char trans) {
