@@ -465,6 +465,21 @@ __global__ __launch_bounds__(BS) void sell_kernel(int nr, const int* __restrict_
     const int w = (sptr[s + 1] - sptr[s]) >> 6;
     V acc = 0;
     int k = 0;
+    // eight entries per step: all index and value loads, then all gathers,
+    // in flight together
+    for (; k + 8 <= w; k += 8) {
+      const int i0 = base + 64 * k;
+      int j[8];
+      V v[8], x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) j[u] = ldnt(ja + i0 + 64 * u);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = (V)ldnt(val + i0 + 64 * u);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = g(j[u]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = acc + v[u] * x[u];
+    }
     for (; k + 4 <= w; k += 4) {
       const int i0 = base + 64 * k;
       const int j0 = ldnt(ja + i0), j1 = ldnt(ja + i0 + 64), j2 = ldnt(ja + i0 + 128), j3 = ldnt(ja + i0 + 192);
