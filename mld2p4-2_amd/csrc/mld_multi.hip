@@ -16,10 +16,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <memory>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mld_devcommon.hpp"
@@ -34,22 +36,46 @@ using namespace mld::k;
 constexpr int MB = 32;  // columns of a block multi-vector
 
 // ----------------------------------------------------- block epilogues
-// operator()(row, idx = row*MB + column, acc)
+// operator()(row, idx = row*MB + column, acc) for one column; vec<R>(row,
+// idx, acc) for R adjacent columns with R-wide loads and stores (idx is a
+// multiple of R, and every multi-vector starts on a 256-byte boundary)
+template <typename V, int R>
+using bv_t = V __attribute__((ext_vector_type(R)));
+
+template <typename V, int R>
+__device__ __forceinline__ bv_t<V, R>& bvat(V* p, int64_t i) {
+  return *reinterpret_cast<bv_t<V, R>*>(p + i);
+}
+template <typename V, int R>
+__device__ __forceinline__ bv_t<V, R> bvld(const V* p, int64_t i) {
+  return *reinterpret_cast<const bv_t<V, R>*>(p + i);
+}
+
 template <typename V>
 struct BSet {
   V* __restrict__ y;
   __device__ __forceinline__ void operator()(int, int64_t i, V acc) const { y[i] = acc; }
+  template <int R>
+  __device__ __forceinline__ void vec(int, int64_t i, bv_t<V, R> a) const { bvat<V, R>(y, i) = a; }
 };
 template <typename V>
 struct BAdd {
   V* __restrict__ y;
   __device__ __forceinline__ void operator()(int, int64_t i, V acc) const { y[i] = y[i] + acc; }
+  template <int R>
+  __device__ __forceinline__ void vec(int, int64_t i, bv_t<V, R> a) const {
+    bvat<V, R>(y, i) = bvld<V, R>(y, i) + a;
+  }
 };
 template <typename V>
 struct BRes {
   const V* __restrict__ b;
   V* __restrict__ r;
   __device__ __forceinline__ void operator()(int, int64_t i, V acc) const { r[i] = b[i] - acc; }
+  template <int R>
+  __device__ __forceinline__ void vec(int, int64_t i, bv_t<V, R> a) const {
+    bvat<V, R>(r, i) = bvld<V, R>(b, i) - a;
+  }
 };
 template <typename V, bool ADD>
 struct BJac {  // t = d_row*(b - ND y);  y = t  or  y = y + t
@@ -60,6 +86,12 @@ struct BJac {  // t = d_row*(b - ND y);  y = t  or  y = y + t
     const V t = d[row] * (b[i] - acc);
     if (ADD) y[i] = y[i] + t;
     else y[i] = t;
+  }
+  template <int R>
+  __device__ __forceinline__ void vec(int row, int64_t i, bv_t<V, R> a) const {
+    const bv_t<V, R> t = d[row] * (bvld<V, R>(b, i) - a);
+    if (ADD) bvat<V, R>(y, i) = bvld<V, R>(y, i) + t;
+    else bvat<V, R>(y, i) = t;
   }
 };
 
@@ -75,6 +107,11 @@ struct BJacOut {
     const V t = d[row] * (b[i] - acc);
     yout[i] = (double)(yin[i] + t);
   }
+  template <int R>
+  __device__ __forceinline__ void vec(int row, int64_t i, bv_t<V, R> a) const {
+    const bv_t<V, R> t = d[row] * (bvld<V, R>(b, i) - a);
+    bvat<double, R>(yout, i) = __builtin_convertvector(bvld<V, R>(yin, i) + t, bv_t<double, R>);
+  }
 };
 
 // Y = A X for row-interleaved X (nc x 32) and Y (nr x 32), CSR A.  A group
@@ -85,10 +122,11 @@ struct BJacOut {
 // column's sum runs over the row in stored order, as the sequential loop.
 template <typename V, int RPL, class E>
 __global__ __launch_bounds__(BS) void bspmm_kernel(int nr, const int* __restrict__ irp, const int* __restrict__ ja,
-                                                   const V* __restrict__ val, const V* __restrict__ x, E e) {
+                                                   const V* __restrict__ val, const V* __restrict__ x, E e,
+                                                   int xchunk) {
   constexpr int LPR = MB / RPL;
   constexpr int RPB = BS / LPR;
-  const int row = blockIdx.x * RPB + threadIdx.x / LPR;
+  const int row = xcd_block(xchunk) * RPB + threadIdx.x / LPR;
   const int lane = threadIdx.x % LPR;
   if (row >= nr) return;  // whole lane groups leave together
   const int k0 = irp[row], k1 = irp[row + 1];
@@ -131,54 +169,113 @@ __global__ __launch_bounds__(BS) void bspmm_kernel(int nr, const int* __restrict
     }
   }
   const int64_t base = (int64_t)row * MB + lane * RPL;
+  if constexpr (RPL == 1) {
+    e(row, base, acc[0]);
+  } else {
+    bv_t<V, RPL> a;
 #pragma unroll
-  for (int r = 0; r < RPL; ++r) e(row, base + r, acc[r]);
+    for (int r = 0; r < RPL; ++r) a[r] = acc[r];
+    e.template vec<RPL>(row, base, a);
+  }
 }
 
 // Y = A X for a DIA operator (see dia_kernel): a lane group per row reads
 // the row's diagonal values (one address per group) and the 32-column rows
 // i + o_d of X, diagonals in ascending offset order (= stored column order).
-template <typename V, int RPL, int ND, class E>
+// DX: the gathered values are d_j * x_j (the first Jacobi sweep from zero,
+// formed here instead of in a separate pass; same products).  With a fixed
+// diagonal count all index, value, X and d loads of a lane are issued before
+// the first product.
+template <typename V, int RPL, int ND, bool DX, class E>
 __global__ __launch_bounds__(BS) void bdia_kernel(int nr, int nc, int nd, DiaOff off, const V* __restrict__ val,
-                                                  const V* __restrict__ x, E e, const V* __restrict__ dx) {
+                                                  const V* __restrict__ x, E e, const V* __restrict__ dx,
+                                                  int xchunk) {
   constexpr int LPR = MB / RPL;
   constexpr int RPB = BS / LPR;
-  const int row = blockIdx.x * RPB + threadIdx.x / LPR;
+  const int row = xcd_block(xchunk) * RPB + threadIdx.x / LPR;
   const int lane = threadIdx.x % LPR;
   if (row >= nr) return;
   typedef V vec_t __attribute__((ext_vector_type(RPL)));
   V acc[RPL];
 #pragma unroll
   for (int r = 0; r < RPL; ++r) acc[r] = (V)0;
-  const int ndg = (ND > 0) ? ND : nd;
+  if constexpr (ND > 0) {
+    int j[ND];
+    V v[ND], dj[ND];
+    vec_t w[ND];
 #pragma unroll
-  for (int k = 0; k < ((ND > 0) ? ND : 8); ++k) {
-    if (ND == 0 && k >= ndg) break;
-    const int j = min(max(row + off.o[k], 0), nc - 1);
-    const V v = val[(int64_t)k * nr + row];
-    // dx != null: the gathered values are d_j * x_j (the first Jacobi sweep
-    // from zero, formed here instead of in a separate pass; same products)
-    if constexpr (RPL == 1) {
-      V w = x[(int64_t)j * MB + lane];
-      if (dx) w = dx[j] * w;
-      acc[0] = acc[0] + v * w;
-    } else {
+    for (int k = 0; k < ND; ++k) j[k] = min(max(row + off.o[k], 0), nc - 1);
+#pragma unroll
+    for (int k = 0; k < ND; ++k) v[k] = val[(int64_t)k * nr + row];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) w[k] = *reinterpret_cast<const vec_t*>(x + (int64_t)j[k] * MB + lane * RPL);
+    if constexpr (DX) {
+#pragma unroll
+      for (int k = 0; k < ND; ++k) dj[k] = dx[j[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) {
+        V t = w[k][r];
+        if constexpr (DX) t = dj[k] * t;
+        acc[r] = acc[r] + v[k] * t;
+      }
+    }
+  } else {
+    for (int k = 0; k < nd; ++k) {
+      const int j = min(max(row + off.o[k], 0), nc - 1);
+      const V v = val[(int64_t)k * nr + row];
       vec_t w = *reinterpret_cast<const vec_t*>(x + (int64_t)j * MB + lane * RPL);
-      if (dx) {
-        const V dj = dx[j];
+      if constexpr (DX) {
+        const V d = dx[j];
 #pragma unroll
-        for (int r = 0; r < RPL; ++r) w[r] = dj * w[r];
+        for (int r = 0; r < RPL; ++r) w[r] = d * w[r];
       }
 #pragma unroll
       for (int r = 0; r < RPL; ++r) acc[r] = acc[r] + v * w[r];
     }
   }
   const int64_t base = (int64_t)row * MB + lane * RPL;
+  if constexpr (RPL == 1) {
+    e(row, base, acc[0]);
+  } else {
+    bv_t<V, RPL> a;
 #pragma unroll
-  for (int r = 0; r < RPL; ++r) e(row, base + r, acc[r]);
+    for (int r = 0; r < RPL; ++r) a[r] = acc[r];
+    e.template vec<RPL>(row, base, a);
+  }
 }
 
 // y = d_row .* x  (ADD: y += d_row .* x) over n rows of 32 columns
+template <class E>
+struct is_bjac : std::false_type {};
+template <typename V, bool ADD>
+struct is_bjac<BJac<V, ADD>> : std::true_type {};
+template <typename V>
+struct is_bjac<BJacOut<V>> : std::true_type {};
+
+// Chunk (in blocks) of the row-block order of a block product (see
+// xcd_block).  A row of a 32-column multi-vector is 128 (FP32) or 256 (FP64)
+// bytes, so an XCD's 4 MB L2 holds only 16-32K rows of X.  xmap 1 on a DIA
+// operator: chunks of (largest |offset|)/8 rows, so that the eight XCDs
+// together advance one offset span per period and the rows a block gathers
+// at -o and +o were, or will be, fetched by the same XCD (for a 192^3 grid:
+// 4608-row chunks, three of them per XCD in flight = 1.8 MB of FP32 X);
+// xmap 1 on CSR: 4096-row chunks; xmap 2: one contiguous range per XCD.
+template <typename T>
+static int bxchunk(const DMat<T>& m, int rpb, int nb) {
+  if (m.xmap == 2) return std::max(1, nb / 8);
+  if (m.xmap != 1) return 0;
+  int rows = 4096;
+  if (m.fmt == FMT_DIA) {
+    int mo = 0;
+    for (int k = 0; k < m.w; ++k) mo = std::max(mo, std::abs(m.off.o[k]));
+    rows = mo / 8;
+  }
+  return std::max(1, rows / rpb);
+}
+
 template <typename V, bool ADD>
 __global__ __launch_bounds__(BS) void bvmul_kernel(int n, const V* __restrict__ d, const V* __restrict__ x,
                                                    V* __restrict__ y) {
@@ -203,9 +300,11 @@ __global__ __launch_bounds__(BS) void convert64_kernel(int64_t n, const TI* __re
 
 // Column-major (ld) <-> row-interleaved (32 columns) through a 64 x 32 LDS
 // tile, both sides coalesced.  Columns >= nrhs are zero in the interleaved
-// copy and not written back.
+// copy and not written back.  The interleaved side is FP64 or the FP32 copy
+// of an FP32 preconditioner (rounded on the way in, widened on the way out).
+template <typename T>
 __global__ __launch_bounds__(BS) void col2il_kernel(int n, int nrhs, const double* __restrict__ src, int64_t ld,
-                                                    double* __restrict__ dst) {
+                                                    T* __restrict__ dst) {
   __shared__ double tile[64][MB + 1];
   const int row0 = blockIdx.x * 64;
   const int t = threadIdx.x;
@@ -218,10 +317,11 @@ __global__ __launch_bounds__(BS) void col2il_kernel(int n, int nrhs, const doubl
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int e = t + BS * k, i = e / MB, j = e % MB;
-    if (row0 + i < n) dst[(int64_t)(row0 + i) * MB + j] = tile[i][j];
+    if (row0 + i < n) dst[(int64_t)(row0 + i) * MB + j] = (T)tile[i][j];
   }
 }
-__global__ __launch_bounds__(BS) void il2col_kernel(int n, int nrhs, const double* __restrict__ src,
+template <typename T>
+__global__ __launch_bounds__(BS) void il2col_kernel(int n, int nrhs, const T* __restrict__ src,
                                                     double* __restrict__ dst, int64_t ld) {
   __shared__ double tile[64][MB + 1];
   const int row0 = blockIdx.x * 64;
@@ -229,7 +329,7 @@ __global__ __launch_bounds__(BS) void il2col_kernel(int n, int nrhs, const doubl
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int e = t + BS * k, i = e / MB, j = e % MB;
-    tile[i][j] = (row0 + i < n) ? src[(int64_t)(row0 + i) * MB + j] : 0.0;
+    tile[i][j] = (row0 + i < n) ? (double)src[(int64_t)(row0 + i) * MB + j] : 0.0;
   }
   __syncthreads();
 #pragma unroll
@@ -828,11 +928,21 @@ class MultiT : public MultiEngine {
     HIPCHECK(hipSetDevice(device_));
     if (nrhs < 1 || nrhs > NB || ldx < n0_ || ldy < n0_) return E_BAD_ARG;
     wait_caller(caller);
-    double* xi = kv_[0];
-    double* yi = kv_[1];
-    to_il(nrhs, x, ldx, xi);
-    apply_il(xi, yi);
-    from_il(nrhs, yi, y, ldy);
+    if constexpr (std::is_same<V, float>::value && NB == MB) {
+      // FP32 preconditioner: the layout change rounds the input into the
+      // fine level's FP32 buffer and widens the FP32 result on the way out
+      // (the same values as rounding, and widening, in separate passes)
+      MLevel& L0 = L_[0];
+      to_il(nrhs, x, ldx, L0.x2l);
+      apply_il(nullptr, nullptr, true);
+      from_il(nrhs, L0.y2l, y, ldy);
+    } else {
+      double* xi = kv_[0];
+      double* yi = kv_[1];
+      to_il(nrhs, x, ldx, xi);
+      apply_il(xi, yi);
+      from_il(nrhs, yi, y, ldy);
+    }
     HIPCHECK(hipEventRecord(ev_, s_));
     HIPCHECK(hipStreamWaitEvent(caller, ev_, 0));
     return 0;
@@ -952,7 +1062,7 @@ class MultiT : public MultiEngine {
     } else {
       DMat<V> m = upload_mat<V>(h, ar_, s_, dia_eligible(h, dia_offsets(h)) ? 6 : 1, 0);
       if (m.fmt != FMT_DIA && m.fmt != FMT_CSR) m = upload_mat<V>(h, ar_, s_, 1, 1);
-      m.rpl = tune_rpl<V>(m);
+      tune_rpl<V>(m);
       return m;
     }
   }
@@ -962,7 +1072,7 @@ class MultiT : public MultiEngine {
     } else {
       DMat<double> m = upload_mat<double>(h, ar_, s_, dia_eligible(h, dia_offsets(h)) ? 6 : 1, 0);
       if (m.fmt != FMT_DIA && m.fmt != FMT_CSR) m = upload_mat<double>(h, ar_, s_, 1, 1);
-      m.rpl = tune_rpl<double>(m);
+      tune_rpl<double>(m);
       return m;
     }
   }
@@ -973,46 +1083,73 @@ class MultiT : public MultiEngine {
     return d;
   }
 
-  template <typename T, class E>
-  void bspmm(const DMat<T>& m, const T* x, E e, int rpl, const T* dx = nullptr) {
-    if (m.nr == 0) return;
-    if (m.fmt == FMT_DIA) {
-      const int lpr = MB / rpl, rpb = BS / lpr, nb = (m.nr + rpb - 1) / rpb;
-      switch (rpl) {
-        case 4:
-          if (m.w == 7) bdia_kernel<T, 4, 7, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
-          else bdia_kernel<T, 4, 0, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
-          break;
-        case 2:
-          if (m.w == 7) bdia_kernel<T, 2, 7, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
-          else bdia_kernel<T, 2, 0, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
-          break;
-        default:
-          if (m.w == 7) bdia_kernel<T, 1, 7, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
-          else bdia_kernel<T, 1, 0, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx);
-          break;
-      }
-      HIPCHECK(hipGetLastError());
-      return;
-    }
+  template <typename T, bool DX, class E>
+  void bdia_launch(const DMat<T>& m, const T* x, E e, const T* dx, int rpl, int nb, int xc) {
     switch (rpl) {
       case 4:
-        bspmm_kernel<T, 4, E><<<(m.nr + BS / 8 - 1) / (BS / 8), BS, 0, s_>>>(m.nr, m.irp, m.ja, m.val, x, e);
+        if (m.w == 7) bdia_kernel<T, 4, 7, DX, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx, xc);
+        else bdia_kernel<T, 4, 0, DX, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx, xc);
         break;
       case 2:
-        bspmm_kernel<T, 2, E><<<(m.nr + BS / 16 - 1) / (BS / 16), BS, 0, s_>>>(m.nr, m.irp, m.ja, m.val, x, e);
+        if (m.w == 7) bdia_kernel<T, 2, 7, DX, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx, xc);
+        else bdia_kernel<T, 2, 0, DX, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx, xc);
         break;
       default:
-        bspmm_kernel<T, 1, E><<<(m.nr + BS / 32 - 1) / (BS / 32), BS, 0, s_>>>(m.nr, m.irp, m.ja, m.val, x, e);
+        if (m.w == 7) bdia_kernel<T, 1, 7, DX, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx, xc);
+        else bdia_kernel<T, 1, 0, DX, E><<<nb, BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, x, e, dx, xc);
         break;
     }
     HIPCHECK(hipGetLastError());
   }
 
-  // columns per lane of a block product, by timing Y = A X (large operators)
+  template <typename T, class E>
+  void bspmm(const DMat<T>& m, const T* x, E e, int rpl, const T* dx = nullptr) {
+    if (m.nr == 0) return;
+    const int lpr = MB / rpl, rpb = BS / lpr, nb = (m.nr + rpb - 1) / rpb;
+    const int xc = bxchunk(m, rpb, nb);
+    if (m.fmt == FMT_DIA) {
+      if constexpr (is_bjac<E>::value) {
+        if (dx) {
+          bdia_launch<T, true>(m, x, e, dx, rpl, nb, xc);
+          return;
+        }
+      }
+      if (dx) throw DevError("bspmm: d .* x gathers only in Jacobi sweeps");
+      bdia_launch<T, false>(m, x, e, dx, rpl, nb, xc);
+      return;
+    }
+    if (dx) throw DevError("bspmm: d .* x gathers need DIA storage");
+    switch (rpl) {
+      case 4:
+        bspmm_kernel<T, 4, E><<<nb, BS, 0, s_>>>(m.nr, m.irp, m.ja, m.val, x, e, xc);
+        break;
+      case 2:
+        bspmm_kernel<T, 2, E><<<nb, BS, 0, s_>>>(m.nr, m.irp, m.ja, m.val, x, e, xc);
+        break;
+      default:
+        bspmm_kernel<T, 1, E><<<nb, BS, 0, s_>>>(m.nr, m.irp, m.ja, m.val, x, e, xc);
+        break;
+    }
+    HIPCHECK(hipGetLastError());
+  }
+
+  // columns per lane and row-block order of a block product, by timing
+  // Y = A X (large operators); ties within 2 % go to the earlier candidate.
+  // MLD_BLOCK_SPLIT="R,X" fixes them for every operator (R in 1,2,4 columns
+  // per lane, X in 0,1,2 as xmap), for tests of each variant.
   template <typename T>
-  int tune_rpl(const DMat<T>& m) {
-    if (m.nr < 4096 || !autotune_enabled()) return 1;
+  void tune_rpl(DMat<T>& m) {
+    m.rpl = 1;
+    m.xmap = 0;
+    if (const char* f = std::getenv("MLD_BLOCK_SPLIT")) {
+      int r = 0, xm = -1;
+      if (std::sscanf(f, "%d,%d", &r, &xm) != 2 || !(r == 1 || r == 2 || r == 4) || xm < 0 || xm > 2)
+        throw DevError(std::string("MLD_BLOCK_SPLIT must be R,X with R in 1,2,4 and X in 0,1,2: ") + f);
+      m.rpl = r;
+      m.xmap = xm;
+      return;
+    }
+    if (m.nr < 4096 || !autotune_enabled()) return;
     T* x = nullptr;
     T* y = nullptr;
     HIPCHECK(hipMalloc(&x, sizeof(T) * (size_t)std::max(1, m.nc) * MB));
@@ -1022,30 +1159,35 @@ class MultiT : public MultiEngine {
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
-    int best = 1;
+    int best_r = 1, best_x = 0;
     float tbest = 1e30f;
-    for (int rpl : {1, 2, 4}) {
-      bspmm(m, x, BSet<T>{y}, rpl);
-      float tmin = 1e30f;
-      for (int trial = 0; trial < 3; ++trial) {
-        HIPCHECK(hipEventRecord(e0, s_));
-        for (int r = 0; r < 3; ++r) bspmm(m, x, BSet<T>{y}, rpl);
-        HIPCHECK(hipEventRecord(e1, s_));
-        HIPCHECK(hipEventSynchronize(e1));
-        float t = 0.f;
-        HIPCHECK(hipEventElapsedTime(&t, e0, e1));
-        tmin = std::min(tmin, t);
-      }
-      if (tmin < tbest) {
-        tbest = tmin;
-        best = rpl;
+    for (int xm : {1, 0, 2}) {
+      for (int rpl : {4, 2, 1}) {
+        m.xmap = xm;
+        bspmm(m, x, BSet<T>{y}, rpl);
+        float tmin = 1e30f;
+        for (int trial = 0; trial < 3; ++trial) {
+          HIPCHECK(hipEventRecord(e0, s_));
+          for (int r = 0; r < 3; ++r) bspmm(m, x, BSet<T>{y}, rpl);
+          HIPCHECK(hipEventRecord(e1, s_));
+          HIPCHECK(hipEventSynchronize(e1));
+          float t = 0.f;
+          HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+          tmin = std::min(tmin, t);
+        }
+        if (tmin < 0.98f * tbest) {
+          tbest = tmin;
+          best_r = rpl;
+          best_x = xm;
+        }
       }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(x);
     (void)hipFree(y);
-    return best;
+    m.rpl = best_r;
+    m.xmap = best_x;
   }
 
   // ------------------------------------------------- operators on levels
@@ -1181,7 +1323,8 @@ class MultiT : public MultiEngine {
 
   // y = M^-1 x, row-interleaved FP64 device vectors of n0 x NB.  FP32 form:
   // preconverted = the fine level's input buffer already holds x in FP32;
-  // the last post-smoothing sweep of the fine level writes y in FP64.
+  // the last post-smoothing sweep of the fine level writes y in FP64, or,
+  // with y = null, leaves the result in the fine level's FP32 output buffer.
   void apply_il(const double* x, double* y, bool preconverted = false) {
     MLevel& L0 = L_[0];
     const int64_t cnt = (int64_t)n0_ * NB;
@@ -1199,7 +1342,7 @@ class MultiT : public MultiEngine {
       out_done_ = false;
       inner(0);
       out64_ = nullptr;
-      if (!out_done_) convert64_kernel<double, V><<<nblk64(cnt), BS, 0, s_>>>(cnt, L0.y2l, y);
+      if (y && !out_done_) convert64_kernel<double, V><<<nblk64(cnt), BS, 0, s_>>>(cnt, L0.y2l, y);
     }
     HIPCHECK(hipGetLastError());
   }
@@ -1213,19 +1356,23 @@ class MultiT : public MultiEngine {
   }
 
   // --------------------------------------------------------- layout
-  void to_il(int nrhs, const double* src, int64_t ld, double* dst) {
+  template <typename T>
+  void to_il(int nrhs, const double* src, int64_t ld, T* dst) {
     if constexpr (NB == 1) {
+      static_assert(std::is_same<T, double>::value, "one-column copies are FP64");
       HIPCHECK(hipMemcpyAsync(dst, src, sizeof(double) * n0_, hipMemcpyDeviceToDevice, s_));
     } else {
-      col2il_kernel<<<(n0_ + 63) / 64, BS, 0, s_>>>(n0_, nrhs, src, ld, dst);
+      col2il_kernel<T><<<(n0_ + 63) / 64, BS, 0, s_>>>(n0_, nrhs, src, ld, dst);
       HIPCHECK(hipGetLastError());
     }
   }
-  void from_il(int nrhs, const double* src, double* dst, int64_t ld) {
+  template <typename T>
+  void from_il(int nrhs, const T* src, double* dst, int64_t ld) {
     if constexpr (NB == 1) {
+      static_assert(std::is_same<T, double>::value, "one-column copies are FP64");
       HIPCHECK(hipMemcpyAsync(dst, src, sizeof(double) * n0_, hipMemcpyDeviceToDevice, s_));
     } else {
-      il2col_kernel<<<(n0_ + 63) / 64, BS, 0, s_>>>(n0_, nrhs, src, dst, ld);
+      il2col_kernel<T><<<(n0_ + 63) / 64, BS, 0, s_>>>(n0_, nrhs, src, dst, ld);
       HIPCHECK(hipGetLastError());
     }
   }

@@ -93,6 +93,32 @@ def test_block_apply_mixed(eng, orc):
         assert relmax(Y[j], po.apply(X[j])) <= 1e-5, j
 
 
+@pytest.mark.parametrize("split", ["1,0", "1,1", "2,1", "4,0", "4,1", "4,2", "2,2"])
+@pytest.mark.parametrize("precision,tol", [("double", 1e-12), ("mixed", 1e-5)])
+def test_block_apply_split(eng, orc, split, precision, tol, monkeypatch):
+    """Every columns-per-lane split and row-block order of the block products
+    (DIA fine level, CSR coarse levels; at idim=20 with one column per lane
+    the DIA chunk is 6 blocks, with a partial last period)."""
+    _say(f"[block apply split] {split} {precision} idim=20")
+    monkeypatch.setenv("MLD_BLOCK_SPLIT", split)
+    a = orc.gen_pde3d(20)
+    A = a.to_scipy()
+    p = eng_prec(eng, A, precision=precision)
+    po = orc_prec(orc, a)
+    X = np.random.default_rng(11).standard_normal((32, A.shape[0]))
+    Y = p.apply_block(dev(X)).cpu().numpy()
+    for j in (0, 9, 22, 31):
+        assert relmax(Y[j], po.apply(X[j])) <= tol, j
+
+
+def test_block_split_rejects_bad_value(eng, orc, monkeypatch):
+    monkeypatch.setenv("MLD_BLOCK_SPLIT", "3,0")
+    a = orc.gen_pde3d(8)
+    p = eng_prec(eng, a.to_scipy())
+    with pytest.raises(Exception, match="MLD_BLOCK_SPLIT"):
+        p.apply_block(dev(np.ones((32, 512))))
+
+
 @pytest.mark.parametrize("method,precision", [("CG", "double"), ("FCG", "mixed")])
 def test_block_krylov(eng, orc, method, precision):
     _say(f"[block krylov] {method} {precision} idim=20")
