@@ -57,7 +57,7 @@ int mldhip_set_precision(mldhip_prec* p, int kind);
  * 4096), 4 = SELL with windows of 256 rows, 5 = SELL without sorting, 6 = DIA
  * (diagonal storage, for operators with at most 8 distinct diagonals; others
  * fall back to the automatic choice).  The automatic choice takes DIA for such
- * banded operators. */
+ * banded operators, else the layout it times fastest. */
 int mldhip_set_format(mldhip_prec* p, int fmt);
 
 /* ---- build -------------------------------------------------------------- */

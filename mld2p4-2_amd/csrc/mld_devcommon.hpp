@@ -55,6 +55,7 @@ struct DSmoother {
   int smoother = JAC, solver = DIAG_SCALE;
   int64_t nnz_nd = 0;
   DMat<V> nd, ndt;     // ND and its transpose (uploaded for 'T' applies)
+  DMat<V> ndd;         // DIA ND with its columns scaled by d (ND D): first sweep from zero
   V* dv = nullptr;
   // ILU(0): factors and level schedules; for 'T': transposed factors
   DMat<V> l, u, lt, ut;

@@ -134,6 +134,30 @@ class DevPrecT : public DevPrec {
     if (dot_u && !out_done_) dot1(dot_u, y, dot_slot);
     return 0;
   }
+  // FP32 form, untransposed multilevel cycle with one outer sweep: z = M^-1 x
+  // left in the fine level's FP32 output buffer (returned), dot u.z into
+  // dot_slot.  The values are those apply_impl widens to FP64.
+  int apply_keep32(const double* x, const double* dot_u, int dot_slot, const V** z) {
+    HIPCHECK(hipSetDevice(device_));
+    if (std::is_same<V, double>::value || L_.size() < 2 || outer_sweeps_ != 1) return E_UNSUPPORTED;
+    keep32_ = true;
+    out64_ = nullptr;
+    dot_u_ = dot_u;
+    dot_slot_ = dot_slot;
+    out_done_ = false;
+    int rc = apply_cycle(x, nullptr, 'N');
+    keep32_ = false;
+    dot_u_ = nullptr;
+    if (rc) return rc;
+    *z = L_[0].y2l;
+    if (dot_u && !out_done_) {
+      dot_kernel<double, 1, V><<<red_blocks(n0_), BS, 0, s_>>>(n0_, dot_u, L_[0].y2l, nullptr, nullptr,
+                                                               red(dot_slot));
+      finalize(1, dot_slot);
+      HIPCHECK(hipGetLastError());
+    }
+    return 0;
+  }
 
   int apply_cycle(const double* x, double* y, char trans) {
     const int nlev = (int)L_.size();
@@ -176,7 +200,7 @@ class DevPrecT : public DevPrec {
     L0.x2l = const_cast<V*>(in_vec(x, L0.x2l, L0.n));
     L0.y2l = out_vec(y, L0.y2l);
     int rc = inner_ml_aply(0, trans);
-    if (!out_done_) finish_out(y, L0.y2l, L0.n);
+    if (y && !out_done_) finish_out(y, L0.y2l, L0.n);
     L0.x2l = x0save;
     L0.y2l = y0save;
     return rc;
@@ -184,6 +208,7 @@ class DevPrecT : public DevPrec {
 
   // fused-output state of the current apply_impl call
   double* out64_ = nullptr;
+  bool keep32_ = false;  // apply_keep32: the cycle's result stays in the FP32 y2l
   const double* dot_u_ = nullptr;
   int dot_slot_ = -1;
   bool out_done_ = false;
@@ -397,6 +422,17 @@ class DevPrecT : public DevPrec {
       HIPCHECK(hipStreamSynchronize(s_));
     }
     if (H.nnz_nd > 0) S.nd = upload_tuned<V>(H.nd, ar_, s_, force);
+    // Jacobi-type solver on a DIA ND: ND D stored beside ND, so that the
+    // first sweep from zero, ND (D x), is the plain product (ND D) x (one
+    // gather per entry instead of two; the product's rounding is
+    // re-associated, (a_ij d_j) x_j for a_ij (d_j x_j))
+    if (H.nnz_nd > 0 && S.nd.fmt == FMT_DIA && S.dv && (H.solver == DIAG_SCALE || H.solver == L1_DIAG)) {
+      S.ndd = S.nd;
+      S.ndd.val = ar_.alloc<V>((size_t)S.nd.nr * S.nd.w);
+      dia_colscale_kernel<V><<<nblk(S.nd.nr), BS, 0, s_>>>(S.nd.nr, S.nd.nc, S.nd.w, S.nd.off, S.nd.val, S.dv,
+                                                           S.ndd.val);
+      HIPCHECK(hipGetLastError());
+    }
     if (H.solver == ILU_N || H.solver == GS || H.solver == BWGS) {
       S.l = upload_mat<V>(H.l, ar_, s_, 1, 1);  // CSR (rows summed in stored order)
       S.u = upload_mat<V>(H.u, ar_, s_, 1, 1);
@@ -675,13 +711,16 @@ class DevPrecT : public DevPrec {
   }
 
   // One Jacobi sweep y_out = D^-1 (b - ND y_in) (FIRST: y_in = D^-1 b).
-  // FIRST on an ELL or DIA operator gathers d_j*x_j directly; on SELL/CSR,
+  // FIRST on a DIA operator ('N') is the product with ND D; FIRST on an ELL
+  // operator (or DIA for 'T') gathers d_j*x_j directly; on SELL/CSR,
   // whose gathers are the costly part, d.*x is formed first in one streaming
   // pass (same products, so the same result) and gathered once.
   template <bool FIRST, int K, class E>
   void jac_launch(DLevel<V>& D, DSmoother<V>& S, const V* b, const V* yin, E e, char trans, Red red) {
     const DMat<V>& nd = (trans != 'N') ? S.ndt : S.nd;
-    if (FIRST && (nd.fmt == FMT_ELL || nd.fmt == FMT_DIA)) {
+    if (FIRST && trans == 'N' && S.ndd.val) {
+      spmv_launch<K>(S.ndd, GatherX<V, V>{b}, e, s_, red);
+    } else if (FIRST && (nd.fmt == FMT_ELL || nd.fmt == FMT_DIA)) {
       spmv_launch<K>(nd, GatherDX<V, V>{S.dv, b}, e, s_, red);
     } else if (FIRST) {
       vmul_kernel<V, V, false><<<nblk(D.n), BS, 0, s_>>>(D.n, S.dv, b, D.t2);
@@ -695,16 +734,20 @@ class DevPrecT : public DevPrec {
     jac_launch<FIRST, 0>(D, S, b, yin, EpiJac<V, V, V, ADD>{S.dv, b, yout}, trans, Red{nullptr, nullptr});
   }
   // Last post-smoothing sweep of the fine level (beta = 1): writes the
-  // FP64 output of the cycle and the requested dot in the same pass.
+  // FP64 output of the cycle (or, keeping the FP32 result, y2l itself) and
+  // the requested dot in the same pass.
   template <bool FIRST>
   void jac_sweep_out(DLevel<V>& D, DSmoother<V>& S, const V* b, const V* yin, V* y2l, char trans) {
-    EpiJacOut<V, V> e{S.dv, b, y2l, out64_, dot_u_};
-    if (dot_u_) jac_launch<FIRST, 1>(D, S, b, yin, e, trans, red(dot_slot_));
-    else jac_launch<FIRST, 0>(D, S, b, yin, e, trans, Red{nullptr, nullptr});
+    auto run = [&](auto e) {
+      if (dot_u_) jac_launch<FIRST, 1>(D, S, b, yin, e, trans, red(dot_slot_));
+      else jac_launch<FIRST, 0>(D, S, b, yin, e, trans, Red{nullptr, nullptr});
+    };
+    if (keep32_) run(EpiJacOut<V, V, false, V>{S.dv, b, y2l, y2l, dot_u_});
+    else run(EpiJacOut<V, V>{S.dv, b, y2l, out64_, dot_u_});
     out_done_ = true;
   }
   bool fuse_out(DLevel<V>& D, int beta, V* y) const {
-    return out64_ != nullptr && &D == &L_[0] && beta == 1 && y == D.y2l;
+    return (out64_ != nullptr || keep32_) && &D == &L_[0] && beta == 1 && y == D.y2l;
   }
 
   // mld_d_jac_smoother_apply_vect (init 'Z'), alpha = 1, beta in {0,1}.
@@ -720,11 +763,14 @@ class DevPrecT : public DevPrec {
     const bool jac = (S.solver == DIAG_SCALE || S.solver == L1_DIAG);
     const bool fo = jac && fuse_out(D, beta, y);
     if (sweeps <= 1 || S.nnz_nd == 0) {
-      if (fo) {  // y = y + d*x, written as the FP64 output
+      if (fo) {  // y = y + d*x, written as the FP64 output (or in place)
         const int nb = nblk(n);
         if (dot_u_) {
-          vmul_out_kernel<V, 1><<<nb, BS, 0, s_>>>(n, S.dv, x, y, out64_, dot_u_, red(dot_slot_));
+          if (keep32_) vmul_out_kernel<V, 1, V><<<nb, BS, 0, s_>>>(n, S.dv, x, y, y, dot_u_, red(dot_slot_));
+          else vmul_out_kernel<V, 1><<<nb, BS, 0, s_>>>(n, S.dv, x, y, out64_, dot_u_, red(dot_slot_));
           finalize_kernel<1><<<1, FIN_T, 0, s_>>>(nb, partial_, scal_ + dot_slot_);
+        } else if (keep32_) {
+          vmul_out_kernel<V, 0, V><<<nb, BS, 0, s_>>>(n, S.dv, x, y, y, nullptr, Red{nullptr, nullptr});
         } else {
           vmul_out_kernel<V, 0><<<nb, BS, 0, s_>>>(n, S.dv, x, y, out64_, nullptr, Red{nullptr, nullptr});
         }
@@ -889,8 +935,8 @@ class DevPrecT : public DevPrec {
   // dot products of level vectors (one or two pairs) into scalar slots
   void ldot(int n, const V* a, const V* b, const V* c, const V* d, int slot) {
     const int nb = red_blocks(n);
-    if (c) dot_kernel<V, 2><<<nb, BS, 0, s_>>>(n, a, b, c, d, red(slot));
-    else dot_kernel<V, 1><<<nb, BS, 0, s_>>>(n, a, b, nullptr, nullptr, red(slot));
+    if (c) dot_kernel<V, 2, V><<<nb, BS, 0, s_>>>(n, a, b, c, d, red(slot));
+    else dot_kernel<V, 1, V><<<nb, BS, 0, s_>>>(n, a, b, nullptr, nullptr, red(slot));
     finalize(c ? 2 : 1, slot, nb);
   }
 
@@ -972,7 +1018,7 @@ class DevPrecT : public DevPrec {
     return hscal_[slot];
   }
   void dot1(const double* a, const double* b, int slot) {
-    dot_kernel<double, 1><<<red_blocks(n0_), BS, 0, s_>>>(n0_, a, b, nullptr, nullptr, red(slot));
+    dot_kernel<double, 1, double><<<red_blocks(n0_), BS, 0, s_>>>(n0_, a, b, nullptr, nullptr, red(slot));
     finalize(1, slot);
     HIPCHECK(hipGetLastError());
   }
@@ -1038,9 +1084,17 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
   if (method == 0) {  // CG
     while (it < itmax) {
       ++it;
-      int rc = apply_impl(r, z, trans, r, (it & 1) ? S_RHO1 : S_RHO0);  // z = M^-1 r, rho = r.z
-      if (rc) return rc;
-      cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, z, p, scal_, it);
+      const int rs = (it & 1) ? S_RHO1 : S_RHO0;
+      if (r32) {  // z = M^-1 r kept in FP32, rho = r.z
+        const V* z32 = nullptr;
+        int rc = apply_keep32(r, r, rs, &z32);
+        if (rc) return rc;
+        cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, z32, p, scal_, it);
+      } else {
+        int rc = apply_impl(r, z, trans, r, rs);  // z = M^-1 r, rho = r.z
+        if (rc) return rc;
+        cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, (const double*)z, p, scal_, it);
+      }
       kspmv<1>(trans, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, red(S_SIGMA));
       xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, (it & 1) ? S_RHO1 : S_RHO0, S_SIGMA, red(S_RR), r32);
       preconv_ = r32 ? r : nullptr;
@@ -1055,10 +1109,18 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
   } else if (method == 1) {  // FCG(1)
     while (it < itmax) {
       ++it;
-      // z = M^-1 r and, from the second iteration, gamma = q.z in the same pass
-      int rc = apply_impl(r, z, trans, it > 1 ? q : nullptr, S_GAMMA);
-      if (rc) return rc;
-      fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z, p, r, scal_, it, red(S_PR));
+      // z = M^-1 r and, from the second iteration, gamma = q.z in the same
+      // pass (an FP32 preconditioner's z stays in FP32)
+      if (r32) {
+        const V* z32 = nullptr;
+        int rc = apply_keep32(r, it > 1 ? q : nullptr, S_GAMMA, &z32);
+        if (rc) return rc;
+        fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z32, p, r, scal_, it, red(S_PR));
+      } else {
+        int rc = apply_impl(r, z, trans, it > 1 ? q : nullptr, S_GAMMA);
+        if (rc) return rc;
+        fcg_pupdate<<<nbd, BS, 0, s_>>>(n, (const double*)z, p, r, scal_, it, red(S_PR));
+      }
       finalize(1, S_PR);
       kspmv<1>(trans, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, red(S_DELTA));
       xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, S_PR, S_DELTA, red(S_RR), r32);

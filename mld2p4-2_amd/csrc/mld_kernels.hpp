@@ -126,19 +126,20 @@ struct EpiJac {  // t = d*(b - ND y);  y = t  or  y = y + t
   bool aligned2() const { return al2(d, sizeof(V)) && al2(b, sizeof(BT)) && al2(y, sizeof(YT)); }
 };
 // Last post-smoothing sweep of the fine level, writing the cycle's output:
-// yn = yin + d*(b - ND y);  yout = yn (FP64);  dot u.yout when u != nullptr.
-// yin and yout may be the same array (FP64 cycle).
-template <typename V, typename BT, bool NT = false>
+// yn = yin + d*(b - ND y);  yout = yn (FP64, or in the cycle's own precision
+// when the caller keeps the FP32 result);  dot u.yn when u != nullptr.
+// yin and yout may be the same array.
+template <typename V, typename BT, bool NT = false, typename YO = double>
 struct EpiJacOut {
   const V* d;
   const BT* b;
   const V* yin;
-  double* yout;
+  YO* yout;  // FP64 output, or (YO = V) yin itself
   const double* u;
   __device__ __forceinline__ void operator()(int i, V acc, double* dd) const {
     const V t = d[i] * ((V)b[i] - acc);
     const V yn = yin[i] + t;
-    stnt<NT>(yout + i, (double)yn);
+    stnt<NT>(yout + i, (YO)yn);
     if (u) dd[0] = u[i] * (double)yn;
   }
   // two rows; their dot terms are added to dd
@@ -146,7 +147,7 @@ struct EpiJacOut {
     const vec2_t<V> dv = *(const vec2_t<V>*)(d + i), yv = *(const vec2_t<V>*)(yin + i);
     const vec2_t<BT> bb = *(const vec2_t<BT>*)(b + i);
     const V y0 = yv.x + dv.x * ((V)bb.x - a0), y1 = yv.y + dv.y * ((V)bb.y - a1);
-    stnt<NT>((vec2_t<double>*)(yout + i), vec2_t<double>{(double)y0, (double)y1});
+    stnt<NT>((vec2_t<YO>*)(yout + i), vec2_t<YO>{(YO)y0, (YO)y1});
     if (u) {
       const vec2_t<double> uu = *(const vec2_t<double>*)(u + i);
       dd[0] += uu.x * (double)y0;
@@ -154,7 +155,7 @@ struct EpiJacOut {
     }
   }
   bool aligned2() const {
-    return al2(d, sizeof(V)) && al2(b, sizeof(BT)) && al2(yin, sizeof(V)) && al2(yout, sizeof(double)) &&
+    return al2(d, sizeof(V)) && al2(b, sizeof(BT)) && al2(yin, sizeof(V)) && al2(yout, sizeof(YO)) &&
            (!u || al2(u, sizeof(double)));
   }
 };
@@ -213,10 +214,10 @@ struct NtForm<EpiJac<V, BT, YT, ADD, false>> {
   using type = EpiJac<V, BT, YT, ADD, true>;
   static type make(const EpiJac<V, BT, YT, ADD, false>& e) { return type{e.d, e.b, e.y}; }
 };
-template <typename V, typename BT>
-struct NtForm<EpiJacOut<V, BT, false>> {
-  using type = EpiJacOut<V, BT, true>;
-  static type make(const EpiJacOut<V, BT, false>& e) { return type{e.d, e.b, e.yin, e.yout, e.u}; }
+template <typename V, typename BT, typename YO>
+struct NtForm<EpiJacOut<V, BT, false, YO>> {
+  using type = EpiJacOut<V, BT, true, YO>;
+  static type make(const EpiJacOut<V, BT, false, YO>& e) { return type{e.d, e.b, e.yin, e.yout, e.u}; }
 };
 template <typename V>
 struct NtForm<EpiSetDot<V, false>> {
@@ -513,15 +514,15 @@ __global__ __launch_bounds__(BS) void vmul_kernel(int n, const V* __restrict__ d
 
 // One-sweep post-smoothing of the fine level writing the cycle's output:
 // yout = yin + d*x (FP64), with the dot u.yout when u != nullptr.
-template <typename V, int K>
+template <typename V, int K, typename YO = double>
 __global__ __launch_bounds__(BS) void vmul_out_kernel(int n, const V* __restrict__ d, const V* __restrict__ x,
-                                                       const V* yin, double* yout, const double* __restrict__ u,
+                                                       const V* yin, YO* yout, const double* __restrict__ u,
                                                        Red red) {
   const int i = blockIdx.x * BS + threadIdx.x;
   double dd[2] = {0.0, 0.0};
   if (i < n) {
     const V yn = yin[i] + d[i] * x[i];
-    yout[i] = (double)yn;
+    yout[i] = (YO)yn;
     if (K > 0) dd[0] = u[i] * (double)yn;
   }
   if constexpr (K > 0) block_partials<K>(dd, red.partial);
@@ -562,10 +563,10 @@ __global__ __launch_bounds__(BS) void axpby_kernel(int n, V a, const V* __restri
 constexpr int RED_EPT = 2;
 __host__ __device__ constexpr int red_blocks(int n) { return (n + BS * RED_EPT - 1) / (BS * RED_EPT); }
 
-// dot products: K pairs (x_q, y_q) in one pass
-template <typename V, int K>
-__global__ __launch_bounds__(BS) void dot_kernel(int n, const V* __restrict__ x0, const V* __restrict__ y0,
-                                                  const V* __restrict__ x1, const V* __restrict__ y1, Red red) {
+// dot products: K pairs (x_q, y_q) in one pass, the y operands of type W
+template <typename V, int K, typename W = V>
+__global__ __launch_bounds__(BS) void dot_kernel(int n, const V* __restrict__ x0, const W* __restrict__ y0,
+                                                  const V* __restrict__ x1, const W* __restrict__ y1, Red red) {
   double s[2] = {0.0, 0.0};
   const int b = blockIdx.x * BS * RED_EPT + threadIdx.x;
 #pragma unroll
@@ -584,17 +585,19 @@ __global__ __launch_bounds__(BS) void dot_kernel(int n, const V* __restrict__ x0
 enum { S_RHO0 = 0, S_RHO1 = 1, S_SIGMA = 2, S_RR = 3, S_PR = 4, S_GAMMA = 5, S_DELTA = 6, S_TMP0 = 8,
        S_TMP1 = 9, S_OMEGA = 10, S_ALPHA = 11, S_K0 = 12, S_K1 = 13, S_NSCAL = 16 };
 
-// CG: p = z (first) or p = z + (rho_new/rho_old) p
-__global__ __launch_bounds__(BS) void cg_pupdate(int n, const double* __restrict__ z, double* __restrict__ p,
+// CG: p = z (first) or p = z + (rho_new/rho_old) p; z in FP64 or the FP32
+// output of an FP32 preconditioner
+template <typename ZT>
+__global__ __launch_bounds__(BS) void cg_pupdate(int n, const ZT* __restrict__ z, double* __restrict__ p,
                                                   const double* __restrict__ scal, int it) {
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
   if (it == 1) {
-    p[i] = z[i];
+    p[i] = (double)z[i];
   } else {
     const double rn = scal[(it & 1) ? S_RHO1 : S_RHO0], ro = scal[(it & 1) ? S_RHO0 : S_RHO1];
     const double beta = rn / ro;
-    p[i] = z[i] + beta * p[i];
+    p[i] = (double)z[i] + beta * p[i];
   }
 }
 
@@ -624,8 +627,10 @@ __global__ __launch_bounds__(BS) void xr_update(int n, double* __restrict__ x, d
   block_partials<1>(s, red.partial);
 }
 
-// FCG(1): p = z - (gamma/delta) p, and sum_i p_i r_i for the step length.
-__global__ __launch_bounds__(BS) void fcg_pupdate(int n, const double* __restrict__ z, double* __restrict__ p,
+// FCG(1): p = z - (gamma/delta) p, and sum_i p_i r_i for the step length
+// (z as in cg_pupdate).
+template <typename ZT>
+__global__ __launch_bounds__(BS) void fcg_pupdate(int n, const ZT* __restrict__ z, double* __restrict__ p,
                                                    const double* __restrict__ r, const double* __restrict__ scal,
                                                    int it, Red red) {
   const double c = (it == 1) ? 0.0 : -(scal[S_GAMMA] / scal[S_DELTA]);
@@ -635,7 +640,7 @@ __global__ __launch_bounds__(BS) void fcg_pupdate(int n, const double* __restric
   for (int e = 0; e < RED_EPT; ++e) {
     const int i = b + e * BS;
     if (i < n) {
-      const double pi = (it == 1) ? z[i] : z[i] + c * p[i];
+      const double pi = (it == 1) ? (double)z[i] : (double)z[i] + c * p[i];
       p[i] = pi;
       s[0] += pi * r[i];
     }
@@ -892,6 +897,20 @@ __global__ __launch_bounds__(BS) void dia_fill_kernel(int nr, int w, DiaOff off,
     int d = 0;
     while (d < w - 1 && off.o[d] != o) ++d;
     oval[(int64_t)d * nr + i] = (V)val[k];
+  }
+}
+
+// DIA operator with its columns scaled: oval[d*nr + i] = val[d*nr + i] * c[j],
+// j = i + off[d] clamped as in the products (padding stays zero)
+template <typename V>
+__global__ __launch_bounds__(BS) void dia_colscale_kernel(int nr, int nc, int w, DiaOff off,
+                                                          const V* __restrict__ val, const V* __restrict__ c,
+                                                          V* __restrict__ oval) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= nr) return;
+  for (int d = 0; d < w; ++d) {
+    const int j = min(max(i + off.o[d], 0), nc - 1);
+    oval[(int64_t)d * nr + i] = val[(int64_t)d * nr + i] * c[j];
   }
 }
 

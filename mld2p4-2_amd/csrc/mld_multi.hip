@@ -248,13 +248,6 @@ __global__ __launch_bounds__(BS) void bdia_kernel(int nr, int nc, int nd, DiaOff
 }
 
 // y = d_row .* x  (ADD: y += d_row .* x) over n rows of 32 columns
-template <class E>
-struct is_bjac : std::false_type {};
-template <typename V, bool ADD>
-struct is_bjac<BJac<V, ADD>> : std::true_type {};
-template <typename V>
-struct is_bjac<BJacOut<V>> : std::true_type {};
-
 // Chunk (in blocks) of the row-block order of a block product (see
 // xcd_block).  A row of a 32-column multi-vector is 128 (FP32) or 256 (FP64)
 // bytes, so an XCD's 4 MB L2 holds only 16-32K rows of X.  xmap 1 on a DIA
@@ -809,6 +802,7 @@ class MultiT : public MultiEngine {
   struct MLevel {
     int n = 0, ni = 0;
     DMat<V> a, nd, nd2, p, r, mc;
+    DMat<V> ndd, ndd2;  // DIA ND (ND2) with columns scaled by dv (dv2): first sweep from zero
     V* dv = nullptr;
     V* dv2 = nullptr;
     int64_t nnz_nd = 0, nnz_nd2 = 0;
@@ -855,10 +849,12 @@ class MultiT : public MultiEngine {
         D.nnz_nd = H.nd.nnz();
         if (D.nnz_nd > 0) D.nd = up(H.nd);
         D.dv = upv(H.dv);
+        if (D.nnz_nd > 0 && D.nd.fmt == FMT_DIA) D.ndd = scaled_cols(D.nd, D.dv);
         if (D.has2) {
           D.nnz_nd2 = H.nd2.nnz();
           if (D.nnz_nd2 > 0) D.nd2 = up(H.nd2);
           D.dv2 = upv(H.dv2);
+          if (D.nnz_nd2 > 0 && D.nd2.fmt == FMT_DIA) D.ndd2 = scaled_cols(D.nd2, D.dv2);
         }
       }
       if (l > 0) {
@@ -1076,6 +1072,17 @@ class MultiT : public MultiEngine {
       return m;
     }
   }
+  // ND D for a DIA ND: the first Jacobi sweep from zero, ND (D x), becomes
+  // the plain product (ND D) x (one gather per entry; the rounding of each
+  // product is re-associated, (a_ij d_j) x_j for a_ij (d_j x_j))
+  DMat<V> scaled_cols(const DMat<V>& m, const V* d) {
+    DMat<V> r = m;
+    r.val = ar_.alloc<V>((size_t)m.nr * m.w);
+    dia_colscale_kernel<V><<<nblk(m.nr), BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, d, r.val);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s_));
+    return r;
+  }
   V* upv(const std::vector<double>& v) {
     std::vector<V> t(v.begin(), v.end());
     V* d = ar_.upload(t.data(), t.size(), s_);
@@ -1107,18 +1114,12 @@ class MultiT : public MultiEngine {
     if (m.nr == 0) return;
     const int lpr = MB / rpl, rpb = BS / lpr, nb = (m.nr + rpb - 1) / rpb;
     const int xc = bxchunk(m, rpb, nb);
+    // the first Jacobi sweep from zero uses ND D (no d .* x gathers here)
+    if (dx) throw DevError("bspmm: d .* x gathers are not used by block products");
     if (m.fmt == FMT_DIA) {
-      if constexpr (is_bjac<E>::value) {
-        if (dx) {
-          bdia_launch<T, true>(m, x, e, dx, rpl, nb, xc);
-          return;
-        }
-      }
-      if (dx) throw DevError("bspmm: d .* x gathers only in Jacobi sweeps");
       bdia_launch<T, false>(m, x, e, dx, rpl, nb, xc);
       return;
     }
-    if (dx) throw DevError("bspmm: d .* x gathers need DIA storage");
     switch (rpl) {
       case 4:
         bspmm_kernel<T, 4, E><<<nb, BS, 0, s_>>>(m.nr, m.irp, m.ja, m.val, x, e, xc);
@@ -1243,6 +1244,7 @@ class MultiT : public MultiEngine {
   void smooth(MLevel& D, bool post, const V* x, int beta, V* y, int sweeps) {
     const bool two = post && D.has2;
     const DMat<V>& nd = two ? D.nd2 : D.nd;
+    const DMat<V>& ndd = two ? D.ndd2 : D.ndd;
     const V* dv = two ? D.dv2 : D.dv;
     const int64_t nnz = two ? D.nnz_nd2 : D.nnz_nd;
     const int n = D.ni;
@@ -1253,30 +1255,30 @@ class MultiT : public MultiEngine {
     }
     // the cycle's FP64 output straight from the fine level's last sweep
     const bool fo = out64_ != nullptr && beta == 1 && &D == &L_[0] && y == D.y2l;
-    // first sweep from zero: gather d_j x_j where the gathers are cheap (DIA,
-    // and ELL for one column), else form d .* x first
-    const bool dxg = nd.fmt == FMT_DIA || (NB == 1 && nd.fmt == FMT_ELL);
+    // first sweep from zero: the product with ND D on DIA, a gather of
+    // d_j x_j where it is cheap (ELL for one column), else d .* x formed first
+    const DMat<V>& nd1 = ndd.val ? ndd : nd;
+    const bool dxg = !ndd.val && NB == 1 && nd.fmt == FMT_ELL;
     const V* xin = x;
-    const V* dxv = dv;
-    if (!dxg) {
+    const V* dxv = dxg ? dv : nullptr;
+    if (!ndd.val && !dxg) {
       vmul<false>(n, dv, x, D.t2);
       xin = D.t2;
-      dxv = nullptr;
     }
     if (sweeps == 2) {
       if (fo) {
-        op_jac_out(nd, xin, dv, x, y, out64_, dxv);
+        op_jac_out(nd1, xin, dv, x, y, out64_, dxv);
         out_done_ = true;
       } else if (beta) {
-        op_jac<true>(nd, xin, dv, x, y, dxv);
+        op_jac<true>(nd1, xin, dv, x, y, dxv);
       } else {
-        op_jac<false>(nd, xin, dv, x, y, dxv);
+        op_jac<false>(nd1, xin, dv, x, y, dxv);
       }
       return;
     }
     V* ta = D.tx;
     V* tb = D.t1;
-    op_jac<false>(nd, xin, dv, x, ta, dxv);
+    op_jac<false>(nd1, xin, dv, x, ta, dxv);
     for (int i = 2; i <= sweeps - 2; ++i) {
       op_jac<false>(nd, ta, dv, x, tb);
       std::swap(ta, tb);
