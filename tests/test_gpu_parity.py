@@ -292,6 +292,26 @@ def test_kcycle_mixed_and_krylov(eng, orc):
         assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-8
 
 
+@pytest.mark.parametrize("method,smoother,sweeps", [("CG", "L1-JACOBI", 2), ("FCG", "L1-JACOBI", 2),
+                                                   ("FCG", "JACOBI", 1), ("CG", "BJAC", 1), ("FCG", "BJAC", 1)])
+def test_mixed_krylov_fp32_output(eng, orc, method, smoother, sweeps):
+    """FP32-mixed CG/FCG keep the preconditioner output in FP32 (fused into
+    the last Jacobi sweep with its dot, or left by the ILU / single-sweep
+    paths and dotted separately): the oracle's FP64 iteration count (+-1) and
+    the true residual."""
+    _say(f"[mixed krylov] {method} {smoother} sweeps={sweeps} idim=28")
+    a = orc.gen_pde3d(28)
+    A = a.to_scipy()
+    po = oracle_prec(orc, a, smoother, sweeps)
+    b = np.ones(A.shape[0])
+    _, ito, _ = po.krylov(method, b, eps=1e-8)
+    p = engine_prec(eng, A, smoother, sweeps, precision="mixed")
+    x, it, err = eng.psb_krylov(method, A, p, dev(b), eps=1e-8)
+    assert err <= 1e-8 and abs(it - ito) <= 1, (it, ito)
+    r = b - A @ x.cpu().numpy()
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-8
+
+
 @pytest.mark.parametrize("method", ["CG", "FCG", "BICGSTAB"])
 def test_krylov_config1(eng, orc, method):
     _say(f"[krylov] {method} idim=40")
