@@ -140,6 +140,13 @@ class DevPrecT : public DevPrec {
   int apply_keep32(const double* x, const double* dot_u, int dot_slot, const V** z) {
     HIPCHECK(hipSetDevice(device_));
     if (std::is_same<V, double>::value || L_.size() < 2 || outer_sweeps_ != 1) return E_UNSUPPORTED;
+    struct Reset {  // also when a HIP error throws out of the cycle
+      DevPrecT& p;
+      ~Reset() {
+        p.keep32_ = false;
+        p.dot_u_ = nullptr;
+      }
+    } reset{*this};
     keep32_ = true;
     out64_ = nullptr;
     dot_u_ = dot_u;
