@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <iterator>
 
 #include "mld_host.hpp"
 
@@ -28,6 +29,93 @@ double wall_seconds() { return omp_get_wtime(); }
 void phase_log(const char* what, int level, double seconds) {
   static const bool on = std::getenv("MLD_VERBOSE") != nullptr;
   if (on) std::fprintf(stderr, "[mld] level %d %-22s %9.3f ms\n", level, what, seconds * 1e3);
+}
+
+std::vector<int> host_dia_offsets(const HCsr& a) {
+  // each thread collects the offsets of a contiguous row range (at most 9 kept:
+  // a ninth means "more than 8"), then the sets are merged
+  const int nt = std::max(1, std::min(omp_get_max_threads(), a.nr / 65536 + 1));
+  std::vector<std::vector<int>> part(nt);
+#pragma omp parallel num_threads(nt)
+  {
+    const int t = omp_get_thread_num();
+    const int r0 = (int)((int64_t)a.nr * t / nt), r1 = (int)((int64_t)a.nr * (t + 1) / nt);
+    std::vector<int>& offs = part[t];
+    for (int i = r0; i < r1 && offs.size() <= 8; ++i) {
+      size_t p = 0;
+      for (int64_t k = a.irp[i]; k < a.irp[i + 1]; ++k) {
+        const int o = a.ja[k] - i;
+        while (p < offs.size() && offs[p] < o) ++p;
+        if (p == offs.size() || offs[p] != o) {
+          offs.insert(offs.begin() + p, o);
+          if (offs.size() > 8) break;
+        }
+        ++p;
+      }
+    }
+  }
+  std::vector<int> all;
+  for (const auto& p : part) {
+    std::vector<int> u;
+    std::set_union(all.begin(), all.end(), p.begin(), p.end(), std::back_inserter(u));
+    all.swap(u);
+    if (all.size() > 8) return {};
+  }
+  return all;
+}
+
+bool host_sell_layout(const HCsr& a, int sigma, int c, int maxrow, std::vector<int>& perm, std::vector<int>& sptr) {
+  const int nr = a.nr;
+  const int nsl = (nr + c - 1) / c;
+  perm.resize(nr);
+  auto len = [&](int i) { return (int)(a.irp[i + 1] - a.irp[i]); };
+  if (sigma <= 1) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nr; ++i) perm[i] = i;
+  } else {
+    // stable counting sort by decreasing length inside each window
+    const int nw = (nr + sigma - 1) / sigma;
+#pragma omp parallel
+    {
+      std::vector<int> cnt((size_t)maxrow + 2);
+#pragma omp for schedule(static)
+      for (int w = 0; w < nw; ++w) {
+        const int w0 = w * sigma, w1 = std::min(nr, w0 + sigma);
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int i = w0; i < w1; ++i) cnt[maxrow - len(i) + 1]++;
+        for (size_t b = 1; b < cnt.size(); ++b) cnt[b] += cnt[b - 1];
+        for (int i = w0; i < w1; ++i) perm[w0 + cnt[maxrow - len(i)]++] = i;
+      }
+    }
+  }
+  std::vector<int64_t> width(nsl);
+#pragma omp parallel for schedule(static)
+  for (int sl = 0; sl < nsl; ++sl) {
+    int wmax = 0;
+    for (int l = 0; l < c; ++l) {
+      const int p = sl * c + l;
+      if (p < nr) wmax = std::max(wmax, len(perm[p]));
+    }
+    width[sl] = (int64_t)wmax * c;
+  }
+  sptr.assign((size_t)nsl + 1, 0);
+  int64_t acc = 0;
+  for (int sl = 0; sl < nsl; ++sl) {
+    acc += width[sl];
+    if (acc >= ((int64_t)1 << 31)) return false;
+    sptr[sl + 1] = (int)acc;
+  }
+  return true;
+}
+
+void par_memcpy(void* dst, const void* src, size_t n) {
+  constexpr size_t PIECE = (size_t)1 << 20;
+  const int64_t np = (int64_t)((n + PIECE - 1) / PIECE);
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < np; ++k) {
+    const size_t o = (size_t)k * PIECE;
+    std::memcpy((char*)dst + o, (const char*)src + o, std::min(PIECE, n - o));
+  }
 }
 
 static std::vector<double> diag_of(const HCsr& a) {

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -88,6 +89,46 @@ struct DLevel {
 // ---------------------------------------------------------------------------
 // Allocation bookkeeping: every device allocation of a handle is freed with it.
 // ---------------------------------------------------------------------------
+// Host-to-device copy of pageable host memory through two pinned staging
+// buffers per device: chunk k+1 is copied into one buffer by the OpenMP
+// threads while the DMA of chunk k from the other buffer is in flight (a
+// direct copy from pageable memory is staged by the runtime at a fraction of
+// the link rate).  A buffer is refilled only after the event recorded behind
+// its previous copy has completed; the caller's host memory may be released
+// when the function returns.  Copies under 4 MB go directly.
+struct H2DStage {
+  static constexpr size_t CH = (size_t)32 << 20;
+  char* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+static void h2d_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes < ((size_t)4 << 20)) {
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return;
+  }
+  static std::mutex mu;
+  static std::vector<H2DStage> stages;  // per device, kept until the process ends
+  std::lock_guard<std::mutex> lock(mu);
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if ((int)stages.size() <= dev) stages.resize(dev + 1);
+  H2DStage& st = stages[dev];
+  if (!st.buf[0]) {
+    for (int b = 0; b < 2; ++b) {
+      HIPCHECK(hipHostMalloc((void**)&st.buf[b], H2DStage::CH, hipHostMallocDefault));
+      HIPCHECK(hipEventCreateWithFlags(&st.ev[b], hipEventDisableTiming));
+    }
+  }
+  int b = 0;
+  for (size_t off = 0; off < bytes; off += H2DStage::CH, b ^= 1) {
+    const size_t len = std::min(H2DStage::CH, bytes - off);
+    HIPCHECK(hipEventSynchronize(st.ev[b]));
+    par_memcpy(st.buf[b], (const char*)src + off, len);
+    HIPCHECK(hipMemcpyAsync((char*)dst + off, st.buf[b], len, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipEventRecord(st.ev[b], s));
+  }
+}
+
 struct DevArena {
   std::vector<void*> ptrs;
   size_t bytes = 0;
@@ -103,7 +144,7 @@ struct DevArena {
   template <typename T>
   T* upload(const T* h, size_t n, hipStream_t s) {
     T* d = alloc<T>(n);
-    if (n) HIPCHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    if (n) h2d_copy(d, h, n * sizeof(T), s);
     return d;
   }
   ~DevArena() {
@@ -136,22 +177,7 @@ struct FmtChoice {
 // empty.
 // Columns ascend within a row, so the offsets of a row ascend too: one merge
 // walk per row against the sorted set found so far.
-static std::vector<int> dia_offsets(const HCsr& h) {
-  std::vector<int> offs;
-  for (int i = 0; i < h.nr; ++i) {
-    size_t p = 0;
-    for (int64_t k = h.irp[i]; k < h.irp[i + 1]; ++k) {
-      const int o = h.ja[k] - i;
-      while (p < offs.size() && offs[p] < o) ++p;
-      if (p == offs.size() || offs[p] != o) {
-        offs.insert(offs.begin() + p, o);
-        if (offs.size() > 8) return {};
-      }
-      ++p;
-    }
-  }
-  return offs;
-}
+static std::vector<int> dia_offsets(const HCsr& h) { return host_dia_offsets(h); }
 
 // DIA pays when the padded diagonals hold at most 10 % more entries than A.
 static bool dia_eligible(const HCsr& h, const std::vector<int>& offs) {
@@ -206,8 +232,7 @@ static DevSrc make_src(const HCsr& h, DevArena& tmp, hipStream_t s) {
   if (d.nnz >= ((int64_t)1 << 31)) throw DevError("operator exceeds 2^31 entries");
   for (int i = 0; i < h.nr; ++i) d.maxrow = std::max(d.maxrow, (int)(h.irp[i + 1] - h.irp[i]));
   int64_t* irp64 = tmp.alloc<int64_t>((size_t)h.nr + 1);
-  if (!h.irp.empty()) HIPCHECK(hipMemcpyAsync(irp64, h.irp.data(), ((size_t)h.nr + 1) * sizeof(int64_t),
-                                              hipMemcpyHostToDevice, s));
+  if (!h.irp.empty()) h2d_copy(irp64, h.irp.data(), ((size_t)h.nr + 1) * sizeof(int64_t), s);
   else HIPCHECK(hipMemsetAsync(irp64, 0, ((size_t)h.nr + 1) * sizeof(int64_t), s));
   d.irp = tmp.alloc<int>((size_t)h.nr + 1);
   irp_to32_kernel<<<nblk(h.nr + 1), BS, 0, s>>>(h.nr + 1, irp64, d.irp);
@@ -241,33 +266,9 @@ static DMat<V> build_mat(const HCsr& h, const DevSrc& src, DevArena& ar, hipStre
   } else if (c.fmt == FMT_SELL) {
     const int nr = h.nr;
     const int nsl = (nr + SELL_C - 1) / SELL_C;
-    std::vector<int> perm(nr);
-    auto len = [&](int i) { return (int)(h.irp[i + 1] - h.irp[i]); };
-    const int sigma = sell_sigma(force);
-    if (sigma <= 1) {
-      for (int i = 0; i < nr; ++i) perm[i] = i;
-    } else {
-      // stable counting sort by decreasing length inside each window
-      std::vector<int> cnt((size_t)src.maxrow + 2);
-      for (int w0 = 0; w0 < nr; w0 += sigma) {
-        const int w1 = std::min(nr, w0 + sigma);
-        std::fill(cnt.begin(), cnt.end(), 0);
-        for (int i = w0; i < w1; ++i) cnt[src.maxrow - len(i) + 1]++;
-        for (size_t b = 1; b < cnt.size(); ++b) cnt[b] += cnt[b - 1];
-        for (int i = w0; i < w1; ++i) perm[w0 + cnt[src.maxrow - len(i)]++] = i;
-      }
-    }
-    std::vector<int> sptr(nsl + 1, 0);
-    for (int sl = 0; sl < nsl; ++sl) {
-      int wmax = 0;
-      for (int l = 0; l < SELL_C; ++l) {
-        const int p = sl * SELL_C + l;
-        if (p < nr) wmax = std::max(wmax, len(perm[p]));
-      }
-      const int64_t next = (int64_t)sptr[sl] + (int64_t)wmax * SELL_C;
-      if (next >= ((int64_t)1 << 31)) throw DevError("SELL storage exceeds 2^31 entries");
-      sptr[sl + 1] = (int)next;
-    }
+    std::vector<int> perm, sptr;
+    if (!host_sell_layout(h, sell_sigma(force), SELL_C, src.maxrow, perm, sptr))
+      throw DevError("SELL storage exceeds 2^31 entries");
     const size_t tot = (size_t)sptr[nsl];
     m.sptr = ar.upload(sptr.data(), sptr.size(), s);
     m.perm = ar.upload(perm.data(), perm.size(), s);

@@ -144,6 +144,14 @@ struct Prec {
 // prints its build phases at debug_level >= psb_debug_outer_).
 double wall_seconds();
 void phase_log(const char* what, int level, double seconds);
+// memcpy of n bytes split over the OpenMP threads (host staging of uploads)
+void par_memcpy(void* dst, const void* src, size_t n);
+// Distinct diagonal offsets j - i of a, ascending; empty when more than 8.
+std::vector<int> host_dia_offsets(const HCsr& a);
+// SELL row order and slice offsets: rows stably sorted by decreasing length
+// inside windows of sigma rows (sigma <= 1: identity), slices of c rows
+// padded to their longest row; false when the storage exceeds 2^31 entries.
+bool host_sell_layout(const HCsr& a, int sigma, int c, int maxrow, std::vector<int>& perm, std::vector<int>& sptr);
 
 // configuration (mld_dprecinit / mld_dprecset / mld_d_base_onelev_*set*)
 int prec_init(Prec& p, const std::string& ptype);
