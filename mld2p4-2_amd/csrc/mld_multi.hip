@@ -15,12 +15,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -528,16 +530,57 @@ static void csr_start(HCsr& u) {
   u.clear();
   u.irp.assign(1, 0);
 }
-static void append_rows(HCsr& u, const HCsr& m, int64_t co) {
-  for (int i = 0; i < m.nr; ++i) {
-    for (int64_t k = m.irp[i]; k < m.irp[i + 1]; ++k) {
-      u.ja.push_back((int)(m.ja[k] + co));
-      u.val.push_back(m.val[k]);
-    }
-    u.irp.push_back((int64_t)u.ja.size());
-  }
-  u.nr += m.nr;
+// Host threads for the union build (OMP_NUM_THREADS, default 8).
+static int host_threads() {
+  int nt = 8;
+  if (const char* s = std::getenv("OMP_NUM_THREADS")) nt = std::atoi(s);
+  return std::max(1, std::min(nt, 64));
 }
+
+// u = the rows of parts[0], parts[1], ... (each with its column shift): sizes
+// summed first, then the parts' rows copied into their own ranges by host
+// threads, in tasks of at most 2^18 rows.
+static void concat_rows(HCsr& u, const std::vector<std::pair<const HCsr*, int64_t>>& parts, int nc) {
+  const size_t np = parts.size();
+  std::vector<int64_t> r0(np + 1, 0), e0(np + 1, 0);
+  for (size_t q = 0; q < np; ++q) {
+    const HCsr& m = *parts[q].first;
+    r0[q + 1] = r0[q] + m.nr;
+    e0[q + 1] = e0[q] + (m.nr ? m.irp[m.nr] - m.irp[0] : 0);
+  }
+  u.clear();
+  u.nr = (int)r0[np];
+  u.nc = nc;
+  u.irp.resize((size_t)u.nr + 1);
+  u.ja.resize((size_t)e0[np]);
+  u.val.resize((size_t)e0[np]);
+  u.irp[0] = 0;
+  constexpr int TASK_ROWS = 1 << 18;
+  std::vector<std::pair<size_t, int>> tasks;  // (part, first row)
+  for (size_t q = 0; q < np; ++q)
+    for (int i = 0; i < parts[q].first->nr; i += TASK_ROWS) tasks.push_back({q, i});
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (size_t t = next++; t < tasks.size(); t = next++) {
+      const size_t q = tasks[t].first;
+      const HCsr& m = *parts[q].first;
+      const int64_t co = parts[q].second;
+      const int i0 = tasks[t].second, i1 = std::min(m.nr, i0 + TASK_ROWS);
+      const int64_t base = m.irp[0];
+      for (int64_t k = m.irp[i0]; k < m.irp[i1]; ++k) {
+        u.ja[(size_t)(e0[q] + k - base)] = (int)(m.ja[k] + co);
+        u.val[(size_t)(e0[q] + k - base)] = m.val[k];
+      }
+      for (int i = i0; i < i1; ++i) u.irp[(size_t)(r0[q] + i + 1)] = e0[q] + (m.irp[i + 1] - base);
+    }
+  };
+  const int nt = (int)std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, tasks.size()));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+}
+
 
 // y = S x for the coarsest-level smoother data of one system
 // (mld_d_jac_smoother_apply_vect.f90:107-109 shortcut or Jacobi sweeps,
@@ -641,23 +684,46 @@ static int append_coarse_operator(HCsr& mc, const Level& L, int64_t co, std::str
       return rc;
     }
   }
-  std::vector<double> m((size_t)n * n), e(n, 0.0), y;
-  for (int j = 0; j < n; ++j) {
-    e[j] = 1.0;
-    int rc = coarse_apply_host(sd, L.parms.sweeps_pre, lu, piv, e, y);
-    if (rc) {
-      err = "coarsest-level solver not supported by the batched engines";
-      return rc;
+  // column j = S e_j; the columns are independent, so they are formed by
+  // host threads (OMP_NUM_THREADS of them, default 8), each into its own
+  // contiguous column of mcol (column-major)
+  std::vector<double> mcol((size_t)n * n);
+  int nt = 8;
+  if (const char* s = std::getenv("OMP_NUM_THREADS")) nt = std::atoi(s);
+  nt = std::max(1, std::min({nt, 64, (n + 15) / 16}));
+  std::vector<int> rcs(nt, 0);
+  auto work = [&](int t) {
+    std::vector<double> e(n, 0.0), y;
+    for (int j = t; j < n; j += nt) {
+      e[j] = 1.0;
+      const int rc = coarse_apply_host(sd, L.parms.sweeps_pre, lu, piv, e, y);
+      e[j] = 0.0;
+      if (rc) {
+        rcs[t] = rc;
+        return;
+      }
+      std::copy(y.begin(), y.end(), mcol.begin() + (size_t)j * n);
     }
-    for (int i = 0; i < n; ++i) m[(size_t)i * n + j] = y[i];
-    e[j] = 0.0;
-  }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+  for (int t = 0; t < nt; ++t)
+    if (rcs[t]) {
+      err = "coarsest-level solver not supported by the batched engines";
+      return rcs[t];
+    }
+  const size_t o = mc.ja.size(), ro = mc.irp.size();
+  mc.ja.resize(o + (size_t)n * n);
+  mc.val.resize(o + (size_t)n * n);
+  mc.irp.resize(ro + (size_t)n);
   for (int i = 0; i < n; ++i) {
     for (int j = 0; j < n; ++j) {
-      mc.ja.push_back((int)(co + j));
-      mc.val.push_back(m[(size_t)i * n + j]);
+      mc.ja[o + (size_t)i * n + j] = (int)(co + j);
+      mc.val[o + (size_t)i * n + j] = mcol[(size_t)j * n + i];
     }
-    mc.irp.push_back((int64_t)mc.ja.size());
+    mc.irp[ro + (size_t)i] = (int64_t)(o + (size_t)(i + 1) * n);
   }
   mc.nr += n;
   return 0;
@@ -729,6 +795,7 @@ static int build_union(const std::vector<const Prec*>& S, std::vector<HULevel>& 
     csr_start(H.nd);
     csr_start(H.nd2);
     csr_start(H.mc);
+    std::vector<std::pair<const HCsr*, int64_t>> pa, pnd, pnd2;
     bool first = true;
     for (int s : present[l]) {
       const Level& L = S[s]->precv[l];
@@ -757,11 +824,11 @@ static int build_union(const std::vector<const Prec*>& S, std::vector<HULevel>& 
           err = "only V- and W-cycles in the batched engines";
           return E_UNSUPPORTED;
         }
-        append_rows(H.a, L.a, off[l][s]);
-        append_rows(H.nd, a1.nd, off[l][s]);
+        pa.push_back({&L.a, off[l][s]});
+        pnd.push_back({&a1.nd, off[l][s]});
         H.dv.insert(H.dv.end(), a1.dv.begin(), a1.dv.end());
         if (has2) {
-          append_rows(H.nd2, a2.nd, off[l][s]);
+          pnd2.push_back({&a2.nd, off[l][s]});
           H.dv2.insert(H.dv2.end(), a2.dv.begin(), a2.dv.end());
         }
       } else {  // coarsest level of this system: explicit coarse operator
@@ -769,19 +836,18 @@ static int build_union(const std::vector<const Prec*>& S, std::vector<HULevel>& 
         if (rc) return rc;
       }
     }
-    H.a.nc = H.ni;
-    H.nd.nc = H.ni;
-    H.nd2.nc = H.ni;
+    concat_rows(H.a, pa, H.ni);
+    concat_rows(H.nd, pnd, H.ni);
+    concat_rows(H.nd2, pnd2, H.ni);
     H.mc.nc = H.n - H.ni;
     if (l + 1 < maxlev) {
       HULevel& C = U[l + 1];
-      csr_start(C.p);
-      csr_start(C.r);
+      std::vector<std::pair<const HCsr*, int64_t>> pp, pr;
       for (int s : present[l])
-        if (nl[s] > l + 1) append_rows(C.p, S[s]->precv[l + 1].p, off[l + 1][s]);
-      for (int s : present[l + 1]) append_rows(C.r, S[s]->precv[l + 1].r, off[l][s]);
-      C.p.nc = C.n;
-      C.r.nc = H.ni;
+        if (nl[s] > l + 1) pp.push_back({&S[s]->precv[l + 1].p, off[l + 1][s]});
+      for (int s : present[l + 1]) pr.push_back({&S[s]->precv[l + 1].r, off[l][s]});
+      concat_rows(C.p, pp, C.n);
+      concat_rows(C.r, pr, H.ni);
       if (C.p.nr != H.ni || C.r.nr != C.n) {
         err = "internal: transfer operator sizes";
         return E_INTERNAL;
@@ -830,8 +896,11 @@ class MultiT : public MultiEngine {
 
   int build(const std::vector<const Prec*>& systems, std::string& err) {
     std::vector<HULevel> U;
+    double t0 = wall_seconds();
     int rc = build_union(systems, U, off0_, err);
     if (rc) return rc;
+    phase_log("group: union on host", -1, wall_seconds() - t0);
+    t0 = wall_seconds();
     G_ = (int)systems.size();
     const int nlev = (int)U.size();
     L_.resize(nlev);
@@ -872,15 +941,18 @@ class MultiT : public MultiEngine {
       D.t1 = ar_.alloc<V>(ni);
       D.t2 = ar_.alloc<V>(ni);
     }
+    phase_log("group: level uploads", -1, wall_seconds() - t0);
+    t0 = wall_seconds();
     n0_ = L_[0].n;
     // Krylov: FP64 union of the fine matrices (every system's level 0)
     {
       HCsr A0;
-      csr_start(A0);
-      for (int s = 0; s < G_; ++s) append_rows(A0, systems[s]->precv[0].a, off0_[s]);
-      A0.nc = n0_;
+      std::vector<std::pair<const HCsr*, int64_t>> pa;
+      for (int s = 0; s < G_; ++s) pa.push_back({&systems[s]->precv[0].a, off0_[s]});
+      concat_rows(A0, pa, n0_);
       akry_ = up64(A0);
     }
+    phase_log("group: Krylov operator", -1, wall_seconds() - t0);
     // reduction blocks: rows of one system each
     const int rb = std::max(1, 8192 / NB);
     std::vector<int4> bt;
