@@ -171,9 +171,9 @@ int mldhip_apply_host(mldhip_prec* p, const double* x, double* y, char trans) {
         if (hipMalloc(&dx, n * sizeof(double) + 8) != hipSuccess) return E_DEVICE;
         if (hipMalloc(&dy, n * sizeof(double) + 8) != hipSuccess) { (void)hipFree(dx); return E_DEVICE; }
         int r = 0;
-        if (hipMemcpyAsync(dx, x, n * sizeof(double), hipMemcpyHostToDevice, p->stream) != hipSuccess) r = E_DEVICE;
-        if (!r) r = mldhip_apply(p, dx, dy, trans);
-        if (!r && hipMemcpyAsync(y, dy, n * sizeof(double), hipMemcpyDeviceToHost, p->stream) != hipSuccess) r = E_DEVICE;
+        mld::host_to_device(dx, x, n * sizeof(double), p->stream);
+        r = mldhip_apply(p, dx, dy, trans);
+        if (!r) mld::device_to_host(y, dy, n * sizeof(double), p->stream);
         if (hipStreamSynchronize(p->stream) != hipSuccess) r = E_DEVICE;
         (void)hipFree(dx);
         (void)hipFree(dy);
@@ -222,10 +222,10 @@ int mldhip_krylov_host(mldhip_prec* p, const char* method, const double* b, doub
         if (hipMalloc(&db, n * sizeof(double) + 8) != hipSuccess) return E_DEVICE;
         if (hipMalloc(&dx, n * sizeof(double) + 8) != hipSuccess) { (void)hipFree(db); return E_DEVICE; }
         int r = 0;
-        if (hipMemcpy(db, b, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) r = E_DEVICE;
-        if (!r && hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) r = E_DEVICE;
-        if (!r) r = mldhip_krylov(p, method, db, dx, eps, itmax, istop, iter, err);
-        if (!r && hipMemcpy(x, dx, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) r = E_DEVICE;
+        mld::host_to_device(db, b, n * sizeof(double), p->stream);
+        mld::host_to_device(dx, x, n * sizeof(double), p->stream);
+        r = mldhip_krylov(p, method, db, dx, eps, itmax, istop, iter, err);
+        if (!r) mld::device_to_host(x, dx, n * sizeof(double), p->stream);
         (void)hipFree(db);
         (void)hipFree(dx);
         return r;

@@ -14,6 +14,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "mld_device.hpp"
 #include "mld_host.hpp"
 #include "mld_kernels.hpp"
 
@@ -89,44 +90,8 @@ struct DLevel {
 // ---------------------------------------------------------------------------
 // Allocation bookkeeping: every device allocation of a handle is freed with it.
 // ---------------------------------------------------------------------------
-// Host-to-device copy of pageable host memory through two pinned staging
-// buffers per device: chunk k+1 is copied into one buffer by the OpenMP
-// threads while the DMA of chunk k from the other buffer is in flight (a
-// direct copy from pageable memory is staged by the runtime at a fraction of
-// the link rate).  A buffer is refilled only after the event recorded behind
-// its previous copy has completed; the caller's host memory may be released
-// when the function returns.  Copies under 4 MB go directly.
-struct H2DStage {
-  static constexpr size_t CH = (size_t)32 << 20;
-  char* buf[2] = {nullptr, nullptr};
-  hipEvent_t ev[2] = {nullptr, nullptr};
-};
-static void h2d_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
-  if (bytes < ((size_t)4 << 20)) {
-    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
-    return;
-  }
-  static std::mutex mu;
-  static std::vector<H2DStage> stages;  // per device, kept until the process ends
-  std::lock_guard<std::mutex> lock(mu);
-  int dev = 0;
-  HIPCHECK(hipGetDevice(&dev));
-  if ((int)stages.size() <= dev) stages.resize(dev + 1);
-  H2DStage& st = stages[dev];
-  if (!st.buf[0]) {
-    for (int b = 0; b < 2; ++b) {
-      HIPCHECK(hipHostMalloc((void**)&st.buf[b], H2DStage::CH, hipHostMallocDefault));
-      HIPCHECK(hipEventCreateWithFlags(&st.ev[b], hipEventDisableTiming));
-    }
-  }
-  int b = 0;
-  for (size_t off = 0; off < bytes; off += H2DStage::CH, b ^= 1) {
-    const size_t len = std::min(H2DStage::CH, bytes - off);
-    HIPCHECK(hipEventSynchronize(st.ev[b]));
-    par_memcpy(st.buf[b], (const char*)src + off, len);
-    HIPCHECK(hipMemcpyAsync((char*)dst + off, st.buf[b], len, hipMemcpyHostToDevice, s));
-    HIPCHECK(hipEventRecord(st.ev[b], s));
-  }
+static inline void h2d_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  host_to_device(dst, src, bytes, s);
 }
 
 struct DevArena {

@@ -4,6 +4,8 @@
 // device-resident CG / FCG / BiCGStab.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -1212,6 +1214,75 @@ int dev_guard(const std::function<int()>& f, std::string& err) {
   } catch (const std::exception& e) {
     err = e.what();
     return E_DEVICE;
+  }
+}
+
+
+namespace {
+struct HostStage {
+  static constexpr size_t CH = (size_t)32 << 20;
+  char* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+std::mutex stage_mu;
+std::vector<HostStage> stages;  // per device, kept until the process ends
+
+HostStage& stage_of_current_device() {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if ((int)stages.size() <= dev) stages.resize(dev + 1);
+  HostStage& st = stages[dev];
+  if (!st.buf[0]) {
+    for (int b = 0; b < 2; ++b) {
+      HIPCHECK(hipHostMalloc((void**)&st.buf[b], HostStage::CH, hipHostMallocDefault));
+      HIPCHECK(hipEventCreateWithFlags(&st.ev[b], hipEventDisableTiming));
+    }
+  }
+  return st;
+}
+}  // namespace
+
+void host_to_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes < ((size_t)4 << 20)) {
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return;
+  }
+  std::lock_guard<std::mutex> lock(stage_mu);
+  HostStage& st = stage_of_current_device();
+  int b = 0;
+  for (size_t off = 0; off < bytes; off += HostStage::CH, b ^= 1) {
+    const size_t len = std::min(HostStage::CH, bytes - off);
+    HIPCHECK(hipEventSynchronize(st.ev[b]));  // the buffer's previous copy has completed
+    par_memcpy(st.buf[b], (const char*)src + off, len);
+    HIPCHECK(hipMemcpyAsync((char*)dst + off, st.buf[b], len, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipEventRecord(st.ev[b], s));
+  }
+}
+
+void device_to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes < ((size_t)4 << 20)) {
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    return;
+  }
+  std::lock_guard<std::mutex> lock(stage_mu);
+  HostStage& st = stage_of_current_device();
+  const size_t nch = (bytes + HostStage::CH - 1) / HostStage::CH;
+  auto issue = [&](size_t k) {
+    const int b = (int)(k & 1);
+    const size_t off = k * HostStage::CH, len = std::min(HostStage::CH, bytes - off);
+    HIPCHECK(hipEventSynchronize(st.ev[b]));
+    HIPCHECK(hipMemcpyAsync(st.buf[b], (const char*)src + off, len, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipEventRecord(st.ev[b], s));
+  };
+  issue(0);
+  if (nch > 1) issue(1);
+  for (size_t k = 0; k < nch; ++k) {
+    const int b = (int)(k & 1);
+    const size_t off = k * HostStage::CH, len = std::min(HostStage::CH, bytes - off);
+    HIPCHECK(hipEventSynchronize(st.ev[b]));
+    par_memcpy((char*)dst + off, st.buf[b], len);
+    if (k + 2 < nch) issue(k + 2);
   }
 }
 
