@@ -19,6 +19,23 @@ struct mldhip_prec {
   std::unique_ptr<mld::DevPrec> dev;
   std::unique_ptr<mld::MultiEngine> blk;  // block (32-column) engine, built on first use
   std::string err;
+  // device copies of the host-vector entry points' two vectors, kept from
+  // call to call (grown when the system grows, freed with the handle)
+  double* hv = nullptr;
+  size_t hv_n = 0;
+  double* host_vectors(size_t n) {
+    if (hv_n < n) {
+      if (hv) (void)hipFree(hv);
+      hv = nullptr;
+      hv_n = 0;
+      if (hipMalloc(&hv, 2 * n * sizeof(double) + 16) != hipSuccess) {
+        hv = nullptr;
+        return nullptr;
+      }
+      hv_n = n;
+    }
+    return hv;
+  }
 };
 
 using mld::E_BAD_ARG;
@@ -73,6 +90,7 @@ int mldhip_destroy(mldhip_prec* p) {
   if (p->device >= 0) (void)hipSetDevice(p->device);
   p->dev.reset();
   p->blk.reset();
+  if (p->hv) (void)hipFree(p->hv);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return 0;
@@ -167,16 +185,14 @@ int mldhip_apply_host(mldhip_prec* p, const double* x, double* y, char trans) {
   const size_t n = (size_t)p->host.precv[0].a.nr;
   return mld::dev_guard(
       [&]() -> int {
-        double *dx = nullptr, *dy = nullptr;
-        if (hipMalloc(&dx, n * sizeof(double) + 8) != hipSuccess) return E_DEVICE;
-        if (hipMalloc(&dy, n * sizeof(double) + 8) != hipSuccess) { (void)hipFree(dx); return E_DEVICE; }
+        double* dx = p->host_vectors(n);
+        if (!dx) return E_DEVICE;
+        double* dy = dx + n;
         int r = 0;
         mld::host_to_device(dx, x, n * sizeof(double), p->stream);
         r = mldhip_apply(p, dx, dy, trans);
         if (!r) mld::device_to_host(y, dy, n * sizeof(double), p->stream);
         if (hipStreamSynchronize(p->stream) != hipSuccess) r = E_DEVICE;
-        (void)hipFree(dx);
-        (void)hipFree(dy);
         return r;
       },
       p->err);
@@ -218,16 +234,14 @@ int mldhip_krylov_host(mldhip_prec* p, const char* method, const double* b, doub
   const size_t n = (size_t)p->host.precv[0].a.nr;
   return mld::dev_guard(
       [&]() -> int {
-        double *db = nullptr, *dx = nullptr;
-        if (hipMalloc(&db, n * sizeof(double) + 8) != hipSuccess) return E_DEVICE;
-        if (hipMalloc(&dx, n * sizeof(double) + 8) != hipSuccess) { (void)hipFree(db); return E_DEVICE; }
+        double* db = p->host_vectors(n);
+        if (!db) return E_DEVICE;
+        double* dx = db + n;
         int r = 0;
         mld::host_to_device(db, b, n * sizeof(double), p->stream);
         mld::host_to_device(dx, x, n * sizeof(double), p->stream);
         r = mldhip_krylov(p, method, db, dx, eps, itmax, istop, iter, err);
         if (!r) mld::device_to_host(x, dx, n * sizeof(double), p->stream);
-        (void)hipFree(db);
-        (void)hipFree(dx);
         return r;
       },
       p->err);
