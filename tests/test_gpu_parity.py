@@ -404,3 +404,28 @@ def test_diffusion_mixed_fcg(eng, orc):
     rel = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
     print(f"mixed FCG iters {it} (oracle FP64 CG to 1e-12: {ito}); solution error vs oracle {rel:.2e}")
     assert rel <= 1e-6
+
+
+def test_host_vector_entry_points(eng, orc):
+    """mldhip_apply_host / mldhip_krylov_host (the Fortran driver's path):
+    copies through pinned staging into device vectors the handle keeps; the
+    results equal the device-vector entry points bit for bit, on repeated
+    calls and after the handle is rebuilt for a larger system (the kept
+    vectors grow; at idim 164 a vector is 35 MB, two chunks of the 32 MB
+    staging buffers)."""
+    _say("[host vectors] idim=24, then 164")
+    for idim in (24, 164):
+        a = orc.gen_pde3d(idim)
+        A = a.to_scipy()
+        p = engine_prec(eng, A, "L1-JACOBI", 2)
+        n = A.shape[0]
+        x = np.random.default_rng(idim).standard_normal(n)
+        for _ in range(2):
+            yh = p.apply(x)
+            yd = p.apply(dev(x)).cpu().numpy()
+            assert (yh == yd).all()
+        b = np.ones(n)
+        xh, ith, erh = eng.psb_krylov("CG", A, p, b, eps=1e-8)
+        xd, itd, erd = eng.psb_krylov("CG", A, p, dev(b), eps=1e-8)
+        assert ith == itd and erh == erd
+        assert (xh == xd.cpu().numpy()).all()
