@@ -31,6 +31,13 @@ void phase_log(const char* what, int level, double seconds) {
   if (on) std::fprintf(stderr, "[mld] level %d %-22s %9.3f ms\n", level, what, seconds * 1e3);
 }
 
+int host_max_row(const HCsr& a) {
+  int m = 0;
+#pragma omp parallel for schedule(static) reduction(max : m)
+  for (int i = 0; i < a.nr; ++i) m = std::max(m, (int)(a.irp[i + 1] - a.irp[i]));
+  return m;
+}
+
 std::vector<int> host_dia_offsets(const HCsr& a) {
   // each thread collects the offsets of a contiguous row range (at most 9 kept:
   // a ninth means "more than 8"), then the sets are merged

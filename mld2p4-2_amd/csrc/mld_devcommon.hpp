@@ -150,8 +150,7 @@ static bool dia_eligible(const HCsr& h, const std::vector<int>& offs) {
 }
 
 static FmtChoice choose_format(const HCsr& h, int force) {
-  int maxrow = 0;
-  for (int i = 0; i < h.nr; ++i) maxrow = std::max(maxrow, (int)(h.irp[i + 1] - h.irp[i]));
+  const int maxrow = host_max_row(h);
   const double nnz = (double)h.nnz();
   const double avg = h.nr ? nnz / h.nr : 0.0;
   FmtChoice c{FMT_CSR, 1, maxrow};
@@ -195,7 +194,7 @@ static DevSrc make_src(const HCsr& h, DevArena& tmp, hipStream_t s) {
   DevSrc d;
   d.nr = h.nr; d.nc = h.nc; d.nnz = h.nnz();
   if (d.nnz >= ((int64_t)1 << 31)) throw DevError("operator exceeds 2^31 entries");
-  for (int i = 0; i < h.nr; ++i) d.maxrow = std::max(d.maxrow, (int)(h.irp[i + 1] - h.irp[i]));
+  d.maxrow = host_max_row(h);
   int64_t* irp64 = tmp.alloc<int64_t>((size_t)h.nr + 1);
   if (!h.irp.empty()) h2d_copy(irp64, h.irp.data(), ((size_t)h.nr + 1) * sizeof(int64_t), s);
   else HIPCHECK(hipMemsetAsync(irp64, 0, ((size_t)h.nr + 1) * sizeof(int64_t), s));
@@ -450,8 +449,7 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
   std::vector<int> offs = dia_offsets(h);
   const bool dia = dia_eligible(h, offs);
   if (!dia) {  // DIA reads strictly fewer bytes than any other layout
-    int maxrow = 0;
-    for (int i = 0; i < h.nr; ++i) maxrow = std::max(maxrow, (int)(h.irp[i + 1] - h.irp[i]));
+    const int maxrow = host_max_row(h);
     const double avg = (double)h.nnz() / std::max(1, h.nr);
     struct Cand { int force, lanes; bool ordered; };
     std::vector<Cand> cands;

@@ -146,6 +146,8 @@ double wall_seconds();
 void phase_log(const char* what, int level, double seconds);
 // memcpy of n bytes split over the OpenMP threads (host staging of uploads)
 void par_memcpy(void* dst, const void* src, size_t n);
+// Longest row of a (over the host threads).
+int host_max_row(const HCsr& a);
 // Distinct diagonal offsets j - i of a, ascending; empty when more than 8.
 std::vector<int> host_dia_offsets(const HCsr& a);
 // SELL row order and slice offsets: rows stably sorted by decreasing length
