@@ -161,31 +161,123 @@ def release(torch):
     torch.cuda.empty_cache()
 
 
-def cpu_baseline(idim, gpu_iters, sample_iters):
-    """Oracle (sequential CPU restatement) timed on this host: hierarchy
-    build, then a bounded sample of CG iterations on the same problem, scaled
-    to the GPU's iteration count."""
-    sys.path.insert(0, ROOT)
+def cpu_info():
+    """Host CPU model, nproc, the CPUs this process may use and the thread
+    count of an all-cores run (OMP_NUM_THREADS when set, else every usable
+    CPU)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count()
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = nproc
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(omp) if omp.isdigit() and int(omp) > 0 else avail
+    return {"cpu_model": model, "nproc": nproc, "cpus_available": avail, "threads_all_cores": min(threads, avail)}
+
+
+def _oracle():
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
     from oracle import oracle as O
 
+    return O
+
+
+def cpu_baseline(idim, gpu_iters, sample_iters=5):
+    """The benchmark workload on the oracle (the CPU restatement; its results
+    do not depend on the thread count) on this host: hierarchy + smoothers
+    build once, then CG iterations on one core and on all cores, one warm-up
+    iteration and the median of `sample_iters` timed ones (steady_clock),
+    scaled to the GPU's iteration count."""
+    O = _oracle()
+    info = cpu_info()
     a = O.gen_pde3d(idim)
     prm = O.config1_params(smoother=O.SM_L1JAC, sweeps_pre=2, sweeps_post=2)
-    t0 = time.perf_counter()
+    O.set_threads(info["threads_all_cores"])
     po = O.Prec(prm).build(a)
-    tb = time.perf_counter() - t0
+    tb = po.build_s
     b = np.ones(a.shape[0])
-    t0 = time.perf_counter()
-    _, it, _ = po.krylov("CG", b, eps=1e-30, itmax=sample_iters)
-    ts = time.perf_counter() - t0
-    per_it = ts / max(it, 1)
+    per_it = {}
+    for cores in (1, info["threads_all_cores"]):
+        O.set_threads(cores)
+        _, it, _, its, _ = po.krylov("CG", b, eps=1e-30, itmax=1 + sample_iters, timings=True)
+        per_it[cores] = float(np.median(its[1:]))
+    one, allc = per_it[1], per_it[info["threads_all_cores"]]
     return {
-        "value": per_it * gpu_iters,
+        "value": round(one * gpu_iters, 3),
         "unit": "s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"oracle (sequential C++ restatement) at idim={idim}: hierarchy build {tb:.1f}s (not in value), "
-                   f"{it} CG iterations timed ({ts:.1f}s, {per_it:.2f}s/iter) x {gpu_iters} GPU iterations"),
+        "sample": (f"oracle (C++ restatement) at idim={idim}: hierarchy + smoothers build {tb:.1f}s (not in value); "
+                   f"CG iterations: 1 warm-up + median of {sample_iters} timed (steady_clock) = {one:.3f}s/iter on "
+                   f"1 core, x {gpu_iters} GPU iterations"),
+        "all_cores": {"value": round(allc * gpu_iters, 3), "cores": info["threads_all_cores"],
+                      "s_per_iter": round(allc, 4),
+                      "note": "row loops of products and vector updates over OpenMP threads; dots and ILU "
+                              "substitutions sequential"},
+        "setup_s": round(tb, 2),
+        **info,
     }
+
+
+def section_c1(eng, torch, problems):
+    """The reference's CPU-runnable case (idim 40, JACOBI 1/1, coarsest 4
+    sweeps of BJAC/ILU(0), CG to 1e-6): the oracle on one core against the
+    GPU path, setup (hierarchy_build + smoothers_build) and solve each as the
+    median of 5 after one warm-up."""
+    O = _oracle()
+    O.set_threads(1)
+    idim = 40
+    a = O.gen_pde3d(idim)
+    b = np.ones(a.shape[0])
+    setups, solves, ito = [], [], 0
+    for k in range(6):
+        po = O.Prec(O.config1_params()).build(a)
+        _, ito, erro, _, tot = po.krylov("CG", b, eps=1e-6, timings=True)
+        if k:
+            setups.append(po.build_s)
+            solves.append(tot)
+    irp, ja, val = problems.pde3d_poisson(idim)
+    bd = torch.ones(len(irp) - 1, dtype=torch.float64, device="cuda:0")
+    gsetups, gsolves, itg = [], [], 0
+    for k in range(6):
+        p = eng.MldPrec(device=0).init("ML")
+        for key, v in [("SMOOTHER_TYPE", "JACOBI"), ("SMOOTHER_SWEEPS", 1), ("AGGR_THRESH", 0.0),
+                       ("AGGR_PROL", "SMOOTHED"), ("MIN_COARSE_SIZE", 200), ("COARSE_SOLVE", "BJAC"),
+                       ("COARSE_SUBSOLVE", "ILU"), ("COARSE_SWEEPS", 4)]:
+            p.set(key, v)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p.build((irp, ja, val))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        x = torch.zeros_like(bd)
+        _, itg, errg = eng.psb_krylov("CG", None, p, bd, x, eps=1e-6)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if k:
+            gsetups.append(t1 - t0)
+            gsolves.append(t2 - t1)
+        del p
+    cpu_solve, gpu_solve = float(np.median(solves)), float(np.median(gsolves))
+    return {"workload": "3-D Poisson idim=40 (n=64000), b=1, x0=0; JACOBI 1/1, theta 0, smoothed prolongator, "
+                        "MIN_COARSE_SIZE 200, coarsest 4 sweeps of BJAC/ILU(0); CG to 1e-6",
+            "cpu": {"setup_s": round(float(np.median(setups)), 4), "solve_s": round(cpu_solve, 4), "iters": ito,
+                    "relres": erro, "cores": 1, "kind": "port"},
+            "gpu": {"setup_s": round(float(np.median(gsetups)), 4), "solve_s": round(gpu_solve, 5), "iters": itg,
+                    "relres": errg},
+            "solve_speedup_gpu_vs_cpu": round(cpu_solve / gpu_solve, 2),
+            "timing": "1 warm-up + median of 5 (CPU: steady_clock inside the oracle; GPU: host clock around "
+                      "synchronized calls)"}
 
 
 # ----------------------------------------------------------------- sections
@@ -475,8 +567,8 @@ def main():
     ap.add_argument("--sweeps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--cpu-sample-iters", type=int, default=2)
-    ap.add_argument("--sections", default="run_b,diffusion,batched,adjoint",
+    ap.add_argument("--cpu-sample-iters", type=int, default=5)
+    ap.add_argument("--sections", default="c1,run_b,diffusion,batched,adjoint",
                     help="extra sections to run (comma list; empty for none)")
     ap.add_argument("--small", action="store_true", help="small sizes for the extra sections (quick check)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -604,6 +696,11 @@ def main():
 
     extra = {}
     small = args.small
+    if "c1" in sections:
+        log("C1: idim 40, oracle on one core vs GPU")
+        extra["c1"] = section_c1(eng, torch, problems)
+        log(f"C1: {extra['c1']}")
+        release(torch)
     if "run_b" in sections:
         log("run B: BJAC/ILU(0) smoother")
         extra["run_b"] = section_run_b(eng, torch, problems, irp, ja, val, n, min(args.steps, 2))

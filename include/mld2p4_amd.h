@@ -70,6 +70,46 @@ int mldhip_smoothers_build(mldhip_prec* p);
 /* prec%build = hierarchy_build + smoothers_build (mlprec/impl/mld_dprecbld.f90:135,143). */
 int mldhip_build(mldhip_prec* p, int n, const int64_t* irp, const int* ja, const double* val);
 
+/* ---- a hierarchy built outside the engine ---------------------------------
+ * The data each precv(l) holds after the reference's hierarchy_build and
+ * smoothers_build (mlprec/mld_d_onelev_mod.f90:122-158): base_a and the map
+ * of mld_aggrmat_asb (mlprec/impl/mld_d_lev_aggrmat_asb.f90:246-247), and
+ * the smoother objects sm / sm2a (mld_d_jac_smoother_bld.f90:73-103 and the
+ * solvers' bld routines).  Levels are 0-based (precv(lev+1)); CSR as for
+ * hierarchy_build (0-based, int64 row pointers from 0, strictly ascending
+ * columns); every array is copied.  The configured parameters (set) are
+ * distributed over the levels as hierarchy_build does. */
+/* Level lev: A (n x n), and for lev >= 1 P (nf x n, nf = rows of level
+ * lev-1: map%p, psb_map_Y2X) and R (n x nf: map%r, psb_map_X2Y); P and R
+ * NULL for lev 0. */
+int mldhip_upload_level(mldhip_prec* p, int lev, int n, const int64_t* irp, const int* ja, const double* val, int nf,
+                        const int64_t* pirp, const int* pja, const double* pval, const int64_t* rirp, const int* rja,
+                        const double* rval);
+/* Smoother data of level lev (uploaded before): which 0 = sm (pre / both),
+ * 1 = sm2a (post).  solver: 4 none, 5 diag (dv = 1/diag), 101 l1-diag
+ * (dv = 1/l1-diag), 7 ILU (L unit lower and U unit upper strict factors,
+ * dv = 1/pivot: mld_dilu0_fact.f90:176-182), 6 GS / 14 BWGS (strict lower L,
+ * strict upper U, dv = the diagonal).  ND (NULL: empty) is the Jacobi
+ * smoother's off-diagonal operator; empty for the block solvers on one
+ * device. */
+int mldhip_upload_smoother(mldhip_prec* p, int lev, int which, int solver, const int64_t* ndirp, const int* ndja,
+                           const double* ndval, const double* dv, const int64_t* lirp, const int* lja,
+                           const double* lval, const int64_t* uirp, const int* uja, const double* uval);
+/* Levels 0 .. nlev-1 become the hierarchy (the state after hierarchy_build).
+ * When smoother data was uploaded for every level, the smoothers are built
+ * too and a device handle uploads to HBM (the state after smoothers_build);
+ * otherwise call mldhip_smoothers_build. */
+int mldhip_finalize_hierarchy(mldhip_prec* p, int nlev);
+/* The hierarchy of a prec%dump (mldhip_dump with ac and rp; mlprec/impl/
+ * level/mld_d_base_onelev_dump.f90:92-121): A of level 1 given here (the
+ * dump does not write it), levels 2.. read from <prefix>_p000_lLLL_{ac,p,r}
+ * .mtx until a level's ac file is missing.  smoothers != 0: also the smoother
+ * files of every level (a dump from level 1 with smoother and solver data),
+ * read for the configured solvers, and the device upload; else call
+ * mldhip_smoothers_build. */
+int mldhip_load_hierarchy(mldhip_prec* p, const char* prefix, int n, const int64_t* irp, const int* ja,
+                          const double* val, int smoothers);
+
 /* ---- apply (the hot path) ------------------------------------------------ */
 /* prec%apply(x, y, desc, info, trans): y = op(M^-1) x;
  * mlprec/impl/mld_dprecaply.f90:305-442 (mld_dprecaply2_vect) ->
@@ -79,6 +119,10 @@ int mldhip_build(mldhip_prec* p, int n, const int64_t* irp, const int* ja, const
 int mldhip_apply(mldhip_prec* p, const double* x, double* y, char trans);
 /* Same with host vectors (copies over PCIe, synchronous). */
 int mldhip_apply_host(mldhip_prec* p, const double* x, double* y, char trans);
+/* mld_dmlprec_aply_vect(alpha, prec, x, beta, y, desc, trans, work, info)
+ * (mlprec/impl/mld_dmlprec_aply.f90:203-357): y = alpha op(M^-1) x + beta y,
+ * the final update of :327; y is not read when beta = 0.  x, y: device. */
+int mldhip_mlprec_aply(mldhip_prec* p, double alpha, const double* x, double beta, double* y, char trans);
 
 /* psb_krylov(method, a, prec, b, x, eps, desc, info, itmax, iter, err,
  * istop) call site (tests/pdegen/mld_d_pde3d.f90:753-755) with the matrix

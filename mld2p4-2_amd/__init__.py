@@ -60,6 +60,11 @@ def lib():
             "mldhip_hierarchy_build": [vp, C.c_int, i64p, ip, dp],
             "mldhip_smoothers_build": [vp],
             "mldhip_build": [vp, C.c_int, i64p, ip, dp],
+            "mldhip_upload_level": [vp, C.c_int, C.c_int, i64p, ip, dp, C.c_int, i64p, ip, dp, i64p, ip, dp],
+            "mldhip_upload_smoother": [vp, C.c_int, C.c_int, C.c_int, i64p, ip, dp, dp, i64p, ip, dp, i64p, ip, dp],
+            "mldhip_finalize_hierarchy": [vp, C.c_int],
+            "mldhip_load_hierarchy": [vp, cp, C.c_int, i64p, ip, dp, C.c_int],
+            "mldhip_mlprec_aply": [vp, C.c_double, vp, C.c_double, vp, C.c_char],
             "mldhip_apply": [vp, vp, vp, C.c_char],
             "mldhip_apply_host": [vp, dp, dp, C.c_char],
             "mldhip_krylov": [vp, cp, vp, vp, C.c_double, C.c_int, C.c_int, ip, dp],
@@ -211,6 +216,62 @@ class MldPrec:
     def build(self, a):
         return self.hierarchy_build(a).smoothers_build()
 
+    # -- a hierarchy built elsewhere ------------------------------------------------
+    SOLVERS = {"NONE": 4, "DIAG": 5, "GS": 6, "ILU": 7, "BWGS": 14, "L1-DIAG": 101}
+
+    def upload_level(self, lev, a, p=None, r=None):
+        """Level lev (0-based) of an externally built hierarchy: A, and for
+        lev >= 1 the prolongation P (n_{lev-1} x n) and restriction R."""
+        n, irp, ja, val = csr_arrays(a)
+        keep = [(irp, ja, val)]
+        if lev > 0:
+            _, pi, pj, pv = csr_arrays(p)
+            _, ri, rj, rv = csr_arrays(r)
+            nf = p.shape[0]
+            keep += [(pi, pj, pv), (ri, rj, rv)]
+        args = []
+        for t in keep + [None] * (3 - len(keep)):
+            if t is None:
+                args += [None, None, None]
+            else:
+                args += [t[0].ctypes.data_as(C.POINTER(C.c_int64)), t[1].ctypes.data_as(C.POINTER(C.c_int)),
+                         _dp(t[2])]
+        self._check(lib().mldhip_upload_level(self.h, lev, n, *args[:3], nf if lev > 0 else 0, *args[3:]),
+                    f"upload_level({lev})")
+        if lev == 0:
+            self.n = n
+        return self
+
+    def upload_smoother(self, lev, solver, dv=None, nd=None, l=None, u=None, which=0):
+        """Smoother data of level lev: solver in SOLVERS, dv, ND, strict L, U."""
+        def csr(m):
+            if m is None:
+                return [None, None, None], None
+            _, i, j, v = csr_arrays(m)
+            return [i.ctypes.data_as(C.POINTER(C.c_int64)), j.ctypes.data_as(C.POINTER(C.c_int)), _dp(v)], (i, j, v)
+        a_nd, k1 = csr(nd)
+        a_l, k2 = csr(l)
+        a_u, k3 = csr(u)
+        d = None if dv is None else np.ascontiguousarray(dv, dtype=np.float64)
+        rc = lib().mldhip_upload_smoother(self.h, lev, which, self.SOLVERS[solver], *a_nd,
+                                          None if d is None else _dp(d), *a_l, *a_u)
+        self._check(rc, f"upload_smoother({lev})")
+        return self
+
+    def finalize_hierarchy(self, nlev):
+        self._check(lib().mldhip_finalize_hierarchy(self.h, nlev), "finalize_hierarchy")
+        return self
+
+    def load_hierarchy(self, prefix, a, smoothers=False):
+        """Hierarchy of a dump under prefix over the fine matrix a."""
+        n, irp, ja, val = csr_arrays(a)
+        self._keep = (irp, ja, val)
+        rc = lib().mldhip_load_hierarchy(self.h, prefix.encode(), n, irp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         ja.ctypes.data_as(C.POINTER(C.c_int)), _dp(val), int(bool(smoothers)))
+        self._check(rc, "load_hierarchy")
+        self.n = n
+        return self
+
     # -- apply ----------------------------------------------------------------
     def apply(self, x, y=None, trans: str = "N"):
         """y = op(M^-1) x.  torch CUDA tensors (float64) stay on the device;
@@ -226,6 +287,14 @@ class MldPrec:
             y = torch.empty_like(x)
         self.use_torch_stream()
         self._check(lib().mldhip_apply(self.h, _ptr(x), _ptr(y), trans.encode()), "apply")
+        return y
+
+    def mlprec_aply(self, alpha, x, beta, y, trans: str = "N"):
+        """y = alpha op(M^-1) x + beta y (mld_dmlprec_aply_vect); torch CUDA
+        float64 tensors."""
+        self.use_torch_stream()
+        self._check(lib().mldhip_mlprec_aply(self.h, float(alpha), _ptr(x), float(beta), _ptr(y), trans.encode()),
+                    "mlprec_aply")
         return y
 
     def apply_block(self, X, Y=None):

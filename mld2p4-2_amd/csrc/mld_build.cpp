@@ -721,6 +721,26 @@ static int build_smoother(const HCsr& a, const SmootherSpec& spec, SmootherData&
   }
 }
 
+void resize_levels(std::vector<Level>& v, int nplevs) {
+  // resize to the target number of levels (mld_d_hierarchy_bld.f90:199-241):
+  // base parms/smoothers at 1, intermediate ones from level 2, coarse ones at
+  // the end
+  const int iszv0 = (int)v.size();
+  if (iszv0 == nplevs) return;
+  std::vector<Level> t(nplevs);
+  Level base = v[0], med = v[std::min(1, iszv0 - 1)], crs = v[iszv0 - 1];
+  t[0].parms = base.parms; t[0].sm = base.sm; t[0].sm2a = base.sm2a;
+  for (int i = 2; i <= std::min(iszv0, nplevs) - 1; ++i) {
+    t[i - 1].parms = med.parms; t[i - 1].sm = v[i - 1].sm; t[i - 1].sm2a = v[i - 1].sm2a;
+  }
+  for (int i = iszv0 - 1; i <= nplevs - 1; ++i) {  // as in the reference, may start at level 1
+    t[i - 1].parms = med.parms; t[i - 1].sm = med.sm; t[i - 1].sm2a = med.sm2a;
+  }
+  t[nplevs - 1].parms = crs.parms; t[nplevs - 1].sm = crs.sm; t[nplevs - 1].sm2a = crs.sm2a;
+  t[0].a = std::move(v[0].a);
+  v = std::move(t);
+}
+
 int hierarchy_build(Prec& p, int n, const int64_t* irp, const int* ja, const double* val) {
   if (!p.initialized || p.precv.empty()) return E_NOT_INIT;
   p.hier_built = p.smoothers_built = false;
@@ -755,24 +775,8 @@ int hierarchy_build(Prec& p, int n, const int64_t* irp, const int* ja, const dou
     casize = std::max(casize, 1) * 40;
     p.min_coarse_size = casize;
   }
-  const int iszv0 = (int)v.size();
   const int nplevs = std::max(2, p.max_levs);
-  // resize to the target number of levels (:199-241): base parms/smoothers
-  // at 1, intermediate ones from level 2, coarse ones at the end
-  if (iszv0 != nplevs) {
-    std::vector<Level> t(nplevs);
-    Level base = v[0], med = v[std::min(1, iszv0 - 1)], crs = v[iszv0 - 1];
-    t[0].parms = base.parms; t[0].sm = base.sm; t[0].sm2a = base.sm2a;
-    for (int i = 2; i <= std::min(iszv0, nplevs) - 1; ++i) {
-      t[i - 1].parms = med.parms; t[i - 1].sm = v[i - 1].sm; t[i - 1].sm2a = v[i - 1].sm2a;
-    }
-    for (int i = iszv0 - 1; i <= nplevs - 1; ++i) {  // as in the reference, may start at level 1
-      t[i - 1].parms = med.parms; t[i - 1].sm = med.sm; t[i - 1].sm2a = med.sm2a;
-    }
-    t[nplevs - 1].parms = crs.parms; t[nplevs - 1].sm = crs.sm; t[nplevs - 1].sm2a = crs.sm2a;
-    t[0].a = std::move(v[0].a);
-    v = std::move(t);
-  }
+  resize_levels(v, nplevs);
   const int iszv = (int)v.size();
   const LevelParms coarseparms = v[iszv - 1].parms;
   const SmootherSpec coarse_sm = v[iszv - 1].sm, coarse_sm2a = v[iszv - 1].sm2a;

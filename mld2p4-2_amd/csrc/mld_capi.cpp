@@ -137,12 +137,10 @@ int mldhip_hierarchy_build(mldhip_prec* p, int n, const int64_t* irp, const int*
   return 0;
 }
 
-int mldhip_smoothers_build(mldhip_prec* p) {
-  if (!p) return E_BAD_ARG;
-  p->dev.reset();
-  p->blk.reset();
-  int rc = mld::smoothers_build(p->host);
-  if (rc) return fail(p, rc, "smoothers_build failed");
+// The device copy of a hierarchy whose smoother data is complete (built or
+// uploaded): HBM upload once, the amold/vmold conversion of
+// mld_d_base_onelev_cnv.f90:56-61.
+static int upload_device(mldhip_prec* p) {
   if (p->device < 0) return 0;
   return mld::dev_guard(
       [&]() -> int {
@@ -155,6 +153,57 @@ int mldhip_smoothers_build(mldhip_prec* p) {
         return p->dev->upload(p->host);
       },
       p->err);
+}
+
+int mldhip_smoothers_build(mldhip_prec* p) {
+  if (!p) return E_BAD_ARG;
+  p->dev.reset();
+  p->blk.reset();
+  int rc = mld::smoothers_build(p->host);
+  if (rc) return fail(p, rc, "smoothers_build failed");
+  return upload_device(p);
+}
+
+int mldhip_upload_level(mldhip_prec* p, int lev, int n, const int64_t* irp, const int* ja, const double* val, int nf,
+                        const int64_t* pirp, const int* pja, const double* pval, const int64_t* rirp, const int* rja,
+                        const double* rval) {
+  if (!p) return E_BAD_ARG;
+  p->dev.reset();
+  p->blk.reset();
+  int rc = mld::stage_level(p->host, lev, n, irp, ja, val, nf, pirp, pja, pval, rirp, rja, rval);
+  if (rc) return fail(p, rc, p->host.last_error);
+  return 0;
+}
+
+int mldhip_upload_smoother(mldhip_prec* p, int lev, int which, int solver, const int64_t* ndirp, const int* ndja,
+                           const double* ndval, const double* dv, const int64_t* lirp, const int* lja,
+                           const double* lval, const int64_t* uirp, const int* uja, const double* uval) {
+  if (!p) return E_BAD_ARG;
+  int rc = mld::stage_smoother(p->host, lev, which, solver, ndirp, ndja, ndval, dv, lirp, lja, lval, uirp, uja, uval);
+  if (rc) return fail(p, rc, p->host.last_error);
+  return 0;
+}
+
+int mldhip_finalize_hierarchy(mldhip_prec* p, int nlev) {
+  if (!p) return E_BAD_ARG;
+  p->dev.reset();
+  p->blk.reset();
+  bool done = false;
+  int rc = mld::finalize_hierarchy(p->host, nlev, done);
+  if (rc) return fail(p, rc, p->host.last_error);
+  return done ? upload_device(p) : 0;
+}
+
+int mldhip_load_hierarchy(mldhip_prec* p, const char* prefix, int n, const int64_t* irp, const int* ja,
+                          const double* val, int smoothers) {
+  if (!p || n < 0 || !irp) return E_BAD_ARG;
+  p->dev.reset();
+  p->blk.reset();
+  bool done = false;
+  std::string e;
+  int rc = mld::load_hierarchy(p->host, S(prefix), n, irp, ja, val, smoothers != 0, done, e);
+  if (rc) return fail(p, rc, e);
+  return done ? upload_device(p) : 0;
 }
 
 int mldhip_build(mldhip_prec* p, int n, const int64_t* irp, const int* ja, const double* val) {
@@ -177,6 +226,15 @@ int mldhip_apply(mldhip_prec* p, const double* x, double* y, char trans) {
   if (trans == 't' || trans == 'C' || trans == 'c') trans = 'T';
   if (trans != 'N' && trans != 'T') return fail(p, E_BAD_ARG, "invalid trans");
   return mld::dev_guard([&]() { return p->dev->apply(x, y, trans); }, p->err);
+}
+
+int mldhip_mlprec_aply(mldhip_prec* p, double alpha, const double* x, double beta, double* y, char trans) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  if (trans == 'n') trans = 'N';
+  if (trans == 't' || trans == 'C' || trans == 'c') trans = 'T';
+  if (trans != 'N' && trans != 'T') return fail(p, E_BAD_ARG, "invalid trans");
+  return mld::dev_guard([&]() { return p->dev->apply_ab(alpha, x, beta, y, trans); }, p->err);
 }
 
 int mldhip_apply_host(mldhip_prec* p, const double* x, double* y, char trans) {

@@ -435,15 +435,22 @@ static float time_spmv(const DMat<V>& m, const V* x, V* y, hipStream_t s, hipEve
 template <typename V>
 static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int force) {
   if (force != 0 || h.nr < TUNE_MIN_ROWS || !autotune_enabled()) return upload_mat<V>(h, ar, s, force);
-  V* x = nullptr;
-  V* y = nullptr;
-  HIPCHECK(hipMalloc(&x, sizeof(V) * (size_t)std::max(1, h.nc)));
-  HIPCHECK(hipMalloc(&y, sizeof(V) * (size_t)std::max(1, h.nr)));
-  fill_kernel<V><<<nblk(h.nc), BS, 0, s>>>(h.nc, x, (V)1);
-  hipEvent_t e0, e1;
-  HIPCHECK(hipEventCreate(&e0));
-  HIPCHECK(hipEventCreate(&e1));
+  // every temporary is released also when a candidate layout or a timing
+  // launch throws
   DevArena tmp;
+  V* x = tmp.alloc<V>((size_t)std::max(1, h.nc));
+  V* y = tmp.alloc<V>((size_t)std::max(1, h.nr));
+  fill_kernel<V><<<nblk(h.nc), BS, 0, s>>>(h.nc, x, (V)1);
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t v : e)
+        if (v) (void)hipEventDestroy(v);
+    }
+  } ev;
+  HIPCHECK(hipEventCreate(&ev.e[0]));
+  HIPCHECK(hipEventCreate(&ev.e[1]));
+  hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
   const DevSrc src = make_src(h, tmp, s);
   int bforce = 6, blanes = 0;
   std::vector<int> offs = dia_offsets(h);
@@ -494,10 +501,7 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
     }
     m.xmap = xb;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(x);
-  (void)hipFree(y);
+  HIPCHECK(hipStreamSynchronize(s));  // the timing launches are done before tmp is freed
   return m;
 }
 

@@ -119,12 +119,33 @@ class DevPrecT : public DevPrec {
   // y = op(M^-1) x (mld_dprecaply2_vect with alpha = 1, beta = 0).
   int apply(const double* x, double* y, char trans) override { return apply_impl(x, y, trans, nullptr, -1); }
 
+  // mld_dmlprec_aply_vect's final update y = alpha vy2l(1) + beta y
+  // (mld_dmlprec_aply.f90:327): the cycle's output in a work vector, then one
+  // streaming pass (alpha = 1, beta = 0 writes y directly).
+  int apply_ab(double alpha, const double* x, double beta, double* y, char trans) override {
+    if (alpha == 1.0 && beta == 0.0) return apply_impl(x, y, trans, nullptr, -1);
+    HIPCHECK(hipSetDevice(device_));
+    if (!wab_) wab_ = ar_.alloc<double>(n0_);
+    int rc = apply_impl(x, wab_, trans, nullptr, -1);
+    if (rc) return rc;
+    axpby(alpha, wab_, beta, y);
+    return 0;
+  }
+  double* wab_ = nullptr;
+
   // y = op(M^-1) x; when dot_u is given, also scal_[dot_slot] = dot_u . y.
   // The fine level's last post-smoothing sweep writes y (FP64) and the dot
   // directly when it can (fused output), else a conversion / dot follows.
   int apply_impl(const double* x, double* y, char trans, const double* dot_u, int dot_slot) {
     HIPCHECK(hipSetDevice(device_));
     if (trans != 'N') ensure_transposes();
+    struct Reset {  // the caller's pointers are dropped also when a HIP error throws
+      DevPrecT& p;
+      ~Reset() {
+        p.out64_ = nullptr;
+        p.dot_u_ = nullptr;
+      }
+    } reset{*this};
     out64_ = y;
     dot_u_ = dot_u;
     dot_slot_ = dot_slot;
@@ -203,15 +224,21 @@ class DevPrecT : public DevPrec {
       out64_ = nullptr;  // intermediate cycles must update y2l itself
       return apply_outer(x, y, trans);
     }
-    // level-0 input/output alias the caller's vectors in FP64
-    V* x0save = L0.x2l;
-    V* y0save = L0.y2l;
+    // level-0 input/output alias the caller's vectors in FP64; the level's
+    // own buffers are restored on every way out, a thrown HIP error included
+    struct Restore {
+      DLevel<V>& L;
+      V* x;
+      V* y;
+      ~Restore() {
+        L.x2l = x;
+        L.y2l = y;
+      }
+    } restore{L0, L0.x2l, L0.y2l};
     L0.x2l = const_cast<V*>(in_vec(x, L0.x2l, L0.n));
     L0.y2l = out_vec(y, L0.y2l);
     int rc = inner_ml_aply(0, trans);
     if (y && !out_done_) finish_out(y, L0.y2l, L0.n);
-    L0.x2l = x0save;
-    L0.y2l = y0save;
     return rc;
   }
 

@@ -15,6 +15,8 @@ class DevPrec {
   virtual ~DevPrec() {}
   virtual int upload(const Prec& p) = 0;
   virtual int apply(const double* x, double* y, char trans) = 0;
+  // y = alpha op(M^-1) x + beta y (y not read when beta == 0)
+  virtual int apply_ab(double alpha, const double* x, double beta, double* y, char trans) = 0;
   virtual int spmv_level(int lev, int what, const double* x, double* y) = 0;
   virtual int smooth_level(int lev, const double* x, int beta, double* y, int sweeps, char trans, int which) = 0;
   // trans 'N': solve A x = b preconditioned by M; 'T': the adjoint system

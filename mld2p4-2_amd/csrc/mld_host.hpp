@@ -126,6 +126,16 @@ struct Level {
   SmootherData smd, smd2;     // smd2 used when sm2a.allocated
 };
 
+// One level handed over by the caller (mldhip_upload_level,
+// mldhip_upload_smoother, mldhip_load_hierarchy) and kept until
+// finalize_hierarchy moves it into precv.
+struct StagedLevel {
+  bool has_a = false;
+  HCsr a, p, r;
+  bool has_sm[2] = {false, false};  // pre smoother (sm), post smoother (sm2a)
+  SmootherData sd[2];
+};
+
 struct Prec {
   bool initialized = false;
   std::string ptype;
@@ -137,6 +147,7 @@ struct Prec {
   bool hier_built = false, smoothers_built = false;
   int value_kind = 0;         // 0 = FP64 hierarchy on device, 1 = FP32 (mixed)
   int dev_format = 0;         // 0 = autotuned, 1 = force CSR, 2 = force ELL
+  std::vector<StagedLevel> staged;
   std::string last_error;
 };
 
@@ -166,6 +177,29 @@ int stringval(const std::string& s);
 // build phases
 int hierarchy_build(Prec& p, int n, const int64_t* irp, const int* ja, const double* val);
 int smoothers_build(Prec& p);
+// The level array for nplevs levels with the configured parameters and
+// smoothers distributed as hierarchy_build does (base, intermediate, coarse).
+void resize_levels(std::vector<Level>& v, int nplevs);
+
+// A hierarchy built outside the engine: levels staged one at a time (0-based
+// lev; P is n_{lev-1} x n, R is n x n_{lev-1}; arrays copied, 0-based CSR
+// with strictly ascending columns), optionally the smoother data of each
+// level, then finalize_hierarchy places them in precv with the configured
+// parameters.  smoothers_done: every level's smoother data was staged (the
+// hierarchy is then ready for upload; else smoothers_build builds them).
+int stage_level(Prec& p, int lev, int n, const int64_t* irp, const int* ja, const double* val, int nf,
+                const int64_t* pirp, const int* pja, const double* pval, const int64_t* rirp, const int* rja,
+                const double* rval);
+int stage_smoother(Prec& p, int lev, int which, int solver, const int64_t* ndirp, const int* ndja,
+                   const double* ndval, const double* dv, const int64_t* lirp, const int* lja, const double* lval,
+                   const int64_t* uirp, const int* uja, const double* uval);
+int finalize_hierarchy(Prec& p, int nlev, bool& smoothers_done);
+// Levels 2.. of a prec%dump (<prefix>_p000_lLLL_{ac,p,r}.mtx, until a level's
+// ac file is missing) over the fine matrix given here; smoothers: also the
+// smoother files <prefix>_sm[2a]_p000_lLLL_{nd,diag,lower,upper}.mtx of every
+// level (a dump from level 1 with smoother and solver data).
+int load_hierarchy(Prec& p, const std::string& prefix, int n, const int64_t* irp, const int* ja, const double* val,
+                   bool smoothers, bool& smoothers_done, std::string& err);
 
 // building blocks, exposed for parity tests through the C-ABI
 int dec_map_bld(int iorder, double theta, const HCsr& a, std::vector<int>& ilaggr, int& naggr);

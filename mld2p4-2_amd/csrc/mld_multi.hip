@@ -692,17 +692,23 @@ static int append_coarse_operator(HCsr& mc, const Level& L, int64_t co, std::str
   if (const char* s = std::getenv("OMP_NUM_THREADS")) nt = std::atoi(s);
   nt = std::max(1, std::min({nt, 64, (n + 15) / 16}));
   std::vector<int> rcs(nt, 0);
+  // an exception (bad_alloc) must not leave a worker thread: it is turned
+  // into the thread's return code and reported after the join
   auto work = [&](int t) {
-    std::vector<double> e(n, 0.0), y;
-    for (int j = t; j < n; j += nt) {
-      e[j] = 1.0;
-      const int rc = coarse_apply_host(sd, L.parms.sweeps_pre, lu, piv, e, y);
-      e[j] = 0.0;
-      if (rc) {
-        rcs[t] = rc;
-        return;
+    try {
+      std::vector<double> e(n, 0.0), y;
+      for (int j = t; j < n; j += nt) {
+        e[j] = 1.0;
+        const int rc = coarse_apply_host(sd, L.parms.sweeps_pre, lu, piv, e, y);
+        e[j] = 0.0;
+        if (rc) {
+          rcs[t] = rc;
+          return;
+        }
+        std::copy(y.begin(), y.end(), mcol.begin() + (size_t)j * n);
       }
-      std::copy(y.begin(), y.end(), mcol.begin() + (size_t)j * n);
+    } catch (const std::exception&) {
+      rcs[t] = E_INTERNAL;
     }
   };
   std::vector<std::thread> pool;
@@ -711,7 +717,8 @@ static int append_coarse_operator(HCsr& mc, const Level& L, int64_t co, std::str
   for (auto& th : pool) th.join();
   for (int t = 0; t < nt; ++t)
     if (rcs[t]) {
-      err = "coarsest-level solver not supported by the batched engines";
+      err = rcs[t] == E_INTERNAL ? "host memory exhausted while forming a coarsest-level operator"
+                                 : "coarsest-level solver not supported by the batched engines";
       return rcs[t];
     }
   const size_t o = mc.ja.size(), ro = mc.irp.size();

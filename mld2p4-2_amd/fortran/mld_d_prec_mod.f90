@@ -49,6 +49,7 @@ module mld_d_prec_mod
     procedure, pass(prec) :: hierarchy_build => mld_d_hierarchy_bld
     procedure, pass(prec) :: smoothers_build => mld_d_smoothers_bld
     procedure, pass(prec) :: build => mld_dprecbld
+    procedure, pass(prec) :: hierarchy_load => mld_d_hierarchy_load
     procedure, pass(prec) :: apply2v => mld_dprecaply2
     procedure, pass(prec) :: apply1v => mld_dprecaply1
     procedure, pass(prec) :: apply_dev => mld_dprecaply_dev
@@ -308,6 +309,37 @@ contains
     info = mldhip_smoothers_build(prec%h)
     call report(prec, info, 'smoothers_build')
   end subroutine mld_d_smoothers_bld
+
+  ! The hierarchy of a prec%dump (levels 2.. from <prefix>_p000_lLLL_
+  ! {ac,p,r}.mtx) over the fine matrix a: the state after hierarchy_build.
+  ! smoothers = .true.: the smoother files of every level too (a dump from
+  ! level 1 with smoother and solver data), the state after smoothers_build.
+  subroutine mld_d_hierarchy_load(a, desc_a, prec, prefix, info, smoothers)
+    type(psb_dspmat_type), intent(in) :: a
+    type(psb_desc_type), intent(in) :: desc_a
+    class(mld_dprec_type), intent(inout) :: prec
+    character(len=*), intent(in) :: prefix
+    integer(psb_ipk_), intent(out) :: info
+    logical, optional, intent(in) :: smoothers
+    integer(c_int64_t), allocatable :: irp0(:)
+    integer(c_int), allocatable :: ja0(:)
+    integer(c_int) :: sm
+    if (.not. c_associated(prec%h)) then
+      info = 3111
+      return
+    end if
+    sm = 0
+    if (present(smoothers)) then
+      if (smoothers) sm = 1
+    end if
+    allocate(irp0(a%m+1), ja0(size(a%ja)))
+    irp0 = a%irp - 1
+    ja0 = a%ja - 1
+    prec%n = a%m
+    info = mldhip_load_hierarchy(prec%h, trim(prefix)//c_null_char, a%m, irp0, ja0, a%val, sm)
+    call report(prec, info, 'hierarchy_load')
+    if (desc_a%n /= a%m .and. desc_a%n /= 0) info = -1
+  end subroutine mld_d_hierarchy_load
 
   ! mlprec/impl/mld_dprecbld.f90:135,143
   subroutine mld_dprecbld(a, desc_a, prec, info)

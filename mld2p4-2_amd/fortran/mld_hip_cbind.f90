@@ -164,6 +164,53 @@ module mld_hip_cbind
       import :: c_int
       integer(c_int) :: e
     end function hipDeviceSynchronize
+    ! a hierarchy built outside the engine (the reference's precv after its
+    ! own hierarchy_build / smoothers_build): 0-based level index and CSR;
+    ! optional arrays are passed as c_null_ptr through the c_ptr arguments
+    function mldhip_upload_level(p, lev, n, irp, ja, val, nf, pirp, pja, pval, rirp, rja, rval) &
+         & bind(c, name='mldhip_upload_level') result(info)
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: p
+      integer(c_int), value :: lev, n, nf
+      integer(c_int64_t) :: irp(*)
+      integer(c_int) :: ja(*)
+      real(c_double) :: val(*)
+      type(c_ptr), value :: pirp, pja, pval, rirp, rja, rval
+      integer(c_int) :: info
+    end function mldhip_upload_level
+    function mldhip_upload_smoother(p, lev, which, solver, ndirp, ndja, ndval, dv, lirp, lja, lval, &
+         & uirp, uja, uval) bind(c, name='mldhip_upload_smoother') result(info)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: p
+      integer(c_int), value :: lev, which, solver
+      type(c_ptr), value :: ndirp, ndja, ndval, dv, lirp, lja, lval, uirp, uja, uval
+      integer(c_int) :: info
+    end function mldhip_upload_smoother
+    function mldhip_finalize_hierarchy(p, nlev) bind(c, name='mldhip_finalize_hierarchy') result(info)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: p
+      integer(c_int), value :: nlev
+      integer(c_int) :: info
+    end function mldhip_finalize_hierarchy
+    function mldhip_load_hierarchy(p, prefix, n, irp, ja, val, smoothers) &
+         & bind(c, name='mldhip_load_hierarchy') result(info)
+      import :: c_ptr, c_int, c_int64_t, c_double, c_char
+      type(c_ptr), value :: p
+      character(kind=c_char), dimension(*) :: prefix
+      integer(c_int), value :: n, smoothers
+      integer(c_int64_t) :: irp(*)
+      integer(c_int) :: ja(*)
+      real(c_double) :: val(*)
+      integer(c_int) :: info
+    end function mldhip_load_hierarchy
+    ! mld_dmlprec_aply_vect(alpha, prec, x, beta, y, ...): device vectors
+    function mldhip_mlprec_aply(p, alpha, x, beta, y, trans) bind(c, name='mldhip_mlprec_aply') result(info)
+      import :: c_ptr, c_int, c_char, c_double
+      type(c_ptr), value :: p, x, y
+      real(c_double), value :: alpha, beta
+      character(kind=c_char), value :: trans
+      integer(c_int) :: info
+    end function mldhip_mlprec_aply
   end interface
 
   integer(c_int), parameter :: hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2

@@ -1,9 +1,10 @@
 // ============================================================================
 // ORACLE — TEST INFRASTRUCTURE ONLY.
 //
-// A sequential CPU restatement of MLD2P4 2.1's smoothed-aggregation AMG
-// preconditioner (build + apply) and of the PSBLAS kernels its apply path
-// calls.  It exists to CHECK the MI355X product library; it is never linked
+// A CPU restatement of MLD2P4 2.1's smoothed-aggregation AMG preconditioner
+// (build + apply) and of the PSBLAS kernels its apply path calls; sequential
+// except for the row loops of products and vector updates, which OpenMP
+// threads split without changing any row's order of evaluation.  It exists to CHECK the MI355X product library; it is never linked
 // into, called by, or shipped as part of that library.  Only tests/,
 // __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it.
 //
@@ -32,7 +33,10 @@
 //   CG/FCG/BiCGStab     : numpy.linalg.solve, golden mld_sol.mtx vector
 //   PSBLAS last-ulp orders (spspmm, spnmi, spsm): parity unpinned.
 // ============================================================================
+#include <omp.h>
+
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -106,9 +110,16 @@ namespace {
 // PSBLAS-semantics kernels (PSBLAS order assumed).
 // ---------------------------------------------------------------------------
 
+// Row loops of the products and elementwise updates are split over OpenMP
+// threads (rows of at least PAR_MIN entries): each row is still summed alone,
+// in stored order, so every result is the same whatever the thread count.
+// Reductions (dot) and substitutions stay sequential.
+constexpr ix PAR_MIN = 32768;
+
 // psb_spmm(alpha, A, x, beta, y): y_i = beta*y_i + alpha*sum_j a_ij x_j, the
 // row sum accumulated in stored order starting from zero.
 void spmv(const Csr& a, double alpha, const double* x, double beta, double* y) {
+#pragma omp parallel for schedule(static) if (a.nr > PAR_MIN)
   for (ix i = 0; i < a.nr; ++i) {
     double acc = 0.0;
     for (ix k = a.irp[i]; k < a.irp[i + 1]; ++k) acc = acc + a.val[k] * x[a.ja[k]];
@@ -136,6 +147,7 @@ void spmv_t(const Csr& a, double alpha, const double* x, double beta, double* y)
 
 // psb_geaxpby(alpha, x, beta, y): y = alpha*x + beta*y.
 void axpby(ix n, double alpha, const double* x, double beta, double* y) {
+#pragma omp parallel for schedule(static) if (n > PAR_MIN)
   for (ix i = 0; i < n; ++i) {
     if (beta == 0.0) y[i] = (alpha == 1.0) ? x[i] : alpha * x[i];
     else if (alpha == 1.0 && beta == 1.0) y[i] = x[i] + y[i];
@@ -146,6 +158,7 @@ void axpby(ix n, double alpha, const double* x, double beta, double* y) {
 // psb_d_vect%mlt(alpha, d, x, beta): y = alpha*d*x + beta*y, y not read when
 // beta == 0 (mld_d_diag_solver_apply_vect.f90:102).
 void vmlt(ix n, double alpha, const double* d, const double* x, double beta, double* y) {
+#pragma omp parallel for schedule(static) if (n > PAR_MIN)
   for (ix i = 0; i < n; ++i) {
     double t = (alpha == 1.0) ? d[i] * x[i] : alpha * d[i] * x[i];
     if (beta == 0.0) y[i] = t;
@@ -615,6 +628,13 @@ struct Smoother {
   std::vector<double> dv;  // diag solver: 1/diag (0 -> 1); ILU: 1/pivot
   Csr l, u;                // ILU(0) factors
   DenseLU dlu;
+  // scratch vectors of the applications, kept between calls (every use
+  // writes them in full before reading them)
+  mutable std::vector<double> sw, st, stx, sty;
+  static double* scratch(std::vector<double>& v, ix n) {
+    if ((ix)v.size() < n) v.resize(n);
+    return v.data();
+  }
   // GS and BWGS are iterative (mld_d_gs_solver.f90:586-591): the jacobi
   // smoother then always runs its sweep loop; diag, ILU and LU are not.
   bool iterative() const { return kind == SM_GS || kind == SM_BWGS; }
@@ -717,23 +737,25 @@ struct Smoother {
   void sv_apply(double alpha, const double* x, double beta, double* y, char trans, ix n) const {
     if (kind == SM_NOPREC) { axpby(n, alpha, x, beta, y); return; }
     if (kind == SM_JAC || kind == SM_L1JAC) { vmlt(n, alpha, dv.data(), x, beta, y); return; }
-    std::vector<double> w(n), t(n);
+    double* w = scratch(sw, n);
+    double* t = scratch(st, n);
     if (kind == SM_DENSELU) {
-      dlu.solve(x, t.data());
-      axpby(n, alpha, t.data(), beta, y);
+      dlu.solve(x, t);
+      axpby(n, alpha, t, beta, y);
       return;
     }
     // mld_d_ilu_solver_apply_vect.f90:130-142
     if (trans == 'N') {
-      trsv_unit(l, true, x, t.data());
+      trsv_unit(l, true, x, t);
+#pragma omp parallel for schedule(static) if (n > PAR_MIN)
       for (ix i = 0; i < n; ++i) w[i] = dv[i] * t[i];
-      trsv_unit(u, false, w.data(), t.data());
+      trsv_unit(u, false, w, t);
     } else {
-      trsv_unit_t(u, false, x, t.data());
+      trsv_unit_t(u, false, x, t);
       for (ix i = 0; i < n; ++i) w[i] = dv[i] * t[i];
-      trsv_unit_t(l, true, w.data(), t.data());
+      trsv_unit_t(l, true, w, t);
     }
-    axpby(n, alpha, t.data(), beta, y);
+    axpby(n, alpha, t, beta, y);
   }
 
   // mld_d_jac_smoother_apply_vect.f90:38-215 with init='Z'
@@ -757,15 +779,16 @@ struct Smoother {
       return 0;
     }
     if (sweeps < 0) return -2;
-    std::vector<double> tx(n), ty(n);
-    sv_apply(1.0, x, 0.0, ty.data(), trans, n);  // init 'Z' (:136)
+    double* tx = scratch(stx, n);
+    double* ty = scratch(sty, n);
+    sv_apply(1.0, x, 0.0, ty, trans, n);  // init 'Z' (:136)
     for (int i = 1; i <= sweeps - 1; ++i) {      // :161-175
-      axpby(n, 1.0, x, 0.0, tx.data());
-      if (trans == 'N') spmv(nd, -1.0, ty.data(), 1.0, tx.data());
-      else spmv_t(nd, -1.0, ty.data(), 1.0, tx.data());
-      sv_apply(1.0, tx.data(), 0.0, ty.data(), trans, n);
+      axpby(n, 1.0, x, 0.0, tx);
+      if (trans == 'N') spmv(nd, -1.0, ty, 1.0, tx);
+      else spmv_t(nd, -1.0, ty, 1.0, tx);
+      sv_apply(1.0, tx, 0.0, ty, trans, n);
     }
-    axpby(n, alpha, ty.data(), beta, y);  // :177
+    axpby(n, alpha, ty, beta, y);  // :177
     return 0;
   }
 };
@@ -1082,8 +1105,18 @@ struct Prec {
 // PSBLAS's source is not in the reference tree: textbook formulations with
 // istop=2 (||r||_2/||b||_2 <= eps, checked after each iteration).
 // ---------------------------------------------------------------------------
+// it_times (may be null): wall-clock seconds of each iteration
+// (std::chrono::steady_clock), itmax entries.
 int krylov(Prec& pr, const Csr& a, int method, const double* b, double* x, double eps, int itmax,
-           int* iters, double* err) {
+           int* iters, double* err, double* it_times = nullptr) {
+  using clk = std::chrono::steady_clock;
+  auto tick = clk::now();
+  auto lap = [&](int k) {
+    if (!it_times || k < 1 || k > itmax) return;
+    const auto now = clk::now();
+    it_times[k - 1] = std::chrono::duration<double>(now - tick).count();
+    tick = now;
+  };
   ix n = a.nr;
   std::vector<double> r(n), z(n), p(n), q(n), w(n), s(n), t(n), rt(n), ph(n), sh(n), zold(n);
   double bn2 = nrm2(n, b);
@@ -1096,6 +1129,8 @@ int krylov(Prec& pr, const Csr& a, int method, const double* b, double* x, doubl
   if (method == 0) {  // CG (Saad 2003, Alg. 9.1)
     double rho = 0.0;
     while (it < itmax) {
+      if (it == 0) tick = clk::now();
+      else lap(it);
       ++it;
       if (pr.apply(r.data(), z.data(), 'N')) return -1;
       double rho_old = rho;
@@ -1117,6 +1152,8 @@ int krylov(Prec& pr, const Csr& a, int method, const double* b, double* x, doubl
   } else if (method == 1) {  // FCG(1), Notay SISC 22(4) 2000: p_k = z_k - (z_k,q_{k-1})/(p_{k-1},q_{k-1}) p_{k-1}
     double delta = 0.0;
     while (it < itmax) {
+      if (it == 0) tick = clk::now();
+      else lap(it);
       ++it;
       if (pr.apply(r.data(), z.data(), 'N')) return -1;
       if (it == 1) axpby(n, 1.0, z.data(), 0.0, p.data());
@@ -1137,6 +1174,8 @@ int krylov(Prec& pr, const Csr& a, int method, const double* b, double* x, doubl
     axpby(n, 1.0, r.data(), 0.0, rt.data());
     double rho = 0.0, rho_old = 1.0, alpha = 1.0, omega = 1.0;
     while (it < itmax) {
+      if (it == 0) tick = clk::now();
+      else lap(it);
       ++it;
       rho = dot(n, rt.data(), r.data());
       if (rho == 0.0) break;
@@ -1176,6 +1215,7 @@ int krylov(Prec& pr, const Csr& a, int method, const double* b, double* x, doubl
   } else {
     return -3;
   }
+  lap(it);
   *iters = it;
   return 0;
 }
@@ -1203,6 +1243,12 @@ struct orc_csr {  // caller-visible view of a CSR held by the oracle
   const int* ja;
   const double* val;
 };
+
+// Threads of the row loops (<= 0: OpenMP's default); returns the count in use.
+int orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
 
 void* orc_gen_pde3d(int idim) { return new Csr(gen_pde3d(idim)); }
 void* orc_csr_new(int64_t n, int64_t nc, const int64_t* irp, const int* ja, const double* val) {
@@ -1305,6 +1351,23 @@ int orc_prec_level_cycle(void* h, int lev, const double* x, double* y, char tran
 int orc_krylov(void* h, void* a, int method, const double* b, double* x, double eps, int itmax, int* iters,
                double* err) {
   return krylov(*(Prec*)h, *(Csr*)a, method, b, x, eps, itmax, iters, err);
+}
+// As orc_krylov, with the wall-clock time of each iteration in it_times
+// (itmax entries) and of the whole call in *total (steady_clock; the CPU
+// baseline's timers).
+int orc_krylov_timed(void* h, void* a, int method, const double* b, double* x, double eps, int itmax, int* iters,
+                     double* err, double* it_times, double* total) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int rc = krylov(*(Prec*)h, *(Csr*)a, method, b, x, eps, itmax, iters, err, it_times);
+  *total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+// Hierarchy and smoother build timed with steady_clock (seconds in *secs).
+int orc_prec_build_timed(void* h, void* a, double* secs) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int rc = ((Prec*)h)->build(*(Csr*)a);
+  *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
 }
 
 }  // extern "C"

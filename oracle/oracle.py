@@ -74,6 +74,8 @@ def lib():
             build()
         L = C.CDLL(_LIB_PATH)
         vp, dp, ip, i64p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int64)
+        L.orc_set_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_gen_pde3d.restype = vp
         L.orc_gen_pde3d.argtypes = [C.c_int]
         L.orc_csr_new.restype = vp
@@ -104,6 +106,8 @@ def lib():
         L.orc_prec_smooth_w.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, C.c_double, dp, C.c_char, C.c_int]
         L.orc_prec_level_cycle.argtypes = [vp, C.c_int, dp, dp, C.c_char]
         L.orc_krylov.argtypes = [vp, vp, C.c_int, dp, dp, C.c_double, C.c_int, ip, dp]
+        L.orc_krylov_timed.argtypes = [vp, vp, C.c_int, dp, dp, C.c_double, C.c_int, ip, dp, dp, dp]
+        L.orc_prec_build_timed.argtypes = [vp, vp, dp]
         _lib = L
     return _lib
 
@@ -172,6 +176,12 @@ class Csr:
             y = np.zeros(nc if trans else nr)
         (lib().orc_spmv_t if trans else lib().orc_spmv)(self.h, alpha, _dp(x), beta, _dp(y))
         return y
+
+
+def set_threads(n=0):
+    """Threads of the oracle's row loops (0: OpenMP's default); returns the
+    count in use.  Results do not depend on it."""
+    return lib().orc_set_threads(int(n))
 
 
 def gen_pde3d(idim):
@@ -254,9 +264,11 @@ class Prec:
 
     def build(self, a: Csr):
         self.a = a
-        rc = lib().orc_prec_build(self.h, a.h)
+        t = C.c_double(0.0)
+        rc = lib().orc_prec_build_timed(self.h, a.h, C.byref(t))
         if rc:
             raise RuntimeError(f"oracle build failed: {rc}")
+        self.build_s = t.value  # hierarchy + smoothers, steady_clock seconds
         return self
 
     @property
@@ -315,15 +327,21 @@ class Prec:
             raise RuntimeError(f"oracle smooth failed: {rc}")
         return y
 
-    def krylov(self, method, b, x0=None, eps=1e-6, itmax=500):
+    def krylov(self, method, b, x0=None, eps=1e-6, itmax=500, timings=False):
+        """(x, iterations, relres); with timings=True also (per-iteration
+        seconds, whole-call seconds), both from steady_clock."""
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
         it = C.c_int(0)
         err = C.c_double(0.0)
-        rc = lib().orc_krylov(self.h, self.a.h, KRYLOV[method.upper()], _dp(b), _dp(x), eps, itmax,
-                              C.byref(it), C.byref(err))
+        its = np.zeros(max(itmax, 1))
+        tot = C.c_double(0.0)
+        rc = lib().orc_krylov_timed(self.h, self.a.h, KRYLOV[method.upper()], _dp(b), _dp(x), eps, itmax,
+                                    C.byref(it), C.byref(err), _dp(its), C.byref(tot))
         if rc:
             raise RuntimeError(f"oracle krylov failed: {rc}")
+        if timings:
+            return x, it.value, err.value, its[:it.value].copy(), tot.value
         return x, it.value, err.value
 
 

@@ -69,6 +69,29 @@ def test_host_build_bitexact_vs_oracle(eng, orc, idim, smoother):
             assert (p.dv(l, which=1) == po.dv(l)).all()
 
 
+def test_host_build_filtered_prolongator_vs_oracle(eng, orc):
+    """AGGR_FILTER = FILTER on a heterogeneous diffusion matrix whose weak
+    couplings the filter drops: aggregation, P, R and A_c equal the oracle's
+    bit for bit."""
+    from mld2p4_amd import problems
+
+    irp, ja, val, _ = problems.diffusion3d(10, seed=11, passes=0)
+    a = orc.Csr.from_arrays(irp, ja, val)
+    p = config1(eng, "JACOBI", 1, mcs=20)
+    p.set("AGGR_THRESH", 0.1)
+    p.set("AGGR_FILTER", "FILTER")
+    p.build((irp, ja, val))
+    po = orc.Prec(orc.config1_params(theta=0.1, min_coarse_size=20, aggr_filter=1)).build(a)
+    pu = orc.Prec(orc.config1_params(theta=0.1, min_coarse_size=20)).build(a)
+    assert p.nlev == po.nlev >= 3
+    for l in range(1, p.nlev):
+        assert (p.aggr(l) == po.ilaggr(l)).all()
+        for w in ("P", "R", "A"):
+            assert same(p.level_csr(l, w), po.level_csr(l, w).to_scipy()), (l, w)
+        assert p.omega(l) == po.omega(l)
+    assert not same(po.level_csr(1, "P").to_scipy(), pu.level_csr(1, "P").to_scipy())
+
+
 def test_one_level_fbgs_build(eng, orc):
     """init('FBGS'): one level, GS before and BWGS after, the split of A as the
     oracle's one-level build (MAX_LEVS 1) makes it."""

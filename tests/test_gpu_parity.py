@@ -248,6 +248,8 @@ def test_krylov_default_ml(eng, orc):
 
 @pytest.mark.parametrize("cycle,smoother,sweeps,idim,mcs", [
     ("ADD", "JACOBI", 1, 16, 40), ("ADD", "FBGS", 1, 16, 40), ("MULT", "L1-JACOBI", 2, 16, 40),
+    ("WCYCLE", "JACOBI", 1, 16, 40), ("WCYCLE", "L1-JACOBI", 2, 24, 40), ("WCYCLE", "FBGS", 1, 20, 40),
+    ("WCYCLE", "BJAC", 1, 20, 40),
     ("KCYCLE", "JACOBI", 2, 24, 40), ("KCYCLESYM", "JACOBI", 2, 24, 40),
     ("KCYCLE", "NOPREC", 1, 12, 60), ("KCYCLESYM", "NOPREC", 1, 12, 60), ("KCYCLESYM", "FBGS", 1, 30, 40)])
 def test_cycles_vs_oracle(eng, orc, cycle, smoother, sweeps, idim, mcs):
@@ -272,6 +274,26 @@ def test_cycles_vs_oracle(eng, orc, cycle, smoother, sweeps, idim, mcs):
         x = rng.standard_normal(A.shape[0])
         y = p.apply(dev(x), trans="T").cpu().numpy()
         assert relmax(y, po.apply(x, trans="T")) <= 1e-11
+
+
+@pytest.mark.parametrize("k,precision", [(2, "double"), (4, "double"), (3, "mixed")])
+def test_outer_sweeps_vs_oracle(eng, orc, k, precision):
+    """OUTER_SWEEPS = k (mld_dmlprec_aply.f90:285-327: k cycles on the
+    residual of the running sum): within 1e-12 of the oracle's application
+    in FP64 (1e-5 with the FP32 hierarchy), 'N' and 'T'; CG converges."""
+    _say(f"[outer sweeps] k={k} {precision} idim=16")
+    a = orc.gen_pde3d(16)
+    A = a.to_scipy()
+    p = engine_prec(eng, A, "L1-JACOBI", 2, mcs=40, precision=precision, extra=(("OUTER_SWEEPS", k),))
+    po = oracle_prec(orc, a, "L1-JACOBI", 2, mcs=40, outer_sweeps=k)
+    x = np.random.default_rng(40 + k).standard_normal(A.shape[0])
+    tol = 1e-12 if precision == "double" else 1e-5
+    for trans in ("N", "T"):
+        assert relmax(p.apply(dev(x), trans=trans).cpu().numpy(), po.apply(x, trans=trans)) <= tol, trans
+    b = np.ones(A.shape[0])
+    xs, it, err = eng.psb_krylov("CG", A, p, dev(b), eps=1e-8)
+    assert err <= 1e-8
+    assert np.linalg.norm(b - A @ xs.cpu().numpy()) / np.linalg.norm(b) <= 1.01e-8
 
 
 def test_kcycle_mixed_and_krylov(eng, orc):
@@ -403,29 +425,61 @@ def test_diffusion_mixed_fcg(eng, orc):
     xg = x.cpu().numpy()
     rel = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
     print(f"mixed FCG iters {it} (oracle FP64 CG to 1e-12: {ito}); solution error vs oracle {rel:.2e}")
-    assert rel <= 1e-6
+    assert rel <= 1e-7
 
 
 def test_host_vector_entry_points(eng, orc):
     """mldhip_apply_host / mldhip_krylov_host (the Fortran driver's path):
-    copies through pinned staging into device vectors the handle keeps; the
-    results equal the device-vector entry points bit for bit, on repeated
-    calls and after the handle is rebuilt for a larger system (the kept
-    vectors grow; at idim 164 a vector is 35 MB, two chunks of the 32 MB
-    staging buffers)."""
-    _say("[host vectors] idim=24, then 164")
+    copies through pinned staging into device vectors the handle keeps.  One
+    handle is built for idim 24 and then rebuilt for idim 164 (the kept
+    vectors grow: freed and allocated again on the same handle; a vector is
+    35 MB, two chunks of the 32 MB staging buffers).  After each build the
+    host-vector results equal the device-vector ones bit for bit on repeated
+    calls, the application equals the oracle's within 1e-12, and CG reaches
+    1e-8 with the true residual checked on the host."""
+    _say("[host vectors] one handle, idim 24 then 164")
+    p = eng.MldPrec(device=0).init("ML")
+    for k, v in [("SMOOTHER_TYPE", "L1-JACOBI"), ("SMOOTHER_SWEEPS", 2), ("AGGR_THRESH", 0.0),
+                 ("AGGR_PROL", "SMOOTHED"), ("MIN_COARSE_SIZE", 200), ("COARSE_SOLVE", "BJAC"),
+                 ("COARSE_SUBSOLVE", "ILU"), ("COARSE_SWEEPS", 4)]:
+        p.set(k, v)
     for idim in (24, 164):
         a = orc.gen_pde3d(idim)
         A = a.to_scipy()
-        p = engine_prec(eng, A, "L1-JACOBI", 2)
+        p.hierarchy_build(A)
+        p.smoothers_build()
+        po = oracle_prec(orc, a, "L1-JACOBI", 2)
         n = A.shape[0]
         x = np.random.default_rng(idim).standard_normal(n)
         for _ in range(2):
             yh = p.apply(x)
             yd = p.apply(dev(x)).cpu().numpy()
             assert (yh == yd).all()
+        assert relmax(yh, po.apply(x)) <= 1e-12
         b = np.ones(n)
         xh, ith, erh = eng.psb_krylov("CG", A, p, b, eps=1e-8)
         xd, itd, erd = eng.psb_krylov("CG", A, p, dev(b), eps=1e-8)
         assert ith == itd and erh == erd
         assert (xh == xd.cpu().numpy()).all()
+        assert erh <= 1e-8
+        assert np.linalg.norm(b - A @ xh) / np.linalg.norm(b) <= 1.01e-8
+
+
+def test_apply_after_error_return(eng, orc):
+    """An application that returns an error (the dense-LU coarsest solver
+    refuses 'T') leaves the handle usable: the next 'N' application equals the
+    oracle's within 1e-12, also with the FP32 hierarchy in FCG."""
+    _say("[apply after error] idim=14")
+    a = orc.gen_pde3d(14)
+    A = a.to_scipy()
+    po = oracle_prec(orc, a, "JACOBI", 2, coarse="DENSE_LU", csweeps=1, mcs=600)
+    x = np.random.default_rng(31).standard_normal(A.shape[0])
+    yo = po.apply(x)
+    for prec, tol in (("double", 1e-12), ("mixed", 1e-5)):
+        p = engine_prec(eng, A, "JACOBI", 2, coarse="DENSE_LU", csweeps=1, mcs=600, precision=prec)
+        with pytest.raises(eng.MldError):
+            p.apply(dev(x), trans="T")
+        assert relmax(p.apply(dev(x)).cpu().numpy(), yo) <= tol
+        b = np.ones(A.shape[0])
+        _, it, err = eng.psb_krylov("FCG", A, p, dev(b), eps=1e-8)
+        assert err <= 1e-8

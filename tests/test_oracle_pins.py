@@ -336,6 +336,98 @@ def test_two_level_error_propagation(orc):
     assert np.allclose(p.apply(x), M @ x, rtol=1e-9, atol=1e-14 * np.abs(M @ x).max())
 
 
+def _dense_apply(p, n, trans="N"):
+    return np.column_stack([p.apply(e, trans=trans) for e in np.eye(n)])
+
+
+def test_three_level_w_cycle_error_propagation(orc):
+    """W-cycle (mld_dmlprec_aply.f90:716-745) on three levels, Jacobi 1/1,
+    exact coarsest solve.  With Galerkin coarse operators two coarse
+    corrections through a level whose own cycle has error propagator E_c are
+    I - P (I - E_c^2) A_c^-1 R A, so
+      E_1 = S_1 (I - P_2 A_2^-1 R_2 A_1) S_1,
+      I - M A = S_0 (I - P_1 (I - E_1^2) A_1^-1 R_1 A_0) S_0,
+    with S_l = I - D_l^-1 A_l; a V-cycle would have I - E_1 in place of
+    I - E_1^2.  Checked on M formed column by column from the oracle."""
+    a = orc.gen_pde3d(7)
+    prm = orc.config1_params(min_coarse_size=8, c_smoother=orc.SM_DENSELU, c_sweeps=1, ml_cycle=orc.CYC_W)
+    p = orc.Prec(prm).build(a)
+    assert p.nlev == 3
+    A = [p.level_csr(l, "A").to_scipy().toarray() for l in range(3)]
+    P = [None] + [p.level_csr(l, "P").to_scipy().toarray() for l in (1, 2)]
+    R = [None] + [p.level_csr(l, "R").to_scipy().toarray() for l in (1, 2)]
+    S = [np.eye(len(Al)) - np.diag(1.0 / np.diag(Al)) @ Al for Al in A]
+    n1 = len(A[1])
+    E1 = S[1] @ (np.eye(n1) - P[2] @ np.linalg.solve(A[2], R[2] @ A[1])) @ S[1]
+    n = len(A[0])
+    E0 = S[0] @ (np.eye(n) - P[1] @ (np.eye(n1) - E1 @ E1) @ np.linalg.solve(A[1], R[1] @ A[0])) @ S[0]
+    M = _dense_apply(p, n)
+    Mref = (np.eye(n) - E0) @ np.linalg.inv(A[0])
+    assert np.abs(M - Mref).max() <= 1e-10 * np.abs(Mref).max()
+    # the V-cycle of the same hierarchy differs (the W branch is taken)
+    pv = orc.Prec(orc.config1_params(min_coarse_size=8, c_smoother=orc.SM_DENSELU, c_sweeps=1)).build(a)
+    assert np.abs(_dense_apply(pv, n) - Mref).max() > 1e-6 * np.abs(Mref).max()
+    # symmetric A, symmetric smoothing: the W-cycle is symmetric and 'T' = 'N'
+    assert np.abs(M - M.T).max() <= 1e-10 * np.abs(M).max()
+    assert np.abs(_dense_apply(p, n, "T") - M).max() <= 1e-10 * np.abs(M).max()
+
+
+@pytest.mark.parametrize("k", [2, 4])
+def test_outer_sweeps_closed_form(orc, k):
+    """outer_sweeps = k (mld_dmlprec_aply.f90:285-327): k cycles, each on the
+    residual of the previous sum, so I - M_k A = (I - M A)^k, i.e.
+    M_k = (I - (I - M A)^k) A^-1 with M the single-sweep preconditioner."""
+    a = orc.gen_pde3d(6)
+    A = a.to_scipy().toarray()
+    n = len(A)
+    p1 = orc.Prec(orc.config1_params(min_coarse_size=20)).build(a)
+    pk = orc.Prec(orc.config1_params(min_coarse_size=20, outer_sweeps=k)).build(a)
+    M = _dense_apply(p1, n)
+    E = np.linalg.matrix_power(np.eye(n) - M @ A, k)
+    Mk = (np.eye(n) - E) @ np.linalg.inv(A)
+    x = np.random.default_rng(k).standard_normal(n)
+    y = pk.apply(x)
+    assert np.abs(y - Mk @ x).max() <= 1e-10 * np.abs(Mk @ x).max()
+    assert np.abs(y - M @ x).max() > 1e-6 * np.abs(y).max()
+
+
+def test_filtered_prolongator_dense(orc):
+    """AGGR_FILTER = FILTER (mld_daggrmat_smth_asb.f90:194-217, 258-291):
+    P = (I - omega D^-1 A_F) Ptilde, where A_F drops every entry with
+    |a_ij| < theta sqrt(|a_ii a_jj|) and subtracts their sum from its
+    diagonal, D = diag(A) and omega = 4 / (3 ||D^-1 A||_inf) come from the
+    unfiltered A; Ptilde from the oracle's (separately pinned) aggregation.
+    An unsmoothed heterogeneous diffusion field makes many couplings weak."""
+    from mld2p4_amd import problems
+
+    irp, ja, val, _ = problems.diffusion3d(6, seed=11, passes=0)
+    a = orc.Csr.from_arrays(irp, ja, val)
+    A = a.to_scipy().toarray()
+    n = len(A)
+    theta = 0.1
+    prm = orc.config1_params(theta=theta, min_coarse_size=10, max_levs=2, aggr_filter=1)
+    p = orc.Prec(prm).build(a)
+    d = np.diag(A).copy()
+    weak = (np.abs(A) < theta * np.sqrt(np.abs(np.outer(d, d)))) & (A != 0)
+    assert weak.sum() > n // 4  # the filter removes something
+    AF = np.where(weak, 0.0, A)
+    AF[np.arange(n), np.arange(n)] -= np.where(weak, A, 0.0).sum(axis=1)
+    Dinv = np.diag(1.0 / d)
+    omega = 4.0 / (3.0 * np.abs(Dinv @ A).sum(axis=1).max())
+    ilaggr = p.ilaggr(1)
+    Pt = np.zeros((n, ilaggr.max() + 1))
+    Pt[np.arange(n), ilaggr] = 1.0
+    Pref = (np.eye(n) - omega * Dinv @ AF) @ Pt
+    P = p.level_csr(1, "P").to_scipy().toarray()
+    assert abs(p.omega(1) - omega) <= 1e-14 * omega
+    assert np.abs(P - Pref).max() <= 1e-13 * np.abs(Pref).max()
+    # unfiltered: A itself in place of A_F, and the two differ
+    pu = orc.Prec(orc.config1_params(theta=theta, min_coarse_size=10, max_levs=2)).build(a)
+    Pu = pu.level_csr(1, "P").to_scipy().toarray()
+    assert np.abs(Pu - (np.eye(n) - omega * Dinv @ A) @ Pt).max() <= 1e-13 * np.abs(Pu).max()
+    assert np.abs(P - Pu).max() > 1e-8
+
+
 @pytest.mark.parametrize("sm", ["JAC", "L1JAC"])
 def test_vcycle_linear_symmetric_transpose(orc, sm):
     a = orc.gen_pde3d(12)
