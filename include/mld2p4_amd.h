@@ -61,6 +61,12 @@ int mldhip_set_precision(mldhip_prec* p, int kind);
 int mldhip_set_format(mldhip_prec* p, int fmt);
 
 /* ---- build -------------------------------------------------------------- */
+/* Where hierarchy_build forms the smoothed-aggregation products P, A P, R
+ * and Ac of each level (mld_daggrmat_smth_asb.f90:291,345,399; aggregation
+ * always runs on the host): 0 host threads (default), 1 the handle's device,
+ * -1 the device when the handle has one and MLD_SETUP is not "host".
+ * Both give bit-identical hierarchies.  No reference counterpart. */
+int mldhip_set_setup(mldhip_prec* p, int where);
 /* prec%hierarchy_build(a, desc, info): mlprec/impl/mld_d_hierarchy_bld.f90:66-490. */
 int mldhip_hierarchy_build(mldhip_prec* p, int n, const int64_t* irp, const int* ja, const double* val);
 /* prec%smoothers_build(a, desc, info, amold, vmold): mlprec/impl/mld_d_smoothers_bld.f90
@@ -219,6 +225,11 @@ int mldhip_get_aggr(mldhip_prec* p, int lev, int* ilaggr);
 /* Smoother diagonal data (1/diag, 1/l1-diag or ILU 1/pivot), length n. */
 int mldhip_get_level_dv(mldhip_prec* p, int lev, int which, double* dv);
 int mldhip_get_omega(mldhip_prec* p, int lev, double* omega);
+/* Number of level sets of the two triangular solves (lower, upper) of the
+ * smoother `which` (0 pre, 1 post) of level lev: the dependency depth of the
+ * ILU / Gauss-Seidel substitutions (0 when the smoother has none).  No
+ * reference counterpart (measurement). */
+int mldhip_get_tri_levels(mldhip_prec* p, int lev, int which, int* nl_lower, int* nl_upper);
 /* Operator complexity sum nnz(A_l)/nnz(A_1) (mld_d_prec_type.f90:465-497). */
 int mldhip_get_complexity(mldhip_prec* p, double* cplx);
 int mldhip_get_device_bytes(mldhip_prec* p, int64_t* bytes);

@@ -57,6 +57,7 @@ def lib():
             "mldhip_setr": [vp, cp, C.c_double, C.c_int, C.c_int, cp],
             "mldhip_set_precision": [vp, C.c_int],
             "mldhip_set_format": [vp, C.c_int],
+            "mldhip_set_setup": [vp, C.c_int],
             "mldhip_hierarchy_build": [vp, C.c_int, i64p, ip, dp],
             "mldhip_smoothers_build": [vp],
             "mldhip_build": [vp, C.c_int, i64p, ip, dp],
@@ -78,6 +79,7 @@ def lib():
             "mldhip_get_aggr": [vp, C.c_int, ip],
             "mldhip_get_level_dv": [vp, C.c_int, C.c_int, dp],
             "mldhip_get_omega": [vp, C.c_int, dp],
+            "mldhip_get_tri_levels": [vp, C.c_int, C.c_int, ip, ip],
             "mldhip_get_complexity": [vp, dp],
             "mldhip_dump": [vp, C.c_int, C.c_int, cp, cp, C.c_int],
             "mldhip_get_device_bytes": [vp, i64p],
@@ -197,6 +199,12 @@ class MldPrec:
 
     def set_format(self, fmt: str):
         self._check(lib().mldhip_set_format(self.h, {"auto": 0, "csr": 1, "ell": 2, "sell": 3, "sell256": 4, "sellu": 5, "dia": 6}[fmt]), "set_format")
+        return self
+
+    def set_setup(self, where: str):
+        """Where hierarchy_build forms the level products: 'host', 'device'
+        or 'auto' (the device when the handle has one)."""
+        self._check(lib().mldhip_set_setup(self.h, {"auto": -1, "host": 0, "device": 1}[where]), "set_setup")
         return self
 
     # -- build ----------------------------------------------------------------
@@ -374,6 +382,12 @@ class MldPrec:
         out = np.zeros(self.level_info(lev)["n"])
         self._check(lib().mldhip_get_level_dv(self.h, lev, which, _dp(out)), "get_level_dv")
         return out
+
+    def tri_levels(self, lev, which=0):
+        """(level sets of the lower solve, of the upper solve) of a smoother."""
+        a, b = C.c_int(), C.c_int()
+        self._check(lib().mldhip_get_tri_levels(self.h, lev, which, C.byref(a), C.byref(b)), "get_tri_levels")
+        return a.value, b.value
 
     def omega(self, lev):
         v = C.c_double()

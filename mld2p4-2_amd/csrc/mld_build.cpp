@@ -317,7 +317,7 @@ static void transpose(const HCsr& A, HCsr& T) {
 }
 
 int aggrmat_smth_asb(const HCsr& a, const std::vector<int>& ilaggr, int naggr, LevelParms& parms, HCsr& ac,
-                     HCsr& p, HCsr& r) {
+                     HCsr& p, HCsr& r, int device) {
   if (parms.aggr_prol != SMOOTH_PROL) return E_UNSUPPORTED;  // only SMOOTHED is provided
   const int n = a.nr;
   std::vector<double> dinv = diag_of(a);
@@ -383,11 +383,28 @@ int aggrmat_smth_asb(const HCsr& a, const std::vector<int>& ilaggr, int naggr, L
   } else {
     sv = CsrView{a.nr, a.nc, a.irp.data(), a.ja.data(), daval.data()};
   }
+  if (device >= 0 && !filter) {
+    // the products on the device (S formed there from A, D^-1 and omega)
+    std::string err;
+    const double t0 = wall_seconds();
+    const int rc = dev_smoothed_products(device, a, nullptr, dinv, omega, ilaggr, naggr, p, r, ac, err);
+    phase_log("P, AP, R, Ac on the device", -1, wall_seconds() - t0);
+    if (rc == 0) return 0;
+    phase_log(("device products not used: " + err).c_str(), -1, 0.0);
+  }
   double* svv = const_cast<double*>(sv.val);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i)
     for (int64_t k = sv.irp[i]; k < sv.irp[i + 1]; ++k)
       svv[k] = (sv.ja[k] == i) ? 1.0 - omega * svv[k] : -(omega * svv[k]);
+  if (device >= 0 && filter) {
+    std::string err;
+    const double t0 = wall_seconds();
+    const int rc = dev_smoothed_products(device, a, &s, dinv, omega, ilaggr, naggr, p, r, ac, err);
+    phase_log("P, AP, R, Ac on the device", -1, wall_seconds() - t0);
+    if (rc == 0) return 0;
+    phase_log(("device products not used: " + err).c_str(), -1, 0.0);
+  }
   // tentative prolongator: row i has a single 1.0 in column ilaggr[i]
   HCsr pt;
   pt.nr = n; pt.nc = naggr;
@@ -807,13 +824,13 @@ int hierarchy_build(Prec& p, int n, const int64_t* irp, const int* ja, const dou
       tgt.parms.aggr_omega_val = aomega;
       tgt.sm = coarse_sm; tgt.sm2a = coarse_sm2a;
       if (newsz < i) { ilaggr = tgt.ilaggr; naggr = tgt.naggr; }
-      rc = aggrmat_smth_asb(v[newsz - 2].a, ilaggr, naggr, tgt.parms, tgt.a, tgt.p, tgt.r);
+      rc = aggrmat_smth_asb(v[newsz - 2].a, ilaggr, naggr, tgt.parms, tgt.a, tgt.p, tgt.r, p.setup_device);
       if (rc) return rc;
       tgt.ilaggr = std::move(ilaggr);
       tgt.naggr = naggr;
       break;
     }
-    rc = aggrmat_smth_asb(prev.a, ilaggr, naggr, cur.parms, cur.a, cur.p, cur.r);
+    rc = aggrmat_smth_asb(prev.a, ilaggr, naggr, cur.parms, cur.a, cur.p, cur.r, p.setup_device);
     if (rc) return rc;
     cur.ilaggr = std::move(ilaggr);
     cur.naggr = naggr;

@@ -1,6 +1,7 @@
 // extern "C" layer: declarations in include/mld2p4_amd.h.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -14,6 +15,7 @@
 struct mldhip_prec {
   mld::Prec host;
   int device = -1;
+  int setup = 0;  // products of the hierarchy build: 0 host, 1 device, -1 device when there is one
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::unique_ptr<mld::DevPrec> dev;
@@ -128,10 +130,23 @@ int mldhip_set_format(mldhip_prec* p, int fmt) {
   return 0;
 }
 
+int mldhip_set_setup(mldhip_prec* p, int where) {
+  if (!p || where < -1 || where > 1) return E_BAD_ARG;
+  if (where == 1 && p->device < 0) return fail(p, E_BAD_ARG, "set_setup: a host-only handle has no device");
+  p->setup = where;
+  return 0;
+}
+
 int mldhip_hierarchy_build(mldhip_prec* p, int n, const int64_t* irp, const int* ja, const double* val) {
   if (!p || n < 0 || !irp || (n > 0 && (!ja || !val))) return E_BAD_ARG;
   p->dev.reset();
   p->blk.reset();
+  {
+    const char* e = std::getenv("MLD_SETUP");
+    const bool host_env = e && std::string(e) == "host";
+    const bool dev = p->device >= 0 && (p->setup == 1 || (p->setup == -1 && !host_env));
+    p->host.setup_device = dev ? p->device : -1;
+  }
   int rc = mld::hierarchy_build(p->host, n, irp, ja, val);
   if (rc) return fail(p, rc, p->host.last_error.empty() ? "hierarchy_build failed" : p->host.last_error);
   return 0;
@@ -381,6 +396,16 @@ int mldhip_get_level_dv(mldhip_prec* p, int lev, int which, double* dv) {
   const mld::Level& L = p->host.precv[lev];
   const auto& v = (which == 1 && L.sm2a.allocated) ? L.smd2.dv : L.smd.dv;
   if (!v.empty()) std::memcpy(dv, v.data(), v.size() * sizeof(double));
+  return 0;
+}
+
+int mldhip_get_tri_levels(mldhip_prec* p, int lev, int which, int* nl_lower, int* nl_upper) {
+  if (!p || !nl_lower || !nl_upper) return E_BAD_ARG;
+  if (lev < 0 || lev >= (int)p->host.precv.size()) return E_BAD_LEVEL;
+  const mld::Level& L = p->host.precv[lev];
+  const mld::SmootherData& sd = (which == 1 && L.sm2a.allocated) ? L.smd2 : L.smd;
+  *nl_lower = sd.lsched_ptr.empty() ? 0 : (int)sd.lsched_ptr.size() - 1;
+  *nl_upper = sd.usched_ptr.empty() ? 0 : (int)sd.usched_ptr.size() - 1;
   return 0;
 }
 

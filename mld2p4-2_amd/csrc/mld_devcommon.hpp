@@ -67,6 +67,12 @@ struct DSmoother {
   std::vector<int> ltptr, utptr;
   bool has_t = false;
   int tri_lanes = 1;  // lanes per row of the ILU triangular solves (1, 4, 16, 32)
+  // sync-free substitutions: chunks of each schedule (positions in its row
+  // list, at most BS/tri_lanes rows, never across level sets) and the chunk
+  // ticket
+  int *lch = nullptr, *uch = nullptr, *ltch = nullptr, *utch = nullptr;
+  int nlch = 0, nuch = 0, nltch = 0, nutch = 0;
+  int* sf_ticket = nullptr;
   // dense LU
   double* lu = nullptr; int* piv = nullptr;
   double* dinv = nullptr;  // DENSE_LU: explicit inverse (row-major)
@@ -280,6 +286,17 @@ static HCsr host_transpose(const HCsr& a) {
       t.ja[q] = i; t.val[q] = a.val[k];
     }
   return t;
+}
+
+// Chunks of a level schedule (ptr: level -> first position) for the
+// sync-free substitution: at most `per` consecutive positions of one level
+// set each; returns the chunk start positions plus the end.
+static std::vector<int> sched_chunks(const std::vector<int>& ptr, int per) {
+  std::vector<int> c;
+  for (size_t lv = 0; lv + 1 < ptr.size(); ++lv)
+    for (int q = ptr[lv]; q < ptr[lv + 1]; q += per) c.push_back(q);
+  c.push_back(ptr.empty() ? 0 : ptr.back());
+  return c;
 }
 
 // Blocks of the grid of a sparse launch (row- or lane-per-thread).

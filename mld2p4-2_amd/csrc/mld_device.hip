@@ -89,6 +89,9 @@ class DevPrecT : public DevPrec {
       }
     }
     n0_ = L_[0].n;
+    // the residual copy of the outer sweeps, allocated here so that no
+    // application allocates
+    if (outer_sweeps_ > 1 && nlev > 1) xres_ = ar_.alloc<V>((size_t)n0_);
     {
       // partial sums of every reducing launch: Krylov SpMV, vector reductions,
       // fused output of the fine level's post smoother
@@ -278,7 +281,9 @@ class DevPrecT : public DevPrec {
              double* err, char trans) override;
 
   int sync() override {
+    HIPCHECK(hipMemcpyAsync(hscal_, scal_, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, s_));
     HIPCHECK(hipStreamSynchronize(s_));
+    check_abort();
     return 0;
   }
 
@@ -483,6 +488,16 @@ class DevPrecT : public DevPrec {
       S.uptr = H.usched_ptr;
       S.lptr_d = ar_.upload(H.lsched_ptr.data(), H.lsched_ptr.size(), s_);
       S.uptr_d = ar_.upload(H.usched_ptr.data(), H.usched_ptr.size(), s_);
+      {
+        const int per = BS / S.tri_lanes;
+        const std::vector<int> lc = sched_chunks(S.lptr, per), uc = sched_chunks(S.uptr, per);
+        S.lch = ar_.upload(lc.data(), lc.size(), s_);
+        S.uch = ar_.upload(uc.data(), uc.size(), s_);
+        S.nlch = (int)lc.size() - 1;
+        S.nuch = (int)uc.size() - 1;
+        S.sf_ticket = ar_.alloc<int>(1);
+        HIPCHECK(hipMemsetAsync(S.sf_ticket, 0, sizeof(int), s_));
+      }
       HIPCHECK(hipStreamSynchronize(s_));
     }
     if (H.solver == DENSE_LU) {
@@ -554,6 +569,14 @@ class DevPrecT : public DevPrec {
           ds.utrows = ar_.upload(r1.data(), r1.size(), s_);
           ds.ltrows = ar_.upload(r2.data(), r2.size(), s_);
           ds.utptr = p1; ds.ltptr = p2;
+          {
+            const int per = BS / ds.tri_lanes;
+            const std::vector<int> c1 = sched_chunks(p1, per), c2 = sched_chunks(p2, per);
+            ds.utch = ar_.upload(c1.data(), c1.size(), s_);
+            ds.ltch = ar_.upload(c2.data(), c2.size(), s_);
+            ds.nutch = (int)c1.size() - 1;
+            ds.nltch = (int)c2.size() - 1;
+          }
           ds.utptr_d = ar_.upload(p1.data(), p1.size(), s_);
           ds.ltptr_d = ar_.upload(p2.data(), p2.size(), s_);
           HIPCHECK(hipStreamSynchronize(s_));
@@ -633,6 +656,15 @@ class DevPrecT : public DevPrec {
       HIPCHECK(hipGetLastError());
       return;
     }
+    if (sync_free()) {
+      const int* c1 = TR ? S.utch : S.lch;
+      const int* c2 = TR ? S.ltch : S.uch;
+      const int n1 = TR ? S.nutch : S.nlch, n2 = TR ? S.nltch : S.nuch;
+      sf_launch<0, 2, TR, LN>(S, n, n1, c1, r1, F1, x, nullptr, t1, nullptr);
+      if (beta) sf_launch<1, 1, TR, LN>(S, n, n2, c2, r2, F2, t1, S.dv, t2, y);
+      else sf_launch<1, 2, TR, LN>(S, n, n2, c2, r2, F2, t1, S.dv, t2, y);
+      return;
+    }
     // level sets of at most RUN_ROWS rows (two passes of the workgroup):
     // consecutive ones in one single-workgroup launch; larger ones: one grid
     // launch each
@@ -676,6 +708,29 @@ class DevPrecT : public DevPrec {
     }
     HIPCHECK(hipGetLastError());
   }
+
+  // One sync-free substitution over a whole schedule (nch chunks) of an
+  // n-row factor: t filled with the sentinel, then the solve.
+  template <int SC, int BT, bool TR, int LN>
+  void sf_launch(DSmoother<V>& S, int n, int nch, const int* ch, const int* rows, const DMat<V>& F, const V* b,
+                 const V* d, V* t, V* y) {
+    if (nch <= 0) return;
+    sf_fill_kernel<V><<<nblk(n), BS, 0, s_>>>(n, t);
+    const int nb = std::min(nch, SF_BLOCKS);
+    trsv_sf_kernel<V, SC, BT, TR, LN><<<nb, BS, 0, s_>>>(nch, ch, rows, F.irp, F.ja, F.val, b, d, t, y,
+                                                          S.sf_ticket, scal_ + S_ABORT, SF_SPIN_MAX);
+    HIPCHECK(hipGetLastError());
+  }
+  // The level-scheduled launches instead (MLD_TRSV=level).
+  static bool sync_free() {
+    static const bool sf = [] {
+      const char* e = std::getenv("MLD_TRSV");
+      return !(e && std::string(e) == "level");
+    }();
+    return sf;
+  }
+  static constexpr int SF_BLOCKS = 2048;
+  static constexpr int SF_SPIN_MAX = 1 << 22;
 
   // Hybrid Gauss-Seidel smoother (jacobi smoother + GS / BWGS solver, which
   // is iterative: mld_d_jac_smoother_apply_vect.f90:117-177 always runs the
@@ -725,6 +780,14 @@ class DevPrecT : public DevPrec {
     };
     if (D.n <= 8192) {  // whole solve in one workgroup
       run(0, nl);
+      return;
+    }
+    if (sync_free()) {
+      const int* ch = lower ? S.lch : S.uch;
+      const int nch = lower ? S.nlch : S.nuch;
+      if (bt == 0) sf_launch<2, 0, false, LN>(S, D.n, nch, ch, rows, T, b, S.dv, t, y);
+      else if (bt == 1) sf_launch<2, 1, false, LN>(S, D.n, nch, ch, rows, T, b, S.dv, t, y);
+      else sf_launch<2, 2, false, LN>(S, D.n, nch, ch, rows, T, b, S.dv, t, y);
       return;
     }
     constexpr int RUN_ROWS = 2048 / LN;
@@ -1022,7 +1085,7 @@ class DevPrecT : public DevPrec {
     DLevel<V>& L0 = L_[0];
     const int n = L0.n;
     V* xin = L0.x2l;  // holds x converted to V for the whole call
-    if (!xres_) xres_ = ar_.alloc<V>(n);
+    if (!xres_) xres_ = ar_.alloc<V>(n);  // allocated at upload
     convert_kernel<V, double><<<nblk(n), BS, 0, s_>>>(n, x, xres_);
     V* ysum = L0.t2;
     for (int isw = 1; isw <= outer_sweeps_; ++isw) {
@@ -1051,7 +1114,13 @@ class DevPrecT : public DevPrec {
   double read_scal(int slot) {
     HIPCHECK(hipMemcpyAsync(hscal_, scal_, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, s_));
     HIPCHECK(hipStreamSynchronize(s_));
+    check_abort();
     return hscal_[slot];
+  }
+  // a sync-free substitution that stopped waiting (see trsv_sf_kernel)
+  void check_abort() {
+    if (hscal_[S_ABORT] != 0.0)
+      throw DevError("sync-free triangular solve: a dependency wait exceeded its bound; results are invalid");
   }
   void dot1(const double* a, const double* b, int slot) {
     dot_kernel<double, 1, double><<<red_blocks(n0_), BS, 0, s_>>>(n0_, a, b, nullptr, nullptr, red(slot));
@@ -1100,6 +1169,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
   dot1(r, r, S_RR);
   HIPCHECK(hipMemcpyAsync(hscal_, scal_, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, s_));
   HIPCHECK(hipStreamSynchronize(s_));
+  check_abort();
   double bn2 = std::sqrt(hscal_[S_TMP0]);
   if (bn2 == 0.0) bn2 = 1.0;
   *err = std::sqrt(hscal_[S_RR]) / bn2;
@@ -1269,6 +1339,9 @@ HostStage& stage_of_current_device() {
 }
 }  // namespace
 
+// Both staging buffers are idle between calls: a call waits for its own
+// copies before it returns, so no call waits on an event recorded on another
+// call's stream (which may have been destroyed or be capturing since).
 void host_to_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
   if (bytes < ((size_t)4 << 20)) {
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
@@ -1276,14 +1349,18 @@ void host_to_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
   }
   std::lock_guard<std::mutex> lock(stage_mu);
   HostStage& st = stage_of_current_device();
+  bool used[2] = {false, false};
   int b = 0;
   for (size_t off = 0; off < bytes; off += HostStage::CH, b ^= 1) {
     const size_t len = std::min(HostStage::CH, bytes - off);
-    HIPCHECK(hipEventSynchronize(st.ev[b]));  // the buffer's previous copy has completed
+    if (used[b]) HIPCHECK(hipEventSynchronize(st.ev[b]));  // the buffer's previous copy has completed
     par_memcpy(st.buf[b], (const char*)src + off, len);
     HIPCHECK(hipMemcpyAsync((char*)dst + off, st.buf[b], len, hipMemcpyHostToDevice, s));
     HIPCHECK(hipEventRecord(st.ev[b], s));
+    used[b] = true;
   }
+  for (int k = 0; k < 2; ++k)
+    if (used[k]) HIPCHECK(hipEventSynchronize(st.ev[k]));
 }
 
 void device_to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
@@ -1295,10 +1372,11 @@ void device_to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
   std::lock_guard<std::mutex> lock(stage_mu);
   HostStage& st = stage_of_current_device();
   const size_t nch = (bytes + HostStage::CH - 1) / HostStage::CH;
+  // every chunk's event is waited for in the drain loop below before the
+  // buffer is reused or the call returns
   auto issue = [&](size_t k) {
     const int b = (int)(k & 1);
     const size_t off = k * HostStage::CH, len = std::min(HostStage::CH, bytes - off);
-    HIPCHECK(hipEventSynchronize(st.ev[b]));
     HIPCHECK(hipMemcpyAsync(st.buf[b], (const char*)src + off, len, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipEventRecord(st.ev[b], s));
   };

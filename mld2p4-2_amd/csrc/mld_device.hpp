@@ -38,12 +38,13 @@ class DevPrec {
 
 DevPrec* make_dev_prec(int value_kind, int device, hipStream_t stream);
 
+
 // Copies between pageable host memory and the device through two pinned
 // staging buffers per device (host threads fill / drain one buffer while the
-// DMA of the other is in flight).  host_to_device may return with its last
-// chunks in flight on s (the host source may be released at once);
-// device_to_host returns once dst holds the data.  Copies under 4 MB go
-// directly.
+// DMA of the other is in flight).  A staged copy returns once its chunks
+// have arrived (the staging buffers are idle between calls); copies under
+// 4 MB go directly (host_to_device then returns with the copy enqueued on s,
+// device_to_host after it).
 void host_to_device(void* dst, const void* src, size_t bytes, hipStream_t s);
 void device_to_host(void* dst, const void* src, size_t bytes, hipStream_t s);
 int dev_guard(const std::function<int()>& f, std::string& err);

@@ -148,6 +148,8 @@ struct Prec {
   int value_kind = 0;         // 0 = FP64 hierarchy on device, 1 = FP32 (mixed)
   int dev_format = 0;         // 0 = autotuned, 1 = force CSR, 2 = force ELL
   std::vector<StagedLevel> staged;
+  // device of the smoothed-aggregation products of hierarchy_build (-1: host)
+  int setup_device = -1;
   std::string last_error;
 };
 
@@ -204,7 +206,14 @@ int load_hierarchy(Prec& p, const std::string& prefix, int n, const int64_t* irp
 // building blocks, exposed for parity tests through the C-ABI
 int dec_map_bld(int iorder, double theta, const HCsr& a, std::vector<int>& ilaggr, int& naggr);
 int aggrmat_smth_asb(const HCsr& a, const std::vector<int>& ilaggr, int naggr, LevelParms& parms, HCsr& ac,
-                     HCsr& p, HCsr& r);
+                     HCsr& p, HCsr& r, int device = -1);
+// The products P = S Ptilde (S = I - omega D^-1 A, or the given filtered S),
+// AP = A P, R = P^T, Ac = R AP on the device, bit-identical to the host
+// products; E_UNSUPPORTED when a row outgrows the device accumulator (the
+// caller then forms them on the host).
+int dev_smoothed_products(int device, const HCsr& a, const HCsr* s_filtered, const std::vector<double>& dinv,
+                          double omega, const std::vector<int>& ilaggr, int naggr, HCsr& p, HCsr& r, HCsr& ac,
+                          std::string& err);
 int ilu0_factor(const HCsr& a, HCsr& l, HCsr& u, std::vector<double>& d);
 int iluk_factor(const HCsr& a, int fill, bool milu, HCsr& l, HCsr& u, std::vector<double>& d);
 void level_schedule(const HCsr& t, bool lower, std::vector<int>& ptr, std::vector<int>& rows);

@@ -583,7 +583,9 @@ __global__ __launch_bounds__(BS) void dot_kernel(int n, const V* __restrict__ x0
 // ------------------------------------------------------------ Krylov steps
 // Scalar slots in device memory.
 enum { S_RHO0 = 0, S_RHO1 = 1, S_SIGMA = 2, S_RR = 3, S_PR = 4, S_GAMMA = 5, S_DELTA = 6, S_TMP0 = 8,
-       S_TMP1 = 9, S_OMEGA = 10, S_ALPHA = 11, S_K0 = 12, S_K1 = 13, S_NSCAL = 16 };
+       S_TMP1 = 9, S_OMEGA = 10, S_ALPHA = 11, S_K0 = 12, S_K1 = 13, S_ABORT = 15, S_NSCAL = 16 };
+// (two-dot reductions write slot and slot + 1: S_TMP0/S_TMP1, S_K1/14;
+// S_ABORT stays clear of them)
 
 // CG: p = z (first) or p = z + (rho_new/rho_old) p; z in FP64 or the FP32
 // output of an FP32 preconditioner
@@ -626,6 +628,7 @@ __global__ __launch_bounds__(BS) void xr_update(int n, double* __restrict__ x, d
   }
   block_partials<1>(s, red.partial);
 }
+
 
 // FCG(1): p = z - (gamma/delta) p, and sum_i p_i r_i for the step length
 // (z as in cg_pupdate).
@@ -744,6 +747,124 @@ __global__ __launch_bounds__(1024) void trsv_run_kernel(int lv0, int lv1, const 
       }
     }
     __syncthreads();
+  }
+}
+
+// Sync-free substitution (no launch or barrier per level set): the rows in
+// level-schedule order are cut into chunks of at most BS/LANES rows that
+// never span two level sets (the rows of a chunk are independent).
+// Workgroups take chunks in order through an atomic ticket, so every chunk a
+// row depends on was taken earlier by a running workgroup.  The solution
+// vector itself signals completion: it is filled with a NaN sentinel first,
+// and a lane reading t_j polls it (device-scope atomic load) until it is not
+// the sentinel, then uses that value in its product; t_i is written with a
+// device-scope atomic store.  Value and signal are one word, so no fence is
+// needed between them.  The per-row arithmetic and summation order are
+// tri_row's.  A poll that exceeds spin_max attempts (or sees another's abort)
+// stops waiting and sets *abort, so the kernel always ends; the host reports
+// the failure.  The workgroup that draws the last ticket resets the counter
+// for the next launch.
+template <typename V>
+struct SfSentinel;
+template <>
+struct SfSentinel<double> {
+  static constexpr uint64_t bits = 0x7ff4deadbeefcafeull;
+  __device__ static bool is(double v) { return __double_as_longlong(v) == (long long)bits; }
+  __device__ static double value() { return __longlong_as_double((long long)bits); }
+};
+template <>
+struct SfSentinel<float> {
+  static constexpr uint32_t bits = 0x7fbadbadu;
+  __device__ static bool is(float v) { return __float_as_uint(v) == bits; }
+  __device__ static float value() { return __uint_as_float(bits); }
+};
+template <typename V>
+__global__ __launch_bounds__(BS) void sf_fill_kernel(int n, V* __restrict__ t) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i < n) t[i] = SfSentinel<V>::value();
+}
+template <typename V>
+__device__ __forceinline__ V sf_wait(const V* tj, double* abort, int spin_max) {
+  V v = __hip_atomic_load(tj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int n = 0;
+  while (SfSentinel<V>::is(v)) {
+    if (++n > spin_max || __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0.0) {
+      __hip_atomic_store(abort, 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return (V)0;
+    }
+    __builtin_amdgcn_s_sleep(2);
+    v = __hip_atomic_load(tj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return v;
+}
+// tri_row with every t_j taken from sf_wait
+template <typename V, int SC, bool TR, int LANES>
+__device__ __forceinline__ V tri_row_sf(int i, int lane, const int* __restrict__ irp, const int* __restrict__ ja,
+                                        const V* __restrict__ val, const V* __restrict__ b,
+                                        const V* __restrict__ d, const V* t, double* abort, int spin_max) {
+  const int k0 = irp[i], k1 = irp[i + 1];
+  const V bi = (SC == 1) ? d[i] * b[i] : b[i];
+  V acc = TR ? bi : (V)0;
+  if constexpr (!(TR && SC == 1)) {
+    for (int kb = k0; kb < k1; kb += LANES) {
+      const int k = kb + lane;
+      V pk = (V)0;
+      if (k < k1) pk = val[k] * sf_wait(t + ja[k], abort, spin_max);
+      const int cnt = min(LANES, k1 - kb);
+      if constexpr (LANES == 1) {
+        acc = TR ? acc - pk : acc + pk;
+      } else {
+        for (int q = 0; q < cnt; ++q) {
+          const V pq = __shfl(pk, q, LANES);
+          acc = TR ? acc - pq : acc + pq;
+        }
+      }
+    }
+  } else {
+    for (int ke = k1; ke > k0; ke -= LANES) {
+      const int kb = max(k0, ke - LANES);
+      const int k = kb + lane;
+      V pk = (V)0;
+      if (k < ke) pk = val[k] * sf_wait(t + ja[k], abort, spin_max);
+      if constexpr (LANES == 1) {
+        acc = acc - pk;
+      } else {
+        for (int q = ke - kb - 1; q >= 0; --q) acc = acc - __shfl(pk, q, LANES);
+      }
+    }
+  }
+  if constexpr (TR) return acc;
+  if constexpr (SC == 2) return (bi - acc) / d[i];
+  return bi - acc;
+}
+template <typename V, int SC, int BETA, bool TR, int LANES>
+__global__ __launch_bounds__(BS) void trsv_sf_kernel(int nchunks, const int* __restrict__ cptr,
+                                                      const int* __restrict__ rows, const int* __restrict__ irp,
+                                                      const int* __restrict__ ja, const V* __restrict__ val,
+                                                      const V* __restrict__ b, const V* __restrict__ d, V* t,
+                                                      V* y, int* ticket, double* abort, int spin_max) {
+  __shared__ int chunk;
+  const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const int c = atomicAdd(ticket, 1);
+      if (c == nchunks + (int)gridDim.x - 1) atomicExch(ticket, 0);  // every draw of this launch is done
+      chunk = c;
+    }
+    __syncthreads();
+    const int c = chunk;
+    __syncthreads();
+    if (c >= nchunks) return;
+    const int q = cptr[c] + grp;
+    if (q < cptr[c + 1]) {
+      const int i = rows[q];
+      const V ti = tri_row_sf<V, SC, TR, LANES>(i, lane, irp, ja, val, b, d, t, abort, spin_max);
+      if (lane == 0) {
+        if (BETA == 0) y[i] = ti;
+        else if (BETA == 1) y[i] = y[i] + ti;
+        __hip_atomic_store(t + i, ti, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
 }
 

@@ -893,8 +893,10 @@ class MultiT : public MultiEngine {
   }
   ~MultiT() override {
     (void)hipSetDevice(device_);
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
+    for (auto& c : graphs_) {
+      (void)hipGraphExecDestroy(c.exec);
+      (void)hipGraphDestroy(c.graph);
+    }
     if (hstat_) (void)hipHostFree(hstat_);
     (void)hipStreamSynchronize(s_);
     (void)hipEventDestroy(ev_);
@@ -1048,13 +1050,16 @@ class MultiT : public MultiEngine {
     int active = read_active();
     int nl = 0, done_it = 0;
     if (active > 0 && itmax > 0) {
-      const int ch = chunk();
-      if (!graph_exec_ || graph_method_ != method || graph_chunk_ != ch || graph_eps_ != eps ||
-          graph_itmax_ != itmax) {
-        capture(method, ch, eps, itmax);
-      }
+      // first launch: as many iterations as the last solve with the same
+      // method and tolerances took (one launch when the systems converge in
+      // as many steps again); then launches of chunk() iterations until every
+      // system has converged.  Iterations after a system's convergence leave
+      // its vectors unchanged.
+      const bool same = (pred_method_ == method && pred_eps_ == eps && pred_itmax_ == itmax);
+      const int first = (same && pred_iters_ > 0) ? std::min(pred_iters_, itmax) : chunk();
       while (done_it < itmax) {
-        HIPCHECK(hipGraphLaunch(graph_exec_, s_));
+        const int ch = (nl == 0) ? first : chunk();
+        HIPCHECK(hipGraphLaunch(graph_for(method, ch, eps, itmax), s_));
         ++nl;
         done_it += ch;
         if (read_active() == 0) break;
@@ -1066,11 +1071,17 @@ class MultiT : public MultiEngine {
     HIPCHECK(hipMemcpyAsync(it.data(), ic_[I_ITERS], sizeof(int) * ncol_, hipMemcpyDeviceToHost, s_));
     HIPCHECK(hipMemcpyAsync(er.data(), sc_[C_ERR], sizeof(double) * ncol_, hipMemcpyDeviceToHost, s_));
     HIPCHECK(hipStreamSynchronize(s_));
+    int itmx = 0;
     for (int j = 0; j < nrhs; ++j)
       for (int s = 0; s < G_; ++s) {
         iters[j * G_ + s] = it[s * NB + j];
         errs[j * G_ + s] = er[s * NB + j];
+        itmx = std::max(itmx, it[s * NB + j]);
       }
+    pred_method_ = method;
+    pred_eps_ = eps;
+    pred_itmax_ = itmax;
+    pred_iters_ = itmx;
     if (launches) *launches = nl;
     HIPCHECK(hipEventRecord(ev_, s_));
     HIPCHECK(hipStreamWaitEvent(caller, ev_, 0));
@@ -1116,10 +1127,17 @@ class MultiT : public MultiEngine {
   int* ic_[NIC] = {};
   double* kv_[6] = {};
   int* hstat_ = nullptr;
-  hipGraph_t graph_ = nullptr;
-  hipGraphExec_t graph_exec_ = nullptr;
-  int graph_method_ = -1, graph_chunk_ = 0, graph_itmax_ = 0;
-  double graph_eps_ = 0.0;
+  // captured iteration sequences, one per (method, iterations, eps, itmax)
+  struct Captured {
+    int method, iters, itmax;
+    double eps;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<Captured> graphs_;
+  // iteration count of the last solve (the length of the next first launch)
+  int pred_method_ = -1, pred_iters_ = 0, pred_itmax_ = 0;
+  double pred_eps_ = 0.0;
 
   static int nblk64(int64_t n) { return (int)((n + BS - 1) / BS); }
 
@@ -1519,29 +1537,36 @@ class MultiT : public MultiEngine {
     HIPCHECK(hipGetLastError());
   }
 
-  // The iteration sequence of `ch` iterations captured once as a graph; its
-  // vectors are the engine's own buffers, so the graph is reused by later
-  // solves with the same method and tolerances.
-  void capture(int method, int ch, double eps, int itmax) {
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
-    graph_exec_ = nullptr;
-    graph_ = nullptr;
+  // The sequence of `ch` iterations captured once as a graph; its vectors
+  // are the engine's own buffers, so the graph is reused by later solves with
+  // the same method and tolerances (a few lengths are kept).
+  hipGraphExec_t graph_for(int method, int ch, double eps, int itmax) {
+    for (auto& c : graphs_)
+      if (c.method == method && c.iters == ch && c.eps == eps && c.itmax == itmax) return c.exec;
+    if (graphs_.size() >= 4) {  // drop the oldest
+      (void)hipGraphExecDestroy(graphs_.front().exec);
+      (void)hipGraphDestroy(graphs_.front().graph);
+      graphs_.erase(graphs_.begin());
+    }
+    hipGraph_t g = nullptr;
     HIPCHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
     try {
       for (int i = 0; i < ch; ++i) iteration(method, eps, itmax);
     } catch (...) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(s_, &g);
-      if (g) (void)hipGraphDestroy(g);
+      hipGraph_t gg = nullptr;
+      (void)hipStreamEndCapture(s_, &gg);
+      if (gg) (void)hipGraphDestroy(gg);
       throw;
     }
-    HIPCHECK(hipStreamEndCapture(s_, &graph_));
-    HIPCHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-    graph_method_ = method;
-    graph_chunk_ = ch;
-    graph_eps_ = eps;
-    graph_itmax_ = itmax;
+    HIPCHECK(hipStreamEndCapture(s_, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      HIPCHECK(e);
+    }
+    graphs_.push_back(Captured{method, ch, itmax, eps, g, ex});
+    return ex;
   }
 };
 

@@ -187,9 +187,11 @@ def test_group_krylov(eng, orc, method, precision, chunk, monkeypatch):
         assert np.linalg.norm(b[sl] - A @ xh[sl]) <= 1.01e-8 * np.linalg.norm(b[sl]), s
         _, it1, _ = eng.psb_krylov(method, A, systems[s], dev(b[sl]), eps=1e-8)
         assert abs(int(it[0, s]) - it1) <= 1, (s, it[0, s], it1)
-    # the captured graph is reused: a second solve gives the same result
-    x2, it2, er2, _ = g.krylov(method, dev(b), eps=1e-8)
+    # a second solve runs as one graph launch of as many iterations as the
+    # first one took, with the same result
+    x2, it2, er2, l2 = g.krylov(method, dev(b), eps=1e-8)
     assert (x2.cpu().numpy() == xh).all() and (it2 == it).all()
+    assert l2 == 1
 
 
 def test_group_block_columns(eng, orc):

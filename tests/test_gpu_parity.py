@@ -121,11 +121,12 @@ def test_level_smoother(eng, orc, smoother, sweeps, beta):
 
 @pytest.mark.parametrize("idim,beta", [(30, 0), (30, 1), (52, 0)])
 def test_ilu_level_scheduled_bitexact(eng, orc, idim, beta):
-    """ILU(0) smoother on levels large enough for the level-scheduled path
-    (grid launches for large level sets, single-workgroup runs for
-    consecutive small ones, 4 or 16 lanes per row) and on a small coarse
-    level (one workgroup): identical to the sequential substitution, for
-    'N' and for 'T' (column-oriented order)."""
+    """ILU(0) smoother on levels large enough for the sync-free substitution
+    (rows in level-schedule order taken by workgroups through a ticket, each
+    product waiting for the value it reads; 4 or 16 lanes per row) and on a
+    small coarse level (one workgroup, level sets with barriers): identical
+    to the sequential substitution, for 'N' and for 'T' (column-oriented
+    order), on repeated applications."""
     _say(f"[ILU level-scheduled] idim={idim} beta={beta}")
     a = orc.gen_pde3d(idim)
     A = a.to_scipy()
@@ -136,7 +137,7 @@ def test_ilu_level_scheduled_bitexact(eng, orc, idim, beta):
         n = p.level_info(lev)["n"]
         x = rng.standard_normal(n)
         y0 = rng.standard_normal(n)
-        for trans in ("N", "T"):
+        for trans in ("N", "T", "N"):
             y = dev(y0.copy())
             p.level_smooth(lev, dev(x), y, beta=beta, sweeps=1, trans=trans)
             yo = po.smooth(lev, x, alpha=1.0, beta=float(beta), y=y0, sweeps=1, trans=trans)
