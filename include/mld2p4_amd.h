@@ -140,6 +140,19 @@ int mldhip_krylov(mldhip_prec* p, const char* method, const double* b, double* x
 int mldhip_krylov_host(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,
                        int istop, int* iter, double* err);
 
+/* mldhip_krylov for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc);
+ * no counterpart in the reference.  Same arguments and results as
+ * mldhip_krylov, methods "CG" and "FCG", trans 'N'.  The four Krylov work
+ * vectors live in a fixed workspace of the library (32 MiB on every GPU it
+ * runs on, held until the process ends) instead of the handle's lazily
+ * allocated vectors: systems of at most 1048576 rows (-1 beyond, with the
+ * reason in mldhip_get_error); the handle must be built (its cycle buffers
+ * are allocated at build).  Calls that overlap on one GPU share the
+ * workspace and race: serialize them. */
+int mldhip_krylov_noalloc(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,
+                          int istop, int* iter, double* err);
+
 /* ---- differentiation (no reference counterpart: the reference computes no
  * gradients) --------------------------------------------------------------- */
 /* Adjoint solve A^T x = b with the same hierarchy applied transposed

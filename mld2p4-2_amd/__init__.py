@@ -70,6 +70,7 @@ def lib():
             "mldhip_apply_host": [vp, dp, dp, C.c_char],
             "mldhip_krylov": [vp, cp, vp, vp, C.c_double, C.c_int, C.c_int, ip, dp],
             "mldhip_krylov_host": [vp, cp, dp, dp, C.c_double, C.c_int, C.c_int, ip, dp],
+            "mldhip_krylov_noalloc": [vp, cp, vp, vp, C.c_double, C.c_int, C.c_int, ip, dp],
             "mldhip_set_stream": [vp, vp],
             "mldhip_synchronize": [vp],
             "mldhip_get_nlev": [vp, ip],

@@ -290,6 +290,25 @@ int mldhip_krylov(mldhip_prec* p, const char* method, const double* b, double* x
   return mld::dev_guard([&]() { return p->dev->krylov(mc, b, x, eps, itmax, istop, iter, err, 'N'); }, p->err);
 }
 
+int mldhip_krylov_noalloc(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,
+                          int istop, int* iter, double* err) {
+  int rc = need_dev(p);
+  if (rc) return rc;
+  const int mc = method_code(method);
+  if (mc != 0 && mc != 1) return fail(p, mld::E_UNSUPPORTED, "krylov_noalloc: method must be CG or FCG");
+  if (!iter || !err) return E_BAD_ARG;
+  const size_t n = (size_t)p->host.precv[0].a.nr;
+  if (n > mld::NOALLOC_MAX_ROWS)
+    return fail(p, E_BAD_ARG, "krylov_noalloc: " + std::to_string(n) + " rows exceed the workspace maximum of " +
+                                  std::to_string(mld::NOALLOC_MAX_ROWS));
+  return mld::dev_guard(
+      [&]() {
+        double* ws = mld::noalloc_workspace(p->device);
+        return p->dev->krylov_ws(mc, b, x, eps, itmax, istop, iter, err, ws, mld::NOALLOC_VECS);
+      },
+      p->err);
+}
+
 int mldhip_krylov_adjoint(mldhip_prec* p, const char* method, const double* b, double* x, double eps, int itmax,
                           int istop, int* iter, double* err) {
   int rc = need_dev(p);

@@ -1108,9 +1108,38 @@ class DevPrecT : public DevPrec {
 
   // --------------------------------------------------------------- Krylov
   double* kvec(int i) {
+    if (ws_) {  // caller's workspace (krylov_ws): nothing is allocated
+      if (i >= ws_nvec_) throw DevError("Krylov workspace holds " + std::to_string(ws_nvec_) + " vectors");
+      const size_t stride = ((size_t)n0_ + 31) / 32 * 32;  // every vector 256-byte aligned, as allocations are
+      return ws_ + (size_t)i * stride;
+    }
     while ((int)kvec_.size() <= i) kvec_.push_back(ar_.alloc<double>(n0_));
     return kvec_[i];
   }
+  double* ws_ = nullptr;
+  int ws_nvec_ = 0;
+
+ public:
+  // krylov ('N') with the work vectors r, z, p, q in ws (nvec vectors of n
+  // doubles): CG and FCG, one outer sweep or more; allocates nothing on the
+  // device (every other buffer of the path was allocated at upload).
+  int krylov_ws(int method, const double* b, double* x, double eps, int itmax, int istop, int* iter, double* err,
+                double* ws, int nvec) override {
+    if (method != 0 && method != 1) return E_UNSUPPORTED;
+    if (!ws || nvec < 4) return E_BAD_ARG;
+    struct Reset {
+      DevPrecT& p;
+      ~Reset() {
+        p.ws_ = nullptr;
+        p.ws_nvec_ = 0;
+      }
+    } reset{*this};
+    ws_ = ws;
+    ws_nvec_ = nvec;
+    return krylov(method, b, x, eps, itmax, istop, iter, err, 'N');
+  }
+
+ private:
   double read_scal(int slot) {
     HIPCHECK(hipMemcpyAsync(hscal_, scal_, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, s_));
     HIPCHECK(hipStreamSynchronize(s_));
@@ -1300,6 +1329,34 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
 }
 
 // ---------------------------------------------------------------------------
+// Workspace of mldhip_krylov_noalloc: the four FP64 Krylov vectors (r, z, p,
+// q) of systems of up to NOALLOC_MAX_ROWS rows, 32 MiB, in a file-scope
+// device array that every GPU holds until the process ends.  Its device
+// address is taken once per device (with that device current) and only that
+// address is used from the host.  Calls on one GPU share the array.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[NOALLOC_VECS * NOALLOC_MAX_ROWS];
+
+double* noalloc_workspace(int device) {
+  static std::mutex mu;
+  static std::vector<double*> ptr;  // per device
+  std::lock_guard<std::mutex> lock(mu);
+  if (device < 0) throw DevError("no device");
+  if ((int)ptr.size() <= device) ptr.resize(device + 1, nullptr);
+  if (!ptr[device]) {
+    int cur = 0;
+    HIPCHECK(hipGetDevice(&cur));
+    if (cur != device) HIPCHECK(hipSetDevice(device));
+    void* a = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (cur != device) HIPCHECK(hipSetDevice(cur));
+    if (e != hipSuccess || !a)
+      throw DevError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ptr[device] = (double*)a;
+  }
+  return ptr[device];
+}
+
 DevPrec* make_dev_prec(int value_kind, int device, hipStream_t stream) {
   if (value_kind == 1) return new DevPrecT<float>(device, stream);
   return new DevPrecT<double>(device, stream);

@@ -23,6 +23,10 @@ class DevPrec {
   // A^T x = b preconditioned by M^T (same hierarchy, transposed operators).
   virtual int krylov(int method, const double* b, double* x, double eps, int itmax, int istop, int* iter,
                      double* err, char trans) = 0;
+  // krylov ('N', CG or FCG) with the caller's work vectors: ws holds nvec
+  // vectors of n doubles; allocates nothing on the device.
+  virtual int krylov_ws(int method, const double* b, double* x, double eps, int itmax, int istop, int* iter,
+                        double* err, double* ws, int nvec) = 0;
   virtual int sync() = 0;
   virtual void set_stream(hipStream_t s) = 0;
   virtual hipStream_t stream() const = 0;
@@ -38,6 +42,11 @@ class DevPrec {
 
 DevPrec* make_dev_prec(int value_kind, int device, hipStream_t stream);
 
+// Device address of the no-allocation Krylov workspace on `device` (NOALLOC_VECS
+// vectors of NOALLOC_MAX_ROWS doubles), taken once per device.
+constexpr size_t NOALLOC_MAX_ROWS = (size_t)1 << 20;
+constexpr int NOALLOC_VECS = 4;
+double* noalloc_workspace(int device);
 
 // Copies between pageable host memory and the device through two pinned
 // staging buffers per device (host threads fill / drain one buffer while the
