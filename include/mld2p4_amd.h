@@ -63,8 +63,8 @@ int mldhip_set_format(mldhip_prec* p, int fmt);
 /* ---- build -------------------------------------------------------------- */
 /* Where hierarchy_build forms the smoothed-aggregation products P, A P, R
  * and Ac of each level (mld_daggrmat_smth_asb.f90:291,345,399; aggregation
- * always runs on the host): 0 host threads (default), 1 the handle's device,
- * -1 the device when the handle has one and MLD_SETUP is not "host".
+ * always runs on the host): 0 host threads, 1 the handle's device, -1
+ * (default) the device when the handle has one and MLD_SETUP is not "host".
  * Both give bit-identical hierarchies.  No reference counterpart. */
 int mldhip_set_setup(mldhip_prec* p, int where);
 /* prec%hierarchy_build(a, desc, info): mlprec/impl/mld_d_hierarchy_bld.f90:66-490. */

@@ -15,7 +15,7 @@
 struct mldhip_prec {
   mld::Prec host;
   int device = -1;
-  int setup = 0;  // products of the hierarchy build: 0 host, 1 device, -1 device when there is one
+  int setup = -1;  // products of the hierarchy build: 0 host, 1 device, -1 device when there is one
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::unique_ptr<mld::DevPrec> dev;
