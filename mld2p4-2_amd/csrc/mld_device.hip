@@ -1216,6 +1216,13 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
     const double*& p;
     ~PreconvReset() { p = nullptr; }
   } preconv_reset{preconv_};
+  // the deferred solution update x += (num/den) p of the last iteration
+  // (slots; -1: none pending), applied by the next direction update or here
+  int xnum = -1, xden = -1;
+  auto flush_x = [&]() {
+    if (xnum >= 0) x_update<<<nblk(n), BS, 0, s_>>>(n, x, p, scal_, xnum, xden);
+    xnum = xden = -1;
+  };
   if (method == 0) {  // CG
     while (it < itmax) {
       ++it;
@@ -1223,15 +1230,17 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       if (r32) {  // z = M^-1 r kept in FP32, rho = r.z
         const V* z32 = nullptr;
         int rc = apply_keep32(r, r, rs, &z32);
-        if (rc) return rc;
-        cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, z32, p, scal_, it);
+        if (rc) { flush_x(); return rc; }
+        cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, z32, p, scal_, it, x, xnum, xden);
       } else {
         int rc = apply_impl(r, z, trans, r, rs);  // z = M^-1 r, rho = r.z
-        if (rc) return rc;
-        cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, (const double*)z, p, scal_, it);
+        if (rc) { flush_x(); return rc; }
+        cg_pupdate<<<nblk(n), BS, 0, s_>>>(n, (const double*)z, p, scal_, it, x, xnum, xden);
       }
       kspmv<1>(trans, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, red(S_SIGMA));
-      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, (it & 1) ? S_RHO1 : S_RHO0, S_SIGMA, red(S_RR), r32);
+      r_update<<<nbd, BS, 0, s_>>>(n, r, q, scal_, rs, S_SIGMA, red(S_RR), r32);
+      xnum = rs;
+      xden = S_SIGMA;
       preconv_ = r32 ? r : nullptr;
       finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
@@ -1241,6 +1250,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       if (*err <= eps) break;
       if (hscal_[S_SIGMA] == 0.0 || (it > 1 && rho_old == 0.0) || rho_new == 0.0) break;
     }
+    flush_x();
   } else if (method == 1) {  // FCG(1)
     while (it < itmax) {
       ++it;
@@ -1249,16 +1259,18 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       if (r32) {
         const V* z32 = nullptr;
         int rc = apply_keep32(r, it > 1 ? q : nullptr, S_GAMMA, &z32);
-        if (rc) return rc;
-        fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z32, p, r, scal_, it, red(S_PR));
+        if (rc) { flush_x(); return rc; }
+        fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z32, p, r, scal_, it, red(S_PR), x, xnum, xden);
       } else {
         int rc = apply_impl(r, z, trans, it > 1 ? q : nullptr, S_GAMMA);
-        if (rc) return rc;
-        fcg_pupdate<<<nbd, BS, 0, s_>>>(n, (const double*)z, p, r, scal_, it, red(S_PR));
+        if (rc) { flush_x(); return rc; }
+        fcg_pupdate<<<nbd, BS, 0, s_>>>(n, (const double*)z, p, r, scal_, it, red(S_PR), x, xnum, xden);
       }
       finalize(1, S_PR);
       kspmv<1>(trans, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, red(S_DELTA));
-      xr_update<<<nbd, BS, 0, s_>>>(n, x, r, p, q, scal_, S_PR, S_DELTA, red(S_RR), r32);
+      r_update<<<nbd, BS, 0, s_>>>(n, r, q, scal_, S_PR, S_DELTA, red(S_RR), r32);
+      xnum = S_PR;
+      xden = S_DELTA;
       preconv_ = r32 ? r : nullptr;
       finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
@@ -1267,6 +1279,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       if (*err <= eps) break;
       if (hscal_[S_DELTA] == 0.0) break;
     }
+    flush_x();
   } else if (method == 2) {  // BiCGStab (right preconditioned)
     double* rt = kvec(4);
     double* s = kvec(5);

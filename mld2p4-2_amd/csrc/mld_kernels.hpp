@@ -587,11 +587,25 @@ enum { S_RHO0 = 0, S_RHO1 = 1, S_SIGMA = 2, S_RR = 3, S_PR = 4, S_GAMMA = 5, S_D
 // (two-dot reductions write slot and slot + 1: S_TMP0/S_TMP1, S_K1/14;
 // S_ABORT stays clear of them)
 
+// Step length num/den from device slots; 0 when den == 0 (breakdown, detected
+// on the host), which leaves x and r unchanged.
+__device__ __forceinline__ double step_len(const double* __restrict__ scal, int num_slot, int den_slot) {
+  const double den = scal[den_slot];
+  return (den != 0.0) ? scal[num_slot] / den : 0.0;
+}
+
+// The solution update x += alpha p of an iteration is deferred to the next
+// direction update, which reads p anyway (one pass over p fewer per
+// iteration); x_update applies the last one when the iteration ends.  The
+// arithmetic is that of the update in place.
+
 // CG: p = z (first) or p = z + (rho_new/rho_old) p; z in FP64 or the FP32
-// output of an FP32 preconditioner
+// output of an FP32 preconditioner.  From the second iteration also the
+// previous iteration's x += alpha p (alpha = scal[xnum] / scal[xden]).
 template <typename ZT>
 __global__ __launch_bounds__(BS) void cg_pupdate(int n, const ZT* __restrict__ z, double* __restrict__ p,
-                                                  const double* __restrict__ scal, int it) {
+                                                  const double* __restrict__ scal, int it,
+                                                  double* __restrict__ x, int xnum, int xden) {
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
   if (it == 1) {
@@ -599,27 +613,25 @@ __global__ __launch_bounds__(BS) void cg_pupdate(int n, const ZT* __restrict__ z
   } else {
     const double rn = scal[(it & 1) ? S_RHO1 : S_RHO0], ro = scal[(it & 1) ? S_RHO0 : S_RHO1];
     const double beta = rn / ro;
-    p[i] = (double)z[i] + beta * p[i];
+    const double pi = p[i];
+    x[i] = step_len(scal, xnum, xden) * pi + x[i];
+    p[i] = (double)z[i] + beta * pi;
   }
 }
 
-// x += alpha p; r -= alpha q; sum r_i^2.  alpha = num/den from device slots;
-// den == 0 leaves x and r unchanged (breakdown, detected on the host).
-// r32 (may be null): also writes r in FP32, the input of the next FP32
-// preconditioner application (saves its conversion pass).
-__global__ __launch_bounds__(BS) void xr_update(int n, double* __restrict__ x, double* __restrict__ r,
-                                                 const double* __restrict__ p, const double* __restrict__ q,
-                                                 const double* __restrict__ scal, int num_slot, int den_slot,
-                                                 Red red, float* __restrict__ r32) {
-  const double den = scal[den_slot];
-  const double alpha = (den != 0.0) ? scal[num_slot] / den : 0.0;
+// r -= alpha q; sum r_i^2.  r32 (may be null): also writes r in FP32, the
+// input of the next FP32 preconditioner application (saves its conversion
+// pass).
+__global__ __launch_bounds__(BS) void r_update(int n, double* __restrict__ r, const double* __restrict__ q,
+                                                const double* __restrict__ scal, int num_slot, int den_slot,
+                                                Red red, float* __restrict__ r32) {
+  const double alpha = step_len(scal, num_slot, den_slot);
   double s[2] = {0.0, 0.0};
   const int b = blockIdx.x * BS * RED_EPT + threadIdx.x;
 #pragma unroll
   for (int e = 0; e < RED_EPT; ++e) {
     const int i = b + e * BS;
     if (i < n) {
-      x[i] = alpha * p[i] + x[i];
       const double ri = (-alpha) * q[i] + r[i];
       r[i] = ri;
       if (r32) r32[i] = (float)ri;
@@ -629,21 +641,36 @@ __global__ __launch_bounds__(BS) void xr_update(int n, double* __restrict__ x, d
   block_partials<1>(s, red.partial);
 }
 
+// x += alpha p (the last deferred update)
+__global__ __launch_bounds__(BS) void x_update(int n, double* __restrict__ x, const double* __restrict__ p,
+                                                const double* __restrict__ scal, int num_slot, int den_slot) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i < n) x[i] = step_len(scal, num_slot, den_slot) * p[i] + x[i];
+}
 
 // FCG(1): p = z - (gamma/delta) p, and sum_i p_i r_i for the step length
-// (z as in cg_pupdate).
+// (z as in cg_pupdate; from the second iteration also the previous
+// iteration's x += alpha p).
 template <typename ZT>
 __global__ __launch_bounds__(BS) void fcg_pupdate(int n, const ZT* __restrict__ z, double* __restrict__ p,
                                                    const double* __restrict__ r, const double* __restrict__ scal,
-                                                   int it, Red red) {
+                                                   int it, Red red, double* __restrict__ x, int xnum, int xden) {
   const double c = (it == 1) ? 0.0 : -(scal[S_GAMMA] / scal[S_DELTA]);
+  const double alpha = (it == 1) ? 0.0 : step_len(scal, xnum, xden);
   double s[2] = {0.0, 0.0};
   const int b = blockIdx.x * BS * RED_EPT + threadIdx.x;
 #pragma unroll
   for (int e = 0; e < RED_EPT; ++e) {
     const int i = b + e * BS;
     if (i < n) {
-      const double pi = (it == 1) ? (double)z[i] : (double)z[i] + c * p[i];
+      double pi;
+      if (it == 1) {
+        pi = (double)z[i];
+      } else {
+        const double po = p[i];
+        x[i] = alpha * po + x[i];
+        pi = (double)z[i] + c * po;
+      }
       p[i] = pi;
       s[0] += pi * r[i];
     }
