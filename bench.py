@@ -291,10 +291,44 @@ def section_run_b(eng, torch, problems, irp, ja, val, n, steps):
     b = torch.ones(n, dtype=torch.float64, device="cuda:0")
     _, tm, it, er = solve_timed(eng, torch, p, "CG", b, 1e-8, steps, 1)
     t_vc = time_vcycle(torch, p, n, reps=5)
-    out = {"smoother": "BJAC/ILU(0) 1 sweep", "solve_s": round(float(np.mean(tm)), 6), "iters": int(it[-1]),
-           "relres": er[-1], "vcycle_ms": round(t_vc * 1e3, 3), "setup_s": round(t_b, 2)}
+    # dependency depth: level sets of each ILU(0) solve per level, times the
+    # solves per V-cycle (pre + post sweeps, each an L and a U solve)
+    levels, steps_vc = [], 0
+    for l in range(p.nlev):
+        i = p.level_info(l)
+        dl, du = p.tri_levels(l)
+        apps = i["sweeps_pre"] + (i["sweeps_post"] if l < p.nlev - 1 else 0)
+        levels.append({"n": i["n"], "levels_lower": dl, "levels_upper": du, "ilu_applications": apps})
+        steps_vc += apps * (dl + du)
     del p
+    release(torch)
+    floor_us = trsv_chain_step(eng, torch)
+    out = {"smoother": "BJAC/ILU(0) 1 sweep", "solve_s": round(float(np.mean(tm)), 6), "iters": int(it[-1]),
+           "relres": er[-1], "vcycle_ms": round(t_vc * 1e3, 3), "setup_s": round(t_b, 2),
+           "ilu_levels": levels, "dependent_steps_per_vcycle": steps_vc,
+           "us_per_dependent_step": round(t_vc * 1e6 / steps_vc, 3),
+           "chain_step_us": round(floor_us, 3),
+           "vcycle_lower_bound_ms": round(steps_vc * floor_us * 1e-3, 3)}
+    log(f"run B: {steps_vc} dependent steps per V-cycle, {t_vc * 1e6 / steps_vc:.2f} us each; "
+        f"one step of a pure chain {floor_us:.2f} us -> lower bound {steps_vc * floor_us * 1e-3:.1f} ms")
     return out
+
+
+def trsv_chain_step(eng, torch, n=1 << 16, reps=3):
+    """Time of one dependent step of the engine's triangular substitution:
+    one-level block Jacobi ILU(0) on the 1-D Laplacian, whose factors are
+    bidiagonal (every level set one row), so an application is 2n dependent
+    steps with nothing beside them."""
+    import scipy.sparse as sp
+
+    a = sp.diags([-np.ones(n - 1), 2.0 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1], format="csr")
+    p = eng.MldPrec(device=0).init("BJAC")
+    p.set("SUB_SOLVE", "ILU")
+    p.build(a)
+    assert p.tri_levels(0) == (n, n)
+    t = time_vcycle(torch, p, n, reps=reps)
+    del p
+    return t * 1e6 / (2 * n)
 
 
 def section_diffusion(eng, torch, problems, idim, steps):
@@ -577,7 +611,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box here has one GPU). "
-              "The multi-GPU sharding design is described in DESIGN.md.", file=sys.stderr)
+              "How the path would shard is a design only.", file=sys.stderr)
         sys.exit(2)
     if args.traffic_probe:
         traffic_probe(args.idim)

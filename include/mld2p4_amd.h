@@ -243,6 +243,12 @@ int mldhip_get_omega(mldhip_prec* p, int lev, double* omega);
  * ILU / Gauss-Seidel substitutions (0 when the smoother has none).  No
  * reference counterpart (measurement). */
 int mldhip_get_tri_levels(mldhip_prec* p, int lev, int which, int* nl_lower, int* nl_upper);
+/* First level of the dense tail (the untransposed cycle on levels lev ..
+ * nlev-1 applied as one dense product, formed at smoothers_build), or -1 when
+ * the cycle runs level by level (MLD_DENSE_TAIL=0 at build, no level of at
+ * most 1024 rows below the fine level, a K-cycle there).  No reference
+ * counterpart. */
+int mldhip_get_dense_tail(mldhip_prec* p, int* lev);
 /* Operator complexity sum nnz(A_l)/nnz(A_1) (mld_d_prec_type.f90:465-497). */
 int mldhip_get_complexity(mldhip_prec* p, double* cplx);
 int mldhip_get_device_bytes(mldhip_prec* p, int64_t* bytes);

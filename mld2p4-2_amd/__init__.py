@@ -81,6 +81,7 @@ def lib():
             "mldhip_get_level_dv": [vp, C.c_int, C.c_int, dp],
             "mldhip_get_omega": [vp, C.c_int, dp],
             "mldhip_get_tri_levels": [vp, C.c_int, C.c_int, ip, ip],
+            "mldhip_get_dense_tail": [vp, ip],
             "mldhip_get_complexity": [vp, dp],
             "mldhip_dump": [vp, C.c_int, C.c_int, cp, cp, C.c_int],
             "mldhip_get_device_bytes": [vp, i64p],
@@ -383,6 +384,12 @@ class MldPrec:
         out = np.zeros(self.level_info(lev)["n"])
         self._check(lib().mldhip_get_level_dv(self.h, lev, which, _dp(out)), "get_level_dv")
         return out
+
+    def dense_tail(self):
+        """First level of the dense tail operator, or -1."""
+        v = C.c_int()
+        self._check(lib().mldhip_get_dense_tail(self.h, C.byref(v)), "get_dense_tail")
+        return v.value
 
     def tri_levels(self, lev, which=0):
         """(level sets of the lower solve, of the upper solve) of a smoother."""

@@ -450,6 +450,12 @@ int mldhip_dump(mldhip_prec* p, int istart, int iend, const char* prefix, const 
   return 0;
 }
 
+int mldhip_get_dense_tail(mldhip_prec* p, int* lev) {
+  if (!p || !lev) return E_BAD_ARG;
+  *lev = p->dev ? p->dev->tail_level() : -1;
+  return 0;
+}
+
 int mldhip_get_device_bytes(mldhip_prec* p, int64_t* bytes) {
   if (!p || !bytes) return E_BAD_ARG;
   *bytes = p->dev ? (int64_t)p->dev->device_bytes() : 0;

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <mutex>
 #include <stdexcept>
@@ -40,6 +41,7 @@ template <typename V>
 struct DMat {
   int nr = 0, nc = 0;
   int64_t nnz = 0;
+  int64_t stored = 0;   // entries in val (padding included; DIA: nr * w)
   int fmt = FMT_CSR, lanes = 1, w = 0, ld = 0;
   int iters = 0;        // CSR: entries per lane loaded up front (0, 2, 4, 8)
   int rpl = 1;          // block products (32 columns): columns per lane (1, 2, 4)
@@ -57,7 +59,7 @@ struct DSmoother {
   int smoother = JAC, solver = DIAG_SCALE;
   int64_t nnz_nd = 0;
   DMat<V> nd, ndt;     // ND and its transpose (uploaded for 'T' applies)
-  DMat<V> ndd;         // DIA ND with its columns scaled by d (ND D): first sweep from zero
+  DMat<V> ndd;         // ND with its columns scaled by d (ND D): first sweep from zero
   V* dv = nullptr;
   // ILU(0): factors and level schedules; for 'T': transposed factors
   DMat<V> l, u, lt, ut;
@@ -172,7 +174,7 @@ static FmtChoice choose_format(const HCsr& h, int force) {
     }
   }
   if (force == 2 && maxrow <= 64) { c.fmt = FMT_ELL; return c; }
-  if (force >= 3 && force <= 5) { c.fmt = FMT_SELL; return c; }
+  if ((force >= 3 && force <= 5) || force == 7) { c.fmt = FMT_SELL; return c; }
   if (maxrow <= 16 && (double)h.nr * maxrow <= 1.10 * nnz) { c.fmt = FMT_ELL; return c; }
   if (h.nr >= 4096) { c.fmt = FMT_SELL; return c; }
   return c;
@@ -182,8 +184,12 @@ static FmtChoice choose_format(const HCsr& h, int force) {
 // so equal-length rows keep their order); slices of 64 positions.  Large
 // windows cut padding; small windows keep consecutive rows (and so the
 // columns they gather) together in a wavefront.  force 3: sigma 4096,
-// force 4: sigma 256, force 5: no sorting.
-static int sell_sigma(int force) { return force == 4 ? 256 : (force == 5 ? 1 : 4096); }
+// force 4: sigma 256, force 5: no sorting, force 7: sigma 128 (the level-1
+// Galerkin operator at 256^3: padding 1.16x unsorted, 1.09x at 128 with the
+// same gathered cache lines per entry, 1.05x at 256 with 16 % more).
+static int sell_sigma(int force) {
+  return force == 4 ? 256 : (force == 5 ? 1 : (force == 7 ? 128 : 4096));
+}
 
 // An operator uploaded once as CSR (int32 row pointers, FP64 values); every
 // candidate layout is then built from it on the device.
@@ -226,10 +232,12 @@ static DMat<V> build_mat(const HCsr& h, const DevSrc& src, DevArena& ar, hipStre
     const std::vector<int> offs = c.offs.empty() ? dia_offsets(h) : c.offs;
     m.w = (int)offs.size();
     for (int d = 0; d < 8; ++d) m.off.o[d] = d < m.w ? offs[d] : 0;
-    m.val = ar.alloc<V>((size_t)h.nr * m.w);
+    m.stored = (int64_t)h.nr * m.w;
+    m.val = ar.alloc<V>((size_t)m.stored);
     dia_fill_kernel<V><<<nblk(h.nr), BS, 0, s>>>(h.nr, m.w, m.off, src.irp, src.ja, src.val, m.val);
   } else if (c.fmt == FMT_ELL) {
     m.ld = h.nr;
+    m.stored = (int64_t)m.ld * m.w;
     m.ja = ar.alloc<int>((size_t)m.ld * m.w);
     m.val = ar.alloc<V>((size_t)m.ld * m.w);
     ell_fill_kernel<V><<<nblk(h.nr), BS, 0, s>>>(h.nr, h.nc, m.w, src.irp, src.ja, src.val, m.ja, m.val);
@@ -240,6 +248,7 @@ static DMat<V> build_mat(const HCsr& h, const DevSrc& src, DevArena& ar, hipStre
     if (!host_sell_layout(h, sell_sigma(force), SELL_C, src.maxrow, perm, sptr))
       throw DevError("SELL storage exceeds 2^31 entries");
     const size_t tot = (size_t)sptr[nsl];
+    m.stored = (int64_t)tot;
     m.sptr = ar.upload(sptr.data(), sptr.size(), s);
     m.perm = ar.upload(perm.data(), perm.size(), s);
     m.ja = ar.alloc<int>(tot);
@@ -247,6 +256,7 @@ static DMat<V> build_mat(const HCsr& h, const DevSrc& src, DevArena& ar, hipStre
     sell_fill_kernel<V><<<nblk((int64_t)nsl * SELL_C), BS, 0, s>>>(nr, h.nc, nsl * SELL_C, m.perm, m.sptr, src.irp,
                                                                   src.ja, src.val, m.ja, m.val);
   } else {
+    m.stored = (int64_t)h.val.size();
     m.irp = ar.alloc<int>((size_t)h.nr + 1);
     m.ja = ar.alloc<int>(h.ja.size());
     m.val = ar.alloc<V>(h.val.size());
@@ -484,6 +494,7 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
       cands.push_back({3, 0, true});
       cands.push_back({4, 0, true});
       cands.push_back({5, 0, true});
+      cands.push_back({7, 0, true});
       for (int l = 1; l <= 64; l *= 2)
         if (l == 1 || l <= avg) cands.push_back({1, l, l == 1});
     }
@@ -499,6 +510,10 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
     }
     bforce = cands[bi].force;
     blanes = cands[bi].lanes;
+    if (std::getenv("MLD_VERBOSE"))
+      std::fprintf(stderr, "[mld] %d x %d operator: layout %s (%s %d), %.1f us\n", h.nr, h.nc,
+                   blanes > 0 ? "CSR" : (bforce == 2 ? "ELL" : "SELL"), blanes > 0 ? "lanes" : "sigma",
+                   blanes > 0 ? blanes : (bforce == 2 ? 0 : sell_sigma(bforce)), best * 1e3);
   }
   FmtChoice fb = dia ? FmtChoice{FMT_DIA, 1, (int)offs.size(), offs} : choose_format(h, bforce);
   if (blanes > 0) { fb.fmt = FMT_CSR; fb.lanes = blanes; }

@@ -106,7 +106,52 @@ class DevPrecT : public DevPrec {
     HIPCHECK(hipHostMalloc((void**)&hscal_, S_NSCAL * sizeof(double), hipHostMallocDefault));
     HIPCHECK(hipStreamSynchronize(s_));
     hprec_ = &p;
+    build_tail();
     return 0;
+  }
+
+  // Dense tail: from the first level l >= 1 with at most DENSE_TAIL_MAX rows,
+  // the untransposed cycle on levels l .. nlev-1 is a fixed linear operator
+  // B_l of x2l (no K-cycle there), formed once here by applying that cycle to
+  // the unit vectors and applied afterwards as one dense product
+  // y2l = B_l x2l: about ten launches of a few microseconds each become one.
+  // 'T' applications keep the level-by-level cycle.  Not taken when that
+  // level is a coarsest level already solved by one dense product
+  // (DENSE_LU), nor with MLD_DENSE_TAIL=0.  Same operator; the sums of the
+  // product are rounded in another order.
+  static constexpr int DENSE_TAIL_MAX = 1024;
+  int tail_lev_ = -1;
+  V* tail_b_ = nullptr;
+  void build_tail() {
+    tail_lev_ = -1;
+    tail_b_ = nullptr;
+    if (const char* e = std::getenv("MLD_DENSE_TAIL"))
+      if (e[0] == '0') return;
+    const int nlev = (int)L_.size();
+    int l = 1;
+    while (l < nlev && L_[l].n > DENSE_TAIL_MAX) ++l;
+    if (l >= nlev) return;
+    for (int k = l; k < nlev; ++k)
+      if (L_[k].ml_cycle == KCYCLE || L_[k].ml_cycle == KCYCLESYM) return;
+    DLevel<V>& D = L_[l];
+    if (l == nlev - 1 && (D.sm.solver == DENSE_LU || D.sm.smoother == NOPREC)) return;
+    const int n = D.n;
+    V* B = ar_.alloc<V>((size_t)n * n);
+    for (int j = 0; j < n; ++j) {
+      unit_vec_kernel<V><<<nblk(n), BS, 0, s_>>>(n, D.x2l, j);
+      int rc = inner_ml_aply(l, 'N');
+      if (rc) return;  // the cycle stays as it is
+      col_store_kernel<V><<<nblk(n), BS, 0, s_>>>(n, D.y2l, B, j);
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s_));
+    tail_b_ = B;
+    tail_lev_ = l;
+  }
+  // y2l = B x2l at the tail level
+  void tail_apply(DLevel<V>& D) {
+    tail_gemv_kernel<V><<<(D.n + BS / 64 - 1) / (BS / 64), BS, 0, s_>>>(D.n, tail_b_, D.x2l, D.y2l);
+    HIPCHECK(hipGetLastError());
   }
 
   ~DevPrecT() override {
@@ -118,6 +163,7 @@ class DevPrecT : public DevPrec {
   }
 
   size_t device_bytes() const override { return ar_.bytes; }
+  int tail_level() const override { return tail_lev_; }
 
   // y = op(M^-1) x (mld_dprecaply2_vect with alpha = 1, beta = 0).
   int apply(const double* x, double* y, char trans) override { return apply_impl(x, y, trans, nullptr, -1); }
@@ -244,6 +290,38 @@ class DevPrecT : public DevPrec {
     if (y && !out_done_) finish_out(y, L0.y2l, L0.n);
     return rc;
   }
+
+  // The Krylov loop enqueues the next application's first kernel, the fine
+  // level's pre-smoothing of x into y (operands aliased as apply_cycle does),
+  // before it waits for the residual norm on the host, so that the GPU works
+  // while the host reads the norm and launches the rest; the next
+  // inner_mult at level 0 then skips that launch.  When the loop stops
+  // instead, the launched sweep only wrote the fine level's output buffer.
+  // Returns false (nothing launched) where the cycle does not start that way.
+  bool launch_pre0(const double* x, double* y, char trans) {
+    if (L_.size() < 2 || outer_sweeps_ != 1) return false;
+    DLevel<V>& L0 = L_[0];
+    if (L0.ml_cycle != VCYCLE && L0.ml_cycle != WCYCLE && L0.ml_cycle != MULT_ML) return false;
+    const bool tn = (trans == 'N');
+    if (!((L0.sweeps_pre > 0 && tn) || (L0.sweeps_post > 0 && !tn))) return false;
+    struct Restore {
+      DLevel<V>& L;
+      V* x;
+      V* y;
+      ~Restore() {
+        L.x2l = x;
+        L.y2l = y;
+      }
+    } restore{L0, L0.x2l, L0.y2l};
+    L0.x2l = const_cast<V*>(in_vec(x, L0.x2l, L0.n));
+    preconv_ = x;  // x2l now holds x (converted in the FP32 form)
+    L0.y2l = out_vec(y, L0.y2l);
+    DSmoother<V>& Spre = (tn || !L0.has_sm2) ? L0.sm : L0.sm2;
+    if (smooth(L0, Spre, L0.x2l, 0, L0.y2l, tn ? L0.sweeps_pre : L0.sweeps_post, trans)) return false;
+    pre0_launched_ = true;
+    return true;
+  }
+  bool pre0_launched_ = false;
 
   // fused-output state of the current apply_impl call
   double* out64_ = nullptr;
@@ -463,15 +541,21 @@ class DevPrecT : public DevPrec {
       HIPCHECK(hipStreamSynchronize(s_));
     }
     if (H.nnz_nd > 0) S.nd = upload_tuned<V>(H.nd, ar_, s_, force);
-    // Jacobi-type solver on a DIA ND: ND D stored beside ND, so that the
-    // first sweep from zero, ND (D x), is the plain product (ND D) x (one
-    // gather per entry instead of two; the product's rounding is
-    // re-associated, (a_ij d_j) x_j for a_ij (d_j x_j))
-    if (H.nnz_nd > 0 && S.nd.fmt == FMT_DIA && S.dv && (H.solver == DIAG_SCALE || H.solver == L1_DIAG)) {
+    // Jacobi-type solver: ND D (ND with its columns scaled by d, sharing ND's
+    // indices) stored beside ND, so that the first sweep from zero, ND (D x),
+    // is the plain product (ND D) x: one gather per entry, and on SELL / CSR
+    // no separate pass forming D x.  The product's rounding is re-associated,
+    // (a_ij d_j) x_j for a_ij (d_j x_j).
+    if (H.nnz_nd > 0 && S.dv && (H.solver == DIAG_SCALE || H.solver == L1_DIAG)) {
       S.ndd = S.nd;
-      S.ndd.val = ar_.alloc<V>((size_t)S.nd.nr * S.nd.w);
-      dia_colscale_kernel<V><<<nblk(S.nd.nr), BS, 0, s_>>>(S.nd.nr, S.nd.nc, S.nd.w, S.nd.off, S.nd.val, S.dv,
-                                                           S.ndd.val);
+      S.ndd.val = ar_.alloc<V>((size_t)std::max<int64_t>(1, S.nd.stored));
+      if (S.nd.fmt == FMT_DIA) {
+        dia_colscale_kernel<V><<<nblk(S.nd.nr), BS, 0, s_>>>(S.nd.nr, S.nd.nc, S.nd.w, S.nd.off, S.nd.val, S.dv,
+                                                             S.ndd.val);
+      } else if (S.nd.stored > 0) {
+        colscale_kernel<V><<<(int)((S.nd.stored + BS - 1) / BS), BS, 0, s_>>>(S.nd.stored, S.nd.ja, S.nd.val,
+                                                                               S.dv, S.ndd.val);
+      }
       HIPCHECK(hipGetLastError());
     }
     if (H.solver == ILU_N || H.solver == GS || H.solver == BWGS) {
@@ -810,10 +894,10 @@ class DevPrecT : public DevPrec {
   }
 
   // One Jacobi sweep y_out = D^-1 (b - ND y_in) (FIRST: y_in = D^-1 b).
-  // FIRST on a DIA operator ('N') is the product with ND D; FIRST on an ELL
-  // operator (or DIA for 'T') gathers d_j*x_j directly; on SELL/CSR,
-  // whose gathers are the costly part, d.*x is formed first in one streaming
-  // pass (same products, so the same result) and gathered once.
+  // FIRST ('N') is the product with ND D; for 'T', FIRST on an ELL or DIA
+  // operator gathers d_j*x_j directly, on SELL/CSR, whose gathers are the
+  // costly part, d.*x is formed first in one streaming pass (same products,
+  // so the same result) and gathered once.
   template <bool FIRST, int K, class E>
   void jac_launch(DLevel<V>& D, DSmoother<V>& S, const V* b, const V* yin, E e, char trans, Red red) {
     const DMat<V>& nd = (trans != 'N') ? S.ndt : S.nd;
@@ -913,6 +997,10 @@ class DevPrecT : public DevPrec {
 
   // inner_ml_aply (mld_dmlprec_aply.f90:382-464): dispatch on the level's cycle
   int inner_ml_aply(int level, char trans) {
+    if (level == tail_lev_ && trans == 'N') {
+      tail_apply(L_[level]);
+      return 0;
+    }
     switch (L_[level].ml_cycle) {
       case ADD_ML: return inner_add(level, trans);
       case MULT_ML: case VCYCLE: case WCYCLE: return inner_mult(level, trans);
@@ -1052,8 +1140,12 @@ class DevPrecT : public DevPrec {
     if (level < nlev - 1) {
       DLevel<V>& C = L_[level + 1];
       if (pre) {
-        rc = smooth(D, Spre, D.x2l, 0, D.y2l, tn ? D.sweeps_pre : D.sweeps_post, trans);
-        if (rc) return rc;
+        if (level == 0 && pre0_launched_) {
+          pre0_launched_ = false;  // enqueued ahead by the Krylov loop, same operands
+        } else {
+          rc = smooth(D, Spre, D.x2l, 0, D.y2l, tn ? D.sweeps_pre : D.sweeps_post, trans);
+          if (rc) return rc;
+        }
         residual(D, D.x2l, D.y2l, D.ty, trans);
         spmv_launch(C.r, GatherX<V, V>{D.ty}, EpiSet<V, V>{C.x2l}, s_);
       } else {
@@ -1212,10 +1304,15 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
     if (trans == 'N' && outer_sweeps_ == 1 && L_.size() > 1) r32 = (float*)L_[0].x2l;
   }
   preconv_ = nullptr;
+  pre0_launched_ = false;
   struct PreconvReset {
     const double*& p;
-    ~PreconvReset() { p = nullptr; }
-  } preconv_reset{preconv_};
+    bool& pre0;
+    ~PreconvReset() {
+      p = nullptr;
+      pre0 = false;  // a sweep launched ahead for an application that did not follow
+    }
+  } preconv_reset{preconv_, pre0_launched_};
   // the deferred solution update x += (num/den) p of the last iteration
   // (slots; -1: none pending), applied by the next direction update or here
   int xnum = -1, xden = -1;
@@ -1244,6 +1341,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       preconv_ = r32 ? r : nullptr;
       finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
+      if (it < itmax) launch_pre0(r, r32 ? nullptr : z, trans);
       read_scal(S_RR);
       *err = std::sqrt(hscal_[S_RR]) / bn2;
       const double rho_new = hscal_[(it & 1) ? S_RHO1 : S_RHO0], rho_old = hscal_[(it & 1) ? S_RHO0 : S_RHO1];
@@ -1274,6 +1372,7 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
       preconv_ = r32 ? r : nullptr;
       finalize(1, S_RR);
       HIPCHECK(hipGetLastError());
+      if (it < itmax) launch_pre0(r, r32 ? nullptr : z, trans);
       read_scal(S_RR);
       *err = std::sqrt(hscal_[S_RR]) / bn2;
       if (*err <= eps) break;

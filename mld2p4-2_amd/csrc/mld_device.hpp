@@ -31,6 +31,8 @@ class DevPrec {
   virtual void set_stream(hipStream_t s) = 0;
   virtual hipStream_t stream() const = 0;
   virtual size_t device_bytes() const = 0;
+  // first level of the dense tail operator, -1 without one
+  virtual int tail_level() const = 0;
   // Average duration (ms) of one launch sequence of a single cycle step at
   // level lev, timed with HIP events on the handle's stream over reps runs.
   // kind: 0 residual r = x - A y, 1 pre smoother, 2 restriction R r,

@@ -1062,6 +1062,43 @@ __global__ __launch_bounds__(BS) void dia_colscale_kernel(int nr, int nc, int w,
   }
 }
 
+// ------------------------------------------------- dense tail operator
+// y = B x for a dense row-major n x n operator: one wavefront per row, the
+// lanes over the columns, then a fixed-order reduction across the lanes.
+template <typename V>
+__global__ __launch_bounds__(BS) void tail_gemv_kernel(int n, const V* __restrict__ B, const V* __restrict__ x,
+                                                       V* __restrict__ y) {
+  const int row = blockIdx.x * (BS / 64) + (int)(threadIdx.x / 64);
+  const int lane = threadIdx.x % 64;
+  if (row >= n) return;  // whole wavefronts
+  const V* b = B + (size_t)row * n;
+  V acc = (V)0;
+  for (int j = lane; j < n; j += 64) acc += b[j] * x[j];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) y[row] = acc;
+}
+// x = e_j
+template <typename V>
+__global__ __launch_bounds__(BS) void unit_vec_kernel(int n, V* __restrict__ x, int j) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i < n) x[i] = (i == j) ? (V)1 : (V)0;
+}
+// column j of the row-major n x n B = y
+template <typename V>
+__global__ __launch_bounds__(BS) void col_store_kernel(int n, const V* __restrict__ y, V* __restrict__ B, int j) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i < n) B[(size_t)i * n + j] = y[i];
+}
+// oval[k] = val[k] * c[ja[k]] over every stored entry of an ELL, SELL or CSR
+// operator (padding entries hold a zero value on a valid column)
+template <typename V>
+__global__ __launch_bounds__(BS) void colscale_kernel(int64_t tot, const int* __restrict__ ja,
+                                                      const V* __restrict__ val, const V* __restrict__ c,
+                                                      V* __restrict__ oval) {
+  const int64_t k = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (k < tot) oval[k] = val[k] * c[ja[k]];
+}
+
 }  // namespace
 }  // namespace k
 }  // namespace mld

@@ -484,3 +484,50 @@ def test_apply_after_error_return(eng, orc):
         b = np.ones(A.shape[0])
         _, it, err = eng.psb_krylov("FCG", A, p, dev(b), eps=1e-8)
         assert err <= 1e-8
+
+
+@pytest.mark.parametrize("cycle,smoother,sweeps,coarse,idim", [
+    ("VCYCLE", "L1-JACOBI", 2, "BJAC", 16), ("WCYCLE", "JACOBI", 1, "BJAC", 16), ("WCYCLE", "L1-JACOBI", 2, "BJAC", 24),
+    ("VCYCLE", "BJAC", 1, "BJAC", 24), ("VCYCLE", "FBGS", 1, "BJAC", 16), ("ADD", "JACOBI", 1, "BJAC", 16),
+    ("VCYCLE", "L1-JACOBI", 2, "DENSE_LU", 24), ("KCYCLE", "JACOBI", 2, "BJAC", 24)])
+def test_dense_tail(eng, orc, monkeypatch, cycle, smoother, sweeps, coarse, idim):
+    """The cycle on the levels from the first one of at most 1024 rows below
+    the fine level, applied as one dense product (formed at build by running
+    that cycle on the unit vectors) in 'N' applications: the same operator as
+    the level-by-level cycle of the same hierarchy (MLD_DENSE_TAIL=0) and
+    within 1e-11 of the oracle, FP64 and FP32; 'T' (where the solvers allow
+    it) keeps the cycle; CG iterations equal.  Not taken with a K-cycle below
+    that level."""
+    _say(f"[dense tail] {cycle} {smoother} sweeps={sweeps} coarse={coarse} idim={idim}")
+    a = orc.gen_pde3d(idim)
+    A = a.to_scipy()
+    csw = 1 if coarse == "DENSE_LU" else 4
+    kw = dict(mcs=40, coarse=coarse, csweeps=csw, extra=(("ML_CYCLE", cycle),))
+    p = engine_prec(eng, A, smoother, sweeps, **kw)
+    monkeypatch.setenv("MLD_DENSE_TAIL", "0")
+    q = engine_prec(eng, A, smoother, sweeps, **kw)
+    q32 = engine_prec(eng, A, smoother, sweeps, precision="mixed", **kw)
+    monkeypatch.delenv("MLD_DENSE_TAIL")
+    p32 = engine_prec(eng, A, smoother, sweeps, precision="mixed", **kw)
+    assert q.dense_tail() == -1 and q32.dense_tail() == -1
+    sizes = [p.level_info(l)["n"] for l in range(p.nlev)]
+    expect = next((l for l in range(1, p.nlev) if sizes[l] <= 1024), -1)
+    if cycle.startswith("K") or (expect == p.nlev - 1 and coarse == "DENSE_LU"):
+        expect = -1
+    assert p.dense_tail() == expect and p32.dense_tail() == expect, sizes
+    assert expect >= 1 or cycle.startswith("K")
+    po = oracle_prec(orc, a, smoother, sweeps, mcs=40, coarse=coarse, csweeps=csw, ml_cycle=CYC[cycle])
+    rng = np.random.default_rng(idim + 11)
+    for trans in (("N", "T") if smoother != "FBGS" and coarse != "DENSE_LU" else ("N",)):
+        x = rng.standard_normal(A.shape[0])
+        yo = po.apply(x, trans=trans)
+        yp = p.apply(dev(x), trans=trans).cpu().numpy()
+        yq = q.apply(dev(x), trans=trans).cpu().numpy()
+        assert relmax(yp, yq) <= 1e-12 and relmax(yp, yo) <= 1e-11, trans
+        y32 = p32.apply(dev(x), trans=trans).cpu().numpy()
+        assert relmax(y32, q32.apply(dev(x), trans=trans).cpu().numpy()) <= 1e-5
+        assert relmax(y32, yo) <= 1e-5, trans
+    b = np.ones(A.shape[0])
+    _, itp, errp = eng.psb_krylov("CG", A, p, dev(b), eps=1e-8)
+    _, itq, _ = eng.psb_krylov("CG", A, q, dev(b), eps=1e-8)
+    assert errp <= 1e-8 and abs(itp - itq) <= 1
