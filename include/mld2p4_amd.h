@@ -56,8 +56,11 @@ int mldhip_set_precision(mldhip_prec* p, int kind);
  * 3 = SELL (sliced ELL, 64-row slices, rows length-sorted in windows of
  * 4096), 4 = SELL with windows of 256 rows, 5 = SELL without sorting, 6 = DIA
  * (diagonal storage, for operators with at most 8 distinct diagonals; others
- * fall back to the automatic choice).  The automatic choice takes DIA for such
- * banded operators, else the layout it times fastest. */
+ * fall back to the automatic choice), 7 = SELL with windows of 128 rows,
+ * 8 / 9 = SELD (SELL with windows of 256 / 4096 rows whose column indices are
+ * 16-bit steps from the previous column of the row; operators with a larger
+ * step fall back to SELL).  The automatic choice takes DIA for such banded
+ * operators, else the layout it times fastest. */
 int mldhip_set_format(mldhip_prec* p, int fmt);
 
 /* ---- build -------------------------------------------------------------- */
@@ -285,7 +288,7 @@ int mldhip_level_smooth(mldhip_prec* p, int lev, int which, const double* x, int
 int mldhip_time_step(mldhip_prec* p, int lev, int kind, int reps, double* avg_ms);
 /* Device storage of a level operator (what 0 A, 1 P, 2 R, 3 ND):
  * fmt 0 = CSR with `lanes` lanes per row, 1 = ELL of width `width`, 2 = SELL,
- * 3 = DIA with `width` diagonals. */
+ * 3 = DIA with `width` diagonals, 4 = SELL with 16-bit column steps. */
 int mldhip_get_op_format(mldhip_prec* p, int lev, int what, int* fmt, int* lanes, int* width);
 
 #ifdef __cplusplus

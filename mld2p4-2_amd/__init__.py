@@ -200,7 +200,8 @@ class MldPrec:
         return self
 
     def set_format(self, fmt: str):
-        self._check(lib().mldhip_set_format(self.h, {"auto": 0, "csr": 1, "ell": 2, "sell": 3, "sell256": 4, "sellu": 5, "dia": 6}[fmt]), "set_format")
+        self._check(lib().mldhip_set_format(self.h, {"auto": 0, "csr": 1, "ell": 2, "sell": 3, "sell256": 4, "sellu": 5, "dia": 6,
+                                                          "sell128": 7, "seld": 8, "seld4096": 9}[fmt]), "set_format")
         return self
 
     def set_setup(self, where: str):
@@ -451,7 +452,7 @@ class MldPrec:
         f, l, w = C.c_int(), C.c_int(), C.c_int()
         self._check(lib().mldhip_get_op_format(self.h, lev, LEVEL_OPS[what], C.byref(f), C.byref(l), C.byref(w)),
                     "get_op_format")
-        return {"fmt": ("CSR", "ELL", "SELL", "DIA")[f.value], "lanes": l.value, "width": w.value}
+        return {"fmt": ("CSR", "ELL", "SELL", "DIA", "SELD")[f.value], "lanes": l.value, "width": w.value}
 
     def level_smooth(self, lev, x, y, beta=0, sweeps=1, trans="N", which=0):
         self.use_torch_stream()

@@ -35,7 +35,7 @@ struct DevError : std::runtime_error {
                      std::to_string(__LINE__));                                                     \
   } while (0)
 
-enum { FMT_CSR = 0, FMT_ELL = 1, FMT_SELL = 2, FMT_DIA = 3 };
+enum { FMT_CSR = 0, FMT_ELL = 1, FMT_SELL = 2, FMT_DIA = 3, FMT_SELD = 4 };
 
 template <typename V>
 struct DMat {
@@ -50,6 +50,10 @@ struct DMat {
   int* irp = nullptr;   // CSR row pointers
   int* sptr = nullptr;  // SELL slice offsets (nslices + 1)
   int* perm = nullptr;  // SELL position -> row
+  // SELD: SELL whose column indices are 16-bit steps: entry k of a position
+  // is column col0 + k + (dja_0 + ... + dja_k) (padding: steps of 0)
+  int* col0 = nullptr;
+  uint16_t* dja = nullptr;
   int* ja = nullptr;
   V* val = nullptr;
 };
@@ -157,6 +161,23 @@ static bool dia_eligible(const HCsr& h, const std::vector<int>& offs) {
   return !offs.empty() && h.nr == h.nc && (double)h.nr * offs.size() <= 1.10 * (double)h.nnz();
 }
 
+// SELD (force 8: sigma 256, force 9: sigma 4096): SELL with each row's
+// column indices stored as 16-bit steps between consecutive stored columns
+// (step - 1, so 1 .. 65536), and the first column per position.  Entries
+// become 10 bytes instead of 12 (FP64), 6 instead of 8 (FP32); the products
+// keep the stored order, so the sums are the SELL sums.  Eligible when every
+// row's columns increase by at most 65536 from one entry to the next.
+static bool host_seld_ok(const HCsr& h) {
+  bool ok = true;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+  for (int i = 0; i < h.nr; ++i)
+    for (int64_t k = h.irp[i] + 1; k < h.irp[i + 1]; ++k) {
+      const int64_t d = (int64_t)h.ja[k] - h.ja[k - 1];
+      if (d < 1 || d > 65536) ok = false;
+    }
+  return ok;
+}
+
 static FmtChoice choose_format(const HCsr& h, int force) {
   const int maxrow = host_max_row(h);
   const double nnz = (double)h.nnz();
@@ -175,6 +196,10 @@ static FmtChoice choose_format(const HCsr& h, int force) {
   }
   if (force == 2 && maxrow <= 64) { c.fmt = FMT_ELL; return c; }
   if ((force >= 3 && force <= 5) || force == 7) { c.fmt = FMT_SELL; return c; }
+  if (force == 8 || force == 9) {  // SELL where the steps do not fit 16 bits
+    c.fmt = host_seld_ok(h) ? FMT_SELD : FMT_SELL;
+    return c;
+  }
   if (maxrow <= 16 && (double)h.nr * maxrow <= 1.10 * nnz) { c.fmt = FMT_ELL; return c; }
   if (h.nr >= 4096) { c.fmt = FMT_SELL; return c; }
   return c;
@@ -188,7 +213,7 @@ static FmtChoice choose_format(const HCsr& h, int force) {
 // Galerkin operator at 256^3: padding 1.16x unsorted, 1.09x at 128 with the
 // same gathered cache lines per entry, 1.05x at 256 with 16 % more).
 static int sell_sigma(int force) {
-  return force == 4 ? 256 : (force == 5 ? 1 : (force == 7 ? 128 : 4096));
+  return (force == 4 || force == 8) ? 256 : (force == 5 ? 1 : (force == 7 ? 128 : 4096));
 }
 
 // An operator uploaded once as CSR (int32 row pointers, FP64 values); every
@@ -241,7 +266,7 @@ static DMat<V> build_mat(const HCsr& h, const DevSrc& src, DevArena& ar, hipStre
     m.ja = ar.alloc<int>((size_t)m.ld * m.w);
     m.val = ar.alloc<V>((size_t)m.ld * m.w);
     ell_fill_kernel<V><<<nblk(h.nr), BS, 0, s>>>(h.nr, h.nc, m.w, src.irp, src.ja, src.val, m.ja, m.val);
-  } else if (c.fmt == FMT_SELL) {
+  } else if (c.fmt == FMT_SELL || c.fmt == FMT_SELD) {
     const int nr = h.nr;
     const int nsl = (nr + SELL_C - 1) / SELL_C;
     std::vector<int> perm, sptr;
@@ -251,10 +276,21 @@ static DMat<V> build_mat(const HCsr& h, const DevSrc& src, DevArena& ar, hipStre
     m.stored = (int64_t)tot;
     m.sptr = ar.upload(sptr.data(), sptr.size(), s);
     m.perm = ar.upload(perm.data(), perm.size(), s);
-    m.ja = ar.alloc<int>(tot);
+    DevArena jtmp;  // SELD: the 32-bit indices only until they are converted
+    int* ja = (c.fmt == FMT_SELL) ? ar.alloc<int>(tot) : jtmp.alloc<int>(std::max<size_t>(1, tot));
     m.val = ar.alloc<V>(tot);
     sell_fill_kernel<V><<<nblk((int64_t)nsl * SELL_C), BS, 0, s>>>(nr, h.nc, nsl * SELL_C, m.perm, m.sptr, src.irp,
-                                                                  src.ja, src.val, m.ja, m.val);
+                                                                  src.ja, src.val, ja, m.val);
+    if (c.fmt == FMT_SELL) {
+      m.ja = ja;
+    } else {
+      m.col0 = ar.alloc<int>((size_t)nsl * SELL_C);
+      m.dja = ar.alloc<uint16_t>(std::max<size_t>(1, tot));
+      seld_convert_kernel<<<nblk((int64_t)nsl * SELL_C), BS, 0, s>>>(nr, nsl * SELL_C, m.perm, m.sptr, src.irp, ja,
+                                                                   m.col0, m.dja);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(s));  // before jtmp is released
+    }
   } else {
     m.stored = (int64_t)h.val.size();
     m.irp = ar.alloc<int>((size_t)h.nr + 1);
@@ -402,6 +438,9 @@ static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red = Red
     }
   } else if (m.fmt == FMT_SELL) {
     sell_kernel<C, K><<<nb, BS, 0, s>>>(m.nr, m.sptr, m.perm, m.ja, m.val, g, e, red, xcd_chunk(m.xmap, nb, BS));
+  } else if (m.fmt == FMT_SELD) {
+    seld_kernel<C, K><<<nb, BS, 0, s>>>(m.nr, m.nc, m.sptr, m.perm, m.col0, m.dja, m.val, g, e, red,
+                                         xcd_chunk(m.xmap, nb, BS));
   } else {
     switch (m.lanes) {
       case 1: csr_launch_l<1, K, C>(m, nb, g, e, s, red); break;
@@ -495,6 +534,10 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
       cands.push_back({4, 0, true});
       cands.push_back({5, 0, true});
       cands.push_back({7, 0, true});
+      if (host_seld_ok(h)) {
+        cands.push_back({8, 0, true});
+        cands.push_back({9, 0, true});
+      }
       for (int l = 1; l <= 64; l *= 2)
         if (l == 1 || l <= avg) cands.push_back({1, l, l == 1});
     }
@@ -512,7 +555,8 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
     blanes = cands[bi].lanes;
     if (std::getenv("MLD_VERBOSE"))
       std::fprintf(stderr, "[mld] %d x %d operator: layout %s (%s %d), %.1f us\n", h.nr, h.nc,
-                   blanes > 0 ? "CSR" : (bforce == 2 ? "ELL" : "SELL"), blanes > 0 ? "lanes" : "sigma",
+                   blanes > 0 ? "CSR" : (bforce == 2 ? "ELL" : (bforce >= 8 ? "SELD" : "SELL")),
+                   blanes > 0 ? "lanes" : "sigma",
                    blanes > 0 ? blanes : (bforce == 2 ? 0 : sell_sigma(bforce)), best * 1e3);
   }
   FmtChoice fb = dia ? FmtChoice{FMT_DIA, 1, (int)offs.size(), offs} : choose_format(h, bforce);

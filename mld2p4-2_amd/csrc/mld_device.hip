@@ -106,7 +106,9 @@ class DevPrecT : public DevPrec {
     HIPCHECK(hipHostMalloc((void**)&hscal_, S_NSCAL * sizeof(double), hipHostMallocDefault));
     HIPCHECK(hipStreamSynchronize(s_));
     hprec_ = &p;
+    const double tt = wall_seconds();
     build_tail();
+    if (tail_lev_ >= 0) phase_log("dense tail", tail_lev_ + 1, wall_seconds() - tt);
     return 0;
   }
 
@@ -552,6 +554,10 @@ class DevPrecT : public DevPrec {
       if (S.nd.fmt == FMT_DIA) {
         dia_colscale_kernel<V><<<nblk(S.nd.nr), BS, 0, s_>>>(S.nd.nr, S.nd.nc, S.nd.w, S.nd.off, S.nd.val, S.dv,
                                                              S.ndd.val);
+      } else if (S.nd.fmt == FMT_SELD) {
+        const int npos = (S.nd.nr + 63) / 64 * 64;
+        seld_colscale_kernel<V><<<nblk(npos), BS, 0, s_>>>(npos, S.nd.nc, S.nd.sptr, S.nd.col0, S.nd.dja,
+                                                             S.nd.val, S.dv, S.ndd.val);
       } else if (S.nd.stored > 0) {
         colscale_kernel<V><<<(int)((S.nd.stored + BS - 1) / BS), BS, 0, s_>>>(S.nd.stored, S.nd.ja, S.nd.val,
                                                                                S.dv, S.ndd.val);
@@ -744,9 +750,9 @@ class DevPrecT : public DevPrec {
       const int* c1 = TR ? S.utch : S.lch;
       const int* c2 = TR ? S.ltch : S.uch;
       const int n1 = TR ? S.nutch : S.nlch, n2 = TR ? S.nltch : S.nuch;
-      sf_launch<0, 2, TR, LN>(S, n, n1, c1, r1, F1, x, nullptr, t1, nullptr);
-      if (beta) sf_launch<1, 1, TR, LN>(S, n, n2, c2, r2, F2, t1, S.dv, t2, y);
-      else sf_launch<1, 2, TR, LN>(S, n, n2, c2, r2, F2, t1, S.dv, t2, y);
+      sf_launch<0, 2, TR, LN>(S, n, n1, nl1, c1, r1, F1, x, nullptr, t1, nullptr);
+      if (beta) sf_launch<1, 1, TR, LN>(S, n, n2, nl2, c2, r2, F2, t1, S.dv, t2, y);
+      else sf_launch<1, 2, TR, LN>(S, n, n2, nl2, c2, r2, F2, t1, S.dv, t2, y);
       return;
     }
     // level sets of at most RUN_ROWS rows (two passes of the workgroup):
@@ -796,15 +802,31 @@ class DevPrecT : public DevPrec {
   // One sync-free substitution over a whole schedule (nch chunks) of an
   // n-row factor: t filled with the sentinel, then the solve.
   template <int SC, int BT, bool TR, int LN>
-  void sf_launch(DSmoother<V>& S, int n, int nch, const int* ch, const int* rows, const DMat<V>& F, const V* b,
-                 const V* d, V* t, V* y) {
+  void sf_launch(DSmoother<V>& S, int n, int nch, int nlv, const int* ch, const int* rows, const DMat<V>& F,
+                 const V* b, const V* d, V* t, V* y) {
     if (nch <= 0) return;
     sf_fill_kernel<V><<<nblk(n), BS, 0, s_>>>(n, t);
-    const int nb = std::min(nch, SF_BLOCKS);
+    const int nb = sf_grid(nch, nlv);
     trsv_sf_kernel<V, SC, BT, TR, LN><<<nb, BS, 0, s_>>>(nch, ch, rows, F.irp, F.ja, F.val, b, d, t, y,
                                                           S.sf_ticket, scal_ + S_ABORT, SF_SPIN_MAX);
     HIPCHECK(hipGetLastError());
   }
+  // Workgroups of a sync-free substitution: enough to hold the chunks of
+  // SF_AHEAD level sets at once, at least SF_MIN_BLOCKS, at most SF_BLOCKS
+  // (MLD_SF_AHEAD, MLD_SF_BLOCKS override; 0 ahead: no such limit).  Every
+  // workgroup beyond that only spins on values not yet written, and the
+  // spinning loads crowd the L2 that the producers' stores pass through: at
+  // 256^3 (run B) the V-cycle takes 89 ms with 2048 workgroups per solve,
+  // 57 ms with two level sets ahead and at most 512.
+  static int sf_grid(int nch, int nlv) {
+    int ahead = SF_AHEAD, cap = SF_BLOCKS;
+    if (const char* e = std::getenv("MLD_SF_AHEAD")) ahead = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("MLD_SF_BLOCKS")) cap = std::max(1, std::atoi(e));
+    int nb = std::min(nch, cap);
+    if (ahead > 0 && nlv > 0) nb = std::min(nb, std::max(SF_MIN_BLOCKS, ahead * ((nch + nlv - 1) / nlv)));
+    return std::max(1, nb);
+  }
+
   // The level-scheduled launches instead (MLD_TRSV=level).
   static bool sync_free() {
     static const bool sf = [] {
@@ -813,7 +835,9 @@ class DevPrecT : public DevPrec {
     }();
     return sf;
   }
-  static constexpr int SF_BLOCKS = 2048;
+  static constexpr int SF_BLOCKS = 512;
+  static constexpr int SF_MIN_BLOCKS = 64;
+  static constexpr int SF_AHEAD = 2;
   static constexpr int SF_SPIN_MAX = 1 << 22;
 
   // Hybrid Gauss-Seidel smoother (jacobi smoother + GS / BWGS solver, which
@@ -869,9 +893,9 @@ class DevPrecT : public DevPrec {
     if (sync_free()) {
       const int* ch = lower ? S.lch : S.uch;
       const int nch = lower ? S.nlch : S.nuch;
-      if (bt == 0) sf_launch<2, 0, false, LN>(S, D.n, nch, ch, rows, T, b, S.dv, t, y);
-      else if (bt == 1) sf_launch<2, 1, false, LN>(S, D.n, nch, ch, rows, T, b, S.dv, t, y);
-      else sf_launch<2, 2, false, LN>(S, D.n, nch, ch, rows, T, b, S.dv, t, y);
+      if (bt == 0) sf_launch<2, 0, false, LN>(S, D.n, nch, nl, ch, rows, T, b, S.dv, t, y);
+      else if (bt == 1) sf_launch<2, 1, false, LN>(S, D.n, nch, nl, ch, rows, T, b, S.dv, t, y);
+      else sf_launch<2, 2, false, LN>(S, D.n, nch, nl, ch, rows, T, b, S.dv, t, y);
       return;
     }
     constexpr int RUN_ROWS = 2048 / LN;

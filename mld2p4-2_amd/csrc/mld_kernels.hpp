@@ -501,6 +501,82 @@ __global__ __launch_bounds__(BS) void sell_kernel(int nr, const int* __restrict_
   if constexpr (K > 0) block_partials<K>(d, red.partial, lb);
 }
 
+// SELD: the SELL product with 16-bit column steps (entry k's column is the
+// previous one + dja[k] + 1, starting from col0 - 1; padding entries, value
+// zero, keep stepping by one and gather the clamped column).  Same stored
+// order as SELL, so the same sums.
+template <typename V, int K, class G, class E, typename S>
+__global__ __launch_bounds__(BS) void seld_kernel(int nr, int nc, const int* __restrict__ sptr,
+                                                  const int* __restrict__ perm, const int* __restrict__ col0,
+                                                  const uint16_t* __restrict__ dja, const S* __restrict__ val, G g,
+                                                  E e, Red red, int xchunk) {
+  const int lb = xcd_block(xchunk);
+  const int p = lb * BS + threadIdx.x;
+  double d[2] = {0.0, 0.0};
+  if (p < nr) {
+    const int s = p >> 6, l = p & 63;
+    const int base = sptr[s] + l;
+    const int w = (sptr[s + 1] - sptr[s]) >> 6;
+    const int cmax = nc - 1;
+    int col = col0[p] - 1;
+    V acc = 0;
+    int k = 0;
+    for (; k + 8 <= w; k += 8) {
+      const int i0 = base + 64 * k;
+      int j[8];
+      V v[8], x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) j[u] = (int)ldnt(dja + i0 + 64 * u);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = (V)ldnt(val + i0 + 64 * u);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        col += j[u] + 1;
+        j[u] = min(col, cmax);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = g(j[u]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = acc + v[u] * x[u];
+    }
+    for (; k < w; ++k) {
+      const int i0 = base + 64 * k;
+      col += (int)ldnt(dja + i0) + 1;
+      acc = acc + (V)ldnt(val + i0) * g(min(col, cmax));
+    }
+    e(perm[p], acc, d);
+  }
+  if constexpr (K > 0) block_partials<K>(d, red.partial, lb);
+}
+
+// SELL column indices (ja, stored order per position) -> SELD steps and
+// first columns; positions past nr and entries past a row's length are
+// padding (step 0).  Rows are strictly increasing with steps <= 65536
+// (checked on the host before this layout is built).
+__global__ __launch_bounds__(BS) void seld_convert_kernel(int nr, int npos, const int* __restrict__ perm,
+                                                          const int* __restrict__ sptr, const int* __restrict__ irp,
+                                                          const int* __restrict__ ja, int* __restrict__ col0,
+                                                          uint16_t* __restrict__ dja) {
+  const int p = blockIdx.x * BS + threadIdx.x;
+  if (p >= npos) return;
+  const int s = p >> 6, l = p & 63;
+  const int base = sptr[s] + l;
+  const int w = (sptr[s + 1] - sptr[s]) >> 6;
+  const int len = (p < nr) ? irp[perm[p] + 1] - irp[perm[p]] : 0;
+  int prev = (len > 0 ? ja[base] : 0) - 1;
+  col0[p] = prev + 1;
+  for (int k = 0; k < w; ++k) {
+    const int i = base + 64 * k;
+    if (k < len) {
+      const int c = ja[i];
+      dja[i] = (uint16_t)(c - prev - 1);
+      prev = c;
+    } else {
+      dja[i] = 0;
+    }
+  }
+}
+
 // --------------------------------------------------------- vector kernels
 template <typename V, typename XT, bool ADD>
 __global__ __launch_bounds__(BS) void vmul_kernel(int n, const V* __restrict__ d, const XT* __restrict__ x,
@@ -1089,6 +1165,27 @@ __global__ __launch_bounds__(BS) void col_store_kernel(int n, const V* __restric
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i < n) B[(size_t)i * n + j] = y[i];
 }
+// ND D of a SELD operator: val times c at each entry's column (decoded as
+// in seld_kernel), one thread per position
+template <typename V>
+__global__ __launch_bounds__(BS) void seld_colscale_kernel(int npos, int nc, const int* __restrict__ sptr,
+                                                           const int* __restrict__ col0,
+                                                           const uint16_t* __restrict__ dja,
+                                                           const V* __restrict__ val, const V* __restrict__ c,
+                                                           V* __restrict__ oval) {
+  const int p = blockIdx.x * BS + threadIdx.x;
+  if (p >= npos) return;
+  const int s = p >> 6, l = p & 63;
+  const int base = sptr[s] + l;
+  const int w = (sptr[s + 1] - sptr[s]) >> 6;
+  int col = col0[p] - 1;
+  for (int k = 0; k < w; ++k) {
+    const int i = base + 64 * k;
+    col += (int)dja[i] + 1;
+    oval[i] = val[i] * c[min(col, nc - 1)];
+  }
+}
+
 // oval[k] = val[k] * c[ja[k]] over every stored entry of an ELL, SELL or CSR
 // operator (padding entries hold a zero value on a valid column)
 template <typename V>

@@ -14,9 +14,12 @@ DIAG_SCALE, ILU_N, L1_DIAG, DENSE_LU, F_NONE = 5, 7, 101, 102, 4
 
 def mat_bytes(nr, nnz, sv, fmt):
     """Compulsory bytes of one pass over a stored operator: values and column
-    indices (CSR also row pointers; DIA stores no indices)."""
+    indices (CSR also row pointers; DIA stores no indices; SELD stores 16-bit
+    column steps and one 32-bit first column per row)."""
     if fmt == "DIA":
         return nnz * sv
+    if fmt == "SELD":
+        return nnz * (sv + 2) + nr * 4
     b = nnz * (sv + 4)
     if fmt == "CSR":
         b += (nr + 1) * 4

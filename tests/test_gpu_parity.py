@@ -72,7 +72,7 @@ def test_apply_fp64(eng, orc, idim, smoother, sweeps):
         assert relmax(y, yo) <= 1e-12
 
 
-@pytest.mark.parametrize("fmt", ["auto", "csr", "ell", "sell", "dia"])
+@pytest.mark.parametrize("fmt", ["auto", "csr", "ell", "sell", "dia", "seld", "seld4096"])
 def test_level_spmv_bitexact(eng, orc, fmt):
     _say(f"[level spmv] fmt={fmt}")
     a = orc.gen_pde3d(20)
@@ -92,7 +92,9 @@ def test_level_spmv_bitexact(eng, orc, fmt):
             yo = m.spmv(x)
             yg = y.cpu().numpy()
             f = p.op_format(lev, what)
-            if f["fmt"] in ("ELL", "SELL", "DIA") or f["lanes"] == 1:
+            if fmt.startswith("seld"):
+                assert f["fmt"] == "SELD", (lev, what, f)
+            if f["fmt"] in ("ELL", "SELL", "DIA", "SELD") or f["lanes"] == 1:
                 # one lane per row, stored order, no fused multiply-add:
                 # identical to the sequential CSR loop
                 assert (yg == yo).all(), (lev, what, f)
