@@ -67,7 +67,8 @@ int mldhip_set_format(mldhip_prec* p, int fmt);
 /* Where hierarchy_build forms the smoothed-aggregation products P, A P, R
  * and Ac of each level (mld_daggrmat_smth_asb.f90:291,345,399; aggregation
  * always runs on the host): 0 host threads, 1 the handle's device, -1
- * (default) the device when the handle has one and MLD_SETUP is not "host".
+ * (default) the device for the products over levels of at least 2^22 rows
+ * when the handle has one and MLD_SETUP is not "host", the host otherwise.
  * Both give bit-identical hierarchies.  No reference counterpart. */
 int mldhip_set_setup(mldhip_prec* p, int where);
 /* prec%hierarchy_build(a, desc, info): mlprec/impl/mld_d_hierarchy_bld.f90:66-490. */

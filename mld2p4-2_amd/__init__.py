@@ -206,7 +206,8 @@ class MldPrec:
 
     def set_setup(self, where: str):
         """Where hierarchy_build forms the level products: 'host', 'device'
-        or 'auto' (the device when the handle has one)."""
+        or 'auto' (the device for levels of 2^22+ rows when the handle has
+        one)."""
         self._check(lib().mldhip_set_setup(self.h, {"auto": -1, "host": 0, "device": 1}[where]), "set_setup")
         return self
 

@@ -824,13 +824,15 @@ int hierarchy_build(Prec& p, int n, const int64_t* irp, const int* ja, const dou
       tgt.parms.aggr_omega_val = aomega;
       tgt.sm = coarse_sm; tgt.sm2a = coarse_sm2a;
       if (newsz < i) { ilaggr = tgt.ilaggr; naggr = tgt.naggr; }
-      rc = aggrmat_smth_asb(v[newsz - 2].a, ilaggr, naggr, tgt.parms, tgt.a, tgt.p, tgt.r, p.setup_device);
+      rc = aggrmat_smth_asb(v[newsz - 2].a, ilaggr, naggr, tgt.parms, tgt.a, tgt.p, tgt.r,
+                            v[newsz - 2].a.nr >= p.setup_min_rows ? p.setup_device : -1);
       if (rc) return rc;
       tgt.ilaggr = std::move(ilaggr);
       tgt.naggr = naggr;
       break;
     }
-    rc = aggrmat_smth_asb(prev.a, ilaggr, naggr, cur.parms, cur.a, cur.p, cur.r, p.setup_device);
+    rc = aggrmat_smth_asb(prev.a, ilaggr, naggr, cur.parms, cur.a, cur.p, cur.r,
+                          prev.a.nr >= p.setup_min_rows ? p.setup_device : -1);
     if (rc) return rc;
     cur.ilaggr = std::move(ilaggr);
     cur.naggr = naggr;

@@ -146,6 +146,11 @@ int mldhip_hierarchy_build(mldhip_prec* p, int n, const int64_t* irp, const int*
     const bool host_env = e && std::string(e) == "host";
     const bool dev = p->device >= 0 && (p->setup == 1 || (p->setup == -1 && !host_env));
     p->host.setup_device = dev ? p->device : -1;
+    // automatic: the device only where its fixed costs (allocations, sorts,
+    // synchronisations) are repaid: products over a level of 2^22+ rows
+    // (at 256^3 the first level's products 0.30 s on the device; below a few
+    // million rows the host's threads are faster)
+    p->host.setup_min_rows = (p->setup == 1) ? 0 : ((int64_t)1 << 22);
   }
   int rc = mld::hierarchy_build(p->host, n, irp, ja, val);
   if (rc) return fail(p, rc, p->host.last_error.empty() ? "hierarchy_build failed" : p->host.last_error);

@@ -148,8 +148,10 @@ struct Prec {
   int value_kind = 0;         // 0 = FP64 hierarchy on device, 1 = FP32 (mixed)
   int dev_format = 0;         // 0 = autotuned, 1 = force CSR, 2 = force ELL
   std::vector<StagedLevel> staged;
-  // device of the smoothed-aggregation products of hierarchy_build (-1: host)
+  // device of the smoothed-aggregation products of hierarchy_build (-1: host),
+  // used for the levels whose finer operator has at least setup_min_rows rows
   int setup_device = -1;
+  int64_t setup_min_rows = 0;
   std::string last_error;
 };
 
