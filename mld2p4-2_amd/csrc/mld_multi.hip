@@ -927,12 +927,12 @@ class MultiT : public MultiEngine {
         D.nnz_nd = H.nd.nnz();
         if (D.nnz_nd > 0) D.nd = up(H.nd);
         D.dv = upv(H.dv);
-        if (D.nnz_nd > 0 && D.nd.fmt == FMT_DIA) D.ndd = scaled_cols(D.nd, D.dv);
+        if (D.nnz_nd > 0) D.ndd = scaled_cols(D.nd, D.dv);
         if (D.has2) {
           D.nnz_nd2 = H.nd2.nnz();
           if (D.nnz_nd2 > 0) D.nd2 = up(H.nd2);
           D.dv2 = upv(H.dv2);
-          if (D.nnz_nd2 > 0 && D.nd2.fmt == FMT_DIA) D.ndd2 = scaled_cols(D.nd2, D.dv2);
+          if (D.nnz_nd2 > 0) D.ndd2 = scaled_cols(D.nd2, D.dv2);
         }
       }
       if (l > 0) {
@@ -1169,13 +1169,21 @@ class MultiT : public MultiEngine {
       return m;
     }
   }
-  // ND D for a DIA ND: the first Jacobi sweep from zero, ND (D x), becomes
-  // the plain product (ND D) x (one gather per entry; the rounding of each
-  // product is re-associated, (a_ij d_j) x_j for a_ij (d_j x_j))
+  // ND D (ND with its columns scaled by d, sharing ND's indices): the first
+  // Jacobi sweep from zero, ND (D x), becomes the plain product (ND D) x (one
+  // gather per entry, no separate D x pass; the rounding of each product is
+  // re-associated, (a_ij d_j) x_j for a_ij (d_j x_j))
   DMat<V> scaled_cols(const DMat<V>& m, const V* d) {
     DMat<V> r = m;
-    r.val = ar_.alloc<V>((size_t)m.nr * m.w);
-    dia_colscale_kernel<V><<<nblk(m.nr), BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, d, r.val);
+    r.val = ar_.alloc<V>((size_t)std::max<int64_t>(1, m.stored));
+    if (m.fmt == FMT_DIA) {
+      dia_colscale_kernel<V><<<nblk(m.nr), BS, 0, s_>>>(m.nr, m.nc, m.w, m.off, m.val, d, r.val);
+    } else if (m.fmt == FMT_SELD) {
+      const int npos = (m.nr + 63) / 64 * 64;
+      seld_colscale_kernel<V><<<nblk(npos), BS, 0, s_>>>(npos, m.nc, m.sptr, m.col0, m.dja, m.val, d, r.val);
+    } else if (m.stored > 0) {
+      colscale_kernel<V><<<(int)((m.stored + BS - 1) / BS), BS, 0, s_>>>(m.stored, m.ja, m.val, d, r.val);
+    }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s_));
     return r;
