@@ -734,15 +734,24 @@ class DevPrecT : public DevPrec {
     const int nl1 = (int)p1.size() - 1, nl2 = (int)p2.size() - 1;
     V* t1 = D.tx;
     V* t2 = beta ? D.t2 : y;
-    if (n <= 8192) {
+    if (n <= SMALL_TRI) {
       // one workgroup; a single wavefront when the matrix is tiny
       const int nt = std::min(1024, (int)(((int64_t)n * LN + 63) / 64) * 64);
-      if (beta)
+      const size_t lds = 2 * (size_t)n * sizeof(V);  // both intermediate vectors in LDS when they fit
+      if (lds <= ILU_LDS_MAX) {
+        if (beta)
+          ilu_small_kernel<V, 1, TR, LN, true><<<1, nt, lds, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val,
+                                                                   S.dv, p1d, r1, nl1, p2d, r2, nl2, x, t1, t2, y);
+        else
+          ilu_small_kernel<V, 0, TR, LN, true><<<1, nt, lds, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val,
+                                                                   S.dv, p1d, r1, nl1, p2d, r2, nl2, x, t1, t2, y);
+      } else if (beta) {
         ilu_small_kernel<V, 1, TR, LN><<<1, nt, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d,
                                                          r1, nl1, p2d, r2, nl2, x, t1, t2, y);
-      else
+      } else {
         ilu_small_kernel<V, 0, TR, LN><<<1, nt, 0, s_>>>(n, F1.irp, F1.ja, F1.val, F2.irp, F2.ja, F2.val, S.dv, p1d,
                                                          r1, nl1, p2d, r2, nl2, x, t1, t2, y);
+      }
       HIPCHECK(hipGetLastError());
       return;
     }
@@ -835,6 +844,8 @@ class DevPrecT : public DevPrec {
     }();
     return sf;
   }
+  static constexpr size_t ILU_LDS_MAX = 64 * 1024;  // LDS of one workgroup for the small ILU solves
+  static constexpr int SMALL_TRI = 8192;  // triangular solves up to this many rows run in one workgroup
   static constexpr int SF_BLOCKS = 512;
   static constexpr int SF_MIN_BLOCKS = 64;
   static constexpr int SF_AHEAD = 2;
@@ -886,7 +897,7 @@ class DevPrecT : public DevPrec {
       else if (bt == 1) trsv_run_kernel<V, 2, 1, false, LN><<<1, 1024, 0, s_>>>(l0, l1, pd, rows, T.irp, T.ja, T.val, b, S.dv, t, y);
       else trsv_run_kernel<V, 2, 2, false, LN><<<1, 1024, 0, s_>>>(l0, l1, pd, rows, T.irp, T.ja, T.val, b, S.dv, t, y);
     };
-    if (D.n <= 8192) {  // whole solve in one workgroup
+    if (D.n <= SMALL_TRI) {  // whole solve in one workgroup
       run(0, nl);
       return;
     }

@@ -973,28 +973,34 @@ __global__ __launch_bounds__(BS) void trsv_sf_kernel(int nchunks, const int* __r
 
 // Whole ILU(0) apply for a small matrix in one workgroup: level sets
 // processed in order with a barrier in between.
-template <typename V, int BETA, bool TR, int LANES>
+template <typename V, int BETA, bool TR, int LANES, bool LDS = false>
 __global__ __launch_bounds__(1024) void ilu_small_kernel(
     int n, const int* __restrict__ lirp, const int* __restrict__ lja, const V* __restrict__ lval,
     const int* __restrict__ uirp, const int* __restrict__ uja, const V* __restrict__ uval,
     const V* __restrict__ d, const int* __restrict__ lptr, const int* __restrict__ lrows, int nlevl,
     const int* __restrict__ uptr, const int* __restrict__ urows, int nlevu, const V* __restrict__ x,
     V* t1, V* t2, V* __restrict__ y) {
+  // LDS: both intermediate vectors live in the workgroup's LDS (2 n values,
+  // dynamic shared memory), so a level set's reads of the previous ones
+  // never leave the CU; the result goes to y only
+  extern __shared__ double ilu_lds[];
+  V* s1 = LDS ? (V*)ilu_lds : t1;
+  V* s2 = LDS ? (V*)ilu_lds + n : t2;
   const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES, ngrp = blockDim.x / LANES;
   for (int lv = 0; lv < nlevl; ++lv) {
     for (int q = lptr[lv] + grp; q < lptr[lv + 1]; q += ngrp) {
       const int i = lrows[q];
-      const V ti = tri_row<V, 0, TR, LANES>(i, lane, lirp, lja, lval, x, nullptr, t1);
-      if (lane == 0) t1[i] = ti;
+      const V ti = tri_row<V, 0, TR, LANES>(i, lane, lirp, lja, lval, x, nullptr, s1);
+      if (lane == 0) s1[i] = ti;
     }
     __syncthreads();
   }
   for (int lv = 0; lv < nlevu; ++lv) {
     for (int q = uptr[lv] + grp; q < uptr[lv + 1]; q += ngrp) {
       const int i = urows[q];
-      const V ti = tri_row<V, 1, TR, LANES>(i, lane, uirp, uja, uval, t1, d, t2);
+      const V ti = tri_row<V, 1, TR, LANES>(i, lane, uirp, uja, uval, s1, d, s2);
       if (lane == 0) {
-        t2[i] = ti;
+        s2[i] = ti;
         if (BETA == 0) y[i] = ti;
         else y[i] = y[i] + ti;
       }
