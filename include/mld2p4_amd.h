@@ -57,7 +57,7 @@ int mldhip_set_precision(mldhip_prec* p, int kind);
  * 4096), 4 = SELL with windows of 256 rows, 5 = SELL without sorting, 6 = DIA
  * (diagonal storage, for operators with at most 8 distinct diagonals; others
  * fall back to the automatic choice), 7 = SELL with windows of 128 rows,
- * 8 / 9 = SELD (SELL with windows of 256 / 4096 rows whose column indices are
+ * 8 / 9 / 10 = SELD (SELL with windows of 256 / 4096 / 128 rows whose column indices are
  * 16-bit steps from the previous column of the row; operators with a larger
  * step fall back to SELL).  The automatic choice takes DIA for such banded
  * operators, else the layout it times fastest. */

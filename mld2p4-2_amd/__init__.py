@@ -201,7 +201,8 @@ class MldPrec:
 
     def set_format(self, fmt: str):
         self._check(lib().mldhip_set_format(self.h, {"auto": 0, "csr": 1, "ell": 2, "sell": 3, "sell256": 4, "sellu": 5, "dia": 6,
-                                                          "sell128": 7, "seld": 8, "seld4096": 9}[fmt]), "set_format")
+                                                          "sell128": 7, "seld": 8, "seld4096": 9,
+                                                          "seld128": 10}[fmt]), "set_format")
         return self
 
     def set_setup(self, where: str):

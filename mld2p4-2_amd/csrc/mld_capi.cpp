@@ -125,7 +125,7 @@ int mldhip_set_precision(mldhip_prec* p, int kind) {
   return 0;
 }
 int mldhip_set_format(mldhip_prec* p, int fmt) {
-  if (!p || fmt < 0 || fmt > 9) return E_BAD_ARG;
+  if (!p || fmt < 0 || fmt > 10) return E_BAD_ARG;
   p->host.dev_format = fmt;
   return 0;
 }

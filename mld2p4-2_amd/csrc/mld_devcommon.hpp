@@ -161,7 +161,7 @@ static bool dia_eligible(const HCsr& h, const std::vector<int>& offs) {
   return !offs.empty() && h.nr == h.nc && (double)h.nr * offs.size() <= 1.10 * (double)h.nnz();
 }
 
-// SELD (force 8: sigma 256, force 9: sigma 4096): SELL with each row's
+// SELD (force 8: sigma 256, force 9: sigma 4096, force 10: sigma 128): SELL with each row's
 // column indices stored as 16-bit steps between consecutive stored columns
 // (step - 1, so 1 .. 65536), and the first column per position.  Entries
 // become 10 bytes instead of 12 (FP64), 6 instead of 8 (FP32); the products
@@ -196,7 +196,7 @@ static FmtChoice choose_format(const HCsr& h, int force) {
   }
   if (force == 2 && maxrow <= 64) { c.fmt = FMT_ELL; return c; }
   if ((force >= 3 && force <= 5) || force == 7) { c.fmt = FMT_SELL; return c; }
-  if (force == 8 || force == 9) {  // SELL where the steps do not fit 16 bits
+  if (force >= 8 && force <= 10) {  // SELL where the steps do not fit 16 bits
     c.fmt = host_seld_ok(h) ? FMT_SELD : FMT_SELL;
     return c;
   }
@@ -213,7 +213,7 @@ static FmtChoice choose_format(const HCsr& h, int force) {
 // Galerkin operator at 256^3: padding 1.16x unsorted, 1.09x at 128 with the
 // same gathered cache lines per entry, 1.05x at 256 with 16 % more).
 static int sell_sigma(int force) {
-  return (force == 4 || force == 8) ? 256 : (force == 5 ? 1 : (force == 7 ? 128 : 4096));
+  return (force == 4 || force == 8) ? 256 : (force == 5 ? 1 : ((force == 7 || force == 10) ? 128 : 4096));
 }
 
 // An operator uploaded once as CSR (int32 row pointers, FP64 values); every
@@ -539,6 +539,7 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
       if (host_seld_ok(h)) {
         cands.push_back({8, 0, true});
         cands.push_back({9, 0, true});
+        cands.push_back({10, 0, true});
       }
       for (int l = 1; l <= 64; l *= 2)
         if (l == 1 || l <= avg) cands.push_back({1, l, l == 1});

@@ -72,7 +72,7 @@ def test_apply_fp64(eng, orc, idim, smoother, sweeps):
         assert relmax(y, yo) <= 1e-12
 
 
-@pytest.mark.parametrize("fmt", ["auto", "csr", "ell", "sell", "dia", "seld", "seld4096"])
+@pytest.mark.parametrize("fmt", ["auto", "csr", "ell", "sell", "dia", "seld", "seld4096", "seld128"])
 def test_level_spmv_bitexact(eng, orc, fmt):
     _say(f"[level spmv] fmt={fmt}")
     a = orc.gen_pde3d(20)
