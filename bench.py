@@ -202,6 +202,9 @@ def cpu_baseline(idim, gpu_iters, sample_iters=5):
     info = cpu_info()
     a = O.gen_pde3d(idim)
     prm = O.config1_params(smoother=O.SM_L1JAC, sweeps_pre=2, sweeps_post=2)
+    # the thread count is process-wide OpenMP state (the engine's host build
+    # shares it): restored on the way out
+    threads0 = O.set_threads(0)
     O.set_threads(info["threads_all_cores"])
     po = O.Prec(prm).build(a)
     tb = po.build_s
@@ -211,6 +214,7 @@ def cpu_baseline(idim, gpu_iters, sample_iters=5):
         O.set_threads(cores)
         _, it, _, its, _ = po.krylov("CG", b, eps=1e-30, itmax=1 + sample_iters, timings=True)
         per_it[cores] = float(np.median(its[1:]))
+    O.set_threads(threads0)
     one, allc = per_it[1], per_it[info["threads_all_cores"]]
     return {
         "value": round(one * gpu_iters, 3),
@@ -235,6 +239,7 @@ def section_c1(eng, torch, problems):
     GPU path, setup (hierarchy_build + smoothers_build) and solve each as the
     median of 5 after one warm-up."""
     O = _oracle()
+    threads0 = O.set_threads(0)  # process-wide OpenMP state, restored below
     O.set_threads(1)
     idim = 40
     a = O.gen_pde3d(idim)
@@ -246,6 +251,7 @@ def section_c1(eng, torch, problems):
         if k:
             setups.append(po.build_s)
             solves.append(tot)
+    O.set_threads(threads0)
     irp, ja, val = problems.pde3d_poisson(idim)
     bd = torch.ones(len(irp) - 1, dtype=torch.float64, device="cuda:0")
     gsetups, gsolves, itg = [], [], 0
