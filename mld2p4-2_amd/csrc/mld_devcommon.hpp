@@ -243,9 +243,18 @@ static DevSrc make_src(const HCsr& h, DevArena& tmp, hipStream_t s) {
   return d;
 }
 
+// Host SELL layouts (position -> row, slice offsets) by sort window, kept
+// while the autotuner builds its candidates (SELL and SELD of one window
+// share a layout).
+struct SellLayouts {
+  std::vector<int> sigma;
+  std::vector<std::vector<int>> perm, sptr;
+};
+
 // Layout c of the operator h (uploaded as src) in arena ar.
 template <typename V>
-static DMat<V> build_mat(const HCsr& h, const DevSrc& src, DevArena& ar, hipStream_t s, FmtChoice c, int force) {
+static DMat<V> build_mat(const HCsr& h, const DevSrc& src, DevArena& ar, hipStream_t s, FmtChoice c, int force,
+                         SellLayouts* cache = nullptr) {
   DMat<V> m;
   m.nr = h.nr; m.nc = h.nc; m.nnz = h.nnz();
   m.fmt = c.fmt; m.lanes = c.lanes; m.w = c.w;
@@ -269,9 +278,28 @@ static DMat<V> build_mat(const HCsr& h, const DevSrc& src, DevArena& ar, hipStre
   } else if (c.fmt == FMT_SELL || c.fmt == FMT_SELD) {
     const int nr = h.nr;
     const int nsl = (nr + SELL_C - 1) / SELL_C;
-    std::vector<int> perm, sptr;
-    if (!host_sell_layout(h, sell_sigma(force), SELL_C, src.maxrow, perm, sptr))
-      throw DevError("SELL storage exceeds 2^31 entries");
+    std::vector<int> perm_own, sptr_own;
+    const std::vector<int>* pp = &perm_own;
+    const std::vector<int>* sp = &sptr_own;
+    const int sigma = sell_sigma(force);
+    size_t ci = cache ? cache->sigma.size() : 0;
+    if (cache)
+      for (size_t k = 0; k < cache->sigma.size(); ++k)
+        if (cache->sigma[k] == sigma) ci = k;
+    if (cache && ci < cache->sigma.size()) {
+      pp = &cache->perm[ci];
+      sp = &cache->sptr[ci];
+    } else {
+      if (!host_sell_layout(h, sigma, SELL_C, src.maxrow, perm_own, sptr_own))
+        throw DevError("SELL storage exceeds 2^31 entries");
+      if (cache) {
+        cache->sigma.push_back(sigma);
+        cache->perm.push_back(perm_own);
+        cache->sptr.push_back(sptr_own);
+      }
+    }
+    const std::vector<int>& perm = *pp;
+    const std::vector<int>& sptr = *sp;
     const size_t tot = (size_t)sptr[nsl];
     m.stored = (int64_t)tot;
     m.sptr = ar.upload(sptr.data(), sptr.size(), s);
@@ -521,6 +549,7 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
   hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
   const DevSrc src = make_src(h, tmp, s);
   int bforce = 6, blanes = 0;
+  SellLayouts layouts;
   std::vector<int> offs = dia_offsets(h);
   const bool dia = dia_eligible(h, offs);
   if (!dia) {  // DIA reads strictly fewer bytes than any other layout
@@ -536,7 +565,8 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
       cands.push_back({4, 0, true});
       cands.push_back({5, 0, true});
       cands.push_back({7, 0, true});
-      if (host_seld_ok(h)) {
+      // (rows of a few entries: the per-row first column eats the saving)
+      if (avg >= 8.0 && host_seld_ok(h)) {
         cands.push_back({8, 0, true});
         cands.push_back({9, 0, true});
         cands.push_back({10, 0, true});
@@ -550,7 +580,7 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
       DevArena ctmp;
       FmtChoice fc = choose_format(h, cands[c].force);
       if (cands[c].lanes > 0) { fc.fmt = FMT_CSR; fc.lanes = cands[c].lanes; }
-      DMat<V> m = build_mat<V>(h, src, ctmp, s, fc, cands[c].force);
+      DMat<V> m = build_mat<V>(h, src, ctmp, s, fc, cands[c].force, &layouts);
       const double t = time_spmv(m, x, y, s, e0, e1) * (cands[c].ordered ? 1.0 : 1.03);
       if (t < best) { best = t; bi = (int)c; }
     }
@@ -564,7 +594,7 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
   }
   FmtChoice fb = dia ? FmtChoice{FMT_DIA, 1, (int)offs.size(), offs} : choose_format(h, bforce);
   if (blanes > 0) { fb.fmt = FMT_CSR; fb.lanes = blanes; }
-  DMat<V> m = build_mat<V>(h, src, ar, s, fb, bforce);
+  DMat<V> m = build_mat<V>(h, src, ar, s, fb, bforce, &layouts);
   // row-block order of the chosen layout: 8192-row chunks per XCD (default),
   // identity, or one contiguous range per XCD; the default wins ties within 2 %
   // (MLD_XMAP=0/1/2 fixes it instead)
