@@ -407,6 +407,14 @@ static void csr_launch_l(const DMat<V>& m, int nb, G g, E e, hipStream_t s, Red 
 // ---------------------------------------------------------------------------
 // DIA levels from this many rows store their outputs non-temporally
 constexpr int NT_MIN_ROWS = 1 << 22;
+// (MLD_NT_MIN_ROWS overrides it, for measurements)
+static int nt_min_rows() {
+  static const int v = [] {
+    const char* e = std::getenv("MLD_NT_MIN_ROWS");
+    return e ? std::atoi(e) : NT_MIN_ROWS;
+  }();
+  return v;
+}
 
 // epilogues with a two-row form (pair / aligned2)
 template <class E, class = void>
@@ -448,7 +456,7 @@ static void spmv_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red = Red
   if (m.nr == 0) return;
   if (m.fmt == FMT_DIA) {
     if constexpr (!std::is_void<typename NtForm<E>::type>::value) {
-      if (m.nr >= NT_MIN_ROWS) {
+      if (m.nr >= nt_min_rows()) {
         dia_launch<K, C>(m, g, NtForm<E>::make(e), s, red);
         return;
       }

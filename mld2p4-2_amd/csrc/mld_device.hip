@@ -1393,16 +1393,16 @@ int DevPrecT<V>::krylov(int method, const double* b, double* x, double eps, int 
         const V* z32 = nullptr;
         int rc = apply_keep32(r, it > 1 ? q : nullptr, S_GAMMA, &z32);
         if (rc) { flush_x(); return rc; }
-        fcg_pupdate<<<nbd, BS, 0, s_>>>(n, z32, p, r, scal_, it, red(S_PR), x, xnum, xden);
+        fcg_pupdate<<<nblk(n), BS, 0, s_>>>(n, z32, p, scal_, it, x, xnum, xden);
       } else {
         int rc = apply_impl(r, z, trans, it > 1 ? q : nullptr, S_GAMMA);
         if (rc) { flush_x(); return rc; }
-        fcg_pupdate<<<nbd, BS, 0, s_>>>(n, (const double*)z, p, r, scal_, it, red(S_PR), x, xnum, xden);
+        fcg_pupdate<<<nblk(n), BS, 0, s_>>>(n, (const double*)z, p, scal_, it, x, xnum, xden);
       }
-      finalize(1, S_PR);
-      kspmv<1>(trans, GatherX<double, double>{p}, EpiSetDot<double>{q, p, nullptr}, red(S_DELTA));
-      r_update<<<nbd, BS, 0, s_>>>(n, r, q, scal_, S_PR, S_DELTA, red(S_RR), r32);
-      xnum = S_PR;
+      // q = A p with p.q (S_DELTA) and p.r (S_DPR) in one pass
+      kspmv<2>(trans, GatherX<double, double>{p}, EpiSetDotUW<double>{q, p, r}, red(S_DELTA));
+      r_update<<<nbd, BS, 0, s_>>>(n, r, q, scal_, S_DPR, S_DELTA, red(S_RR), r32);
+      xnum = S_DPR;
       xden = S_DELTA;
       preconv_ = r32 ? r : nullptr;
       finalize(1, S_RR);

@@ -191,6 +191,31 @@ struct EpiSetDot {
   bool aligned2() const { return al2(y, sizeof(V)) && (!u || al2(u, sizeof(V))) && (!w || al2(w, sizeof(V))); }
 };
 
+// FCG's Krylov SpMV: y = A x with x = u (the direction p); dots u.y (p.Ap)
+// and u.w (p.r with w = r, the step's numerator, taken here where p is read
+// anyway instead of in a pass of its own)
+template <typename V, bool NT = false>
+struct EpiSetDotUW {
+  V* y;
+  const V* u;
+  const V* w;
+  __device__ __forceinline__ void operator()(int i, V acc, double* d) const {
+    stnt<NT>(y + i, acc);
+    const double ui = (double)u[i];
+    d[0] = ui * (double)acc;
+    d[1] = ui * (double)w[i];
+  }
+  __device__ __forceinline__ void pair(int i, V a0, V a1, double* d) const {
+    stnt<NT>((vec2_t<V>*)(y + i), vec2_t<V>{a0, a1});
+    const vec2_t<V> uu = *(const vec2_t<V>*)(u + i), ww = *(const vec2_t<V>*)(w + i);
+    d[0] += (double)uu.x * (double)a0;
+    d[0] += (double)uu.y * (double)a1;
+    d[1] += (double)uu.x * (double)ww.x;
+    d[1] += (double)uu.y * (double)ww.y;
+  }
+  bool aligned2() const { return al2(y, sizeof(V)) && al2(u, sizeof(V)) && al2(w, sizeof(V)); }
+};
+
 // The NT form of an epilogue (same data, non-temporal stores).
 template <class E>
 struct NtForm { using type = void; };
@@ -223,6 +248,11 @@ template <typename V>
 struct NtForm<EpiSetDot<V, false>> {
   using type = EpiSetDot<V, true>;
   static type make(const EpiSetDot<V, false>& e) { return type{e.y, e.u, e.w}; }
+};
+template <typename V>
+struct NtForm<EpiSetDotUW<V, false>> {
+  using type = EpiSetDotUW<V, true>;
+  static type make(const EpiSetDotUW<V, false>& e) { return type{e.y, e.u, e.w}; }
 };
 
 // ------------------------------------------------------------- reductions
@@ -658,10 +688,10 @@ __global__ __launch_bounds__(BS) void dot_kernel(int n, const V* __restrict__ x0
 
 // ------------------------------------------------------------ Krylov steps
 // Scalar slots in device memory.
-enum { S_RHO0 = 0, S_RHO1 = 1, S_SIGMA = 2, S_RR = 3, S_PR = 4, S_GAMMA = 5, S_DELTA = 6, S_TMP0 = 8,
-       S_TMP1 = 9, S_OMEGA = 10, S_ALPHA = 11, S_K0 = 12, S_K1 = 13, S_ABORT = 15, S_NSCAL = 16 };
-// (two-dot reductions write slot and slot + 1: S_TMP0/S_TMP1, S_K1/14;
-// S_ABORT stays clear of them)
+enum { S_RHO0 = 0, S_RHO1 = 1, S_SIGMA = 2, S_RR = 3, S_PR = 4, S_GAMMA = 5, S_DELTA = 6, S_DPR = 7,
+       S_TMP0 = 8, S_TMP1 = 9, S_OMEGA = 10, S_ALPHA = 11, S_K0 = 12, S_K1 = 13, S_ABORT = 15, S_NSCAL = 16 };
+// (two-dot reductions write slot and slot + 1: S_DELTA/S_DPR (FCG's p.Ap and
+// p.r), S_TMP0/S_TMP1, S_K1/14; S_ABORT stays clear of them)
 
 // Step length num/den from device slots; 0 when den == 0 (breakdown, detected
 // on the host), which leaves x and r unchanged.
@@ -724,34 +754,23 @@ __global__ __launch_bounds__(BS) void x_update(int n, double* __restrict__ x, co
   if (i < n) x[i] = step_len(scal, num_slot, den_slot) * p[i] + x[i];
 }
 
-// FCG(1): p = z - (gamma/delta) p, and sum_i p_i r_i for the step length
-// (z as in cg_pupdate; from the second iteration also the previous
-// iteration's x += alpha p).
+// FCG(1): p = z - (gamma/delta) p (z as in cg_pupdate; from the second
+// iteration also the previous iteration's x += alpha p).  The step's
+// numerator p.r is reduced by the following product (EpiSetDotUW).
 template <typename ZT>
 __global__ __launch_bounds__(BS) void fcg_pupdate(int n, const ZT* __restrict__ z, double* __restrict__ p,
-                                                   const double* __restrict__ r, const double* __restrict__ scal,
-                                                   int it, Red red, double* __restrict__ x, int xnum, int xden) {
-  const double c = (it == 1) ? 0.0 : -(scal[S_GAMMA] / scal[S_DELTA]);
-  const double alpha = (it == 1) ? 0.0 : step_len(scal, xnum, xden);
-  double s[2] = {0.0, 0.0};
-  const int b = blockIdx.x * BS * RED_EPT + threadIdx.x;
-#pragma unroll
-  for (int e = 0; e < RED_EPT; ++e) {
-    const int i = b + e * BS;
-    if (i < n) {
-      double pi;
-      if (it == 1) {
-        pi = (double)z[i];
-      } else {
-        const double po = p[i];
-        x[i] = alpha * po + x[i];
-        pi = (double)z[i] + c * po;
-      }
-      p[i] = pi;
-      s[0] += pi * r[i];
-    }
+                                                   const double* __restrict__ scal, int it,
+                                                   double* __restrict__ x, int xnum, int xden) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  if (it == 1) {
+    p[i] = (double)z[i];
+  } else {
+    const double c = -(scal[S_GAMMA] / scal[S_DELTA]);
+    const double po = p[i];
+    x[i] = step_len(scal, xnum, xden) * po + x[i];
+    p[i] = (double)z[i] + c * po;
   }
-  block_partials<1>(s, red.partial);
 }
 
 // ---------------------------------------------------- triangular solves
