@@ -426,9 +426,10 @@ struct has_pair<E, std::void_t<decltype(std::declval<const E&>().aligned2())>> :
 // thread when the epilogue has the two-row form and the arrays allow it.
 template <int K, typename C, typename V, class G, class E>
 static void dia_launch(const DMat<V>& m, G g, E e, hipStream_t s, Red red) {
-  // (FP64 arithmetic only: with FP32 values and FP32 vectors the two-row
-  // form was slower, fine-level residual 94 -> 127 us at 256^3)
-  if constexpr (has_pair<E>::value && std::is_same<C, double>::value) {
+  // (FP64 values only: with FP32 values the two-row form was slower, the
+  // fine-level FP32 residual 94 -> 127 us at 256^3, the FP32-stored Krylov
+  // product summed in FP64 144 -> 150 us)
+  if constexpr (has_pair<E>::value && std::is_same<V, double>::value) {
     if ((m.nr & 1) == 0 && al2(m.val, sizeof(V)) && e.aligned2()) {
       const int nbp = (int)(((int64_t)m.nr / 2 + BS - 1) / BS);
       const int xc = xcd_chunk(m.xmap, nbp, 2 * BS);
