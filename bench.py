@@ -569,14 +569,17 @@ def section_adjoint(eng, torch, problems, perfmodel, idim):
             "note": "setup (host assembly + hierarchy build + upload) is inside forward_s"}
 
 
-def traffic_probe(idim):
+def traffic_probe(idim, tuned=False):
     """Launch sequence for counter runs: 3 copies of 2^27 doubles (1 GiB read
     + 1 GiB written each), then 3 launches of each fine-level step of the FP64
-    benchmark hierarchy (storage autotuning off: the fine level is DIA
-    regardless, and the tuner's trial launches would swamp the counters)."""
+    benchmark hierarchy and of level 1's residual and transfers.  Storage
+    autotuning is off unless `tuned` (the fine level is DIA regardless; with
+    tuning the level-1 layouts are the benchmark's, and the probe's launches
+    are the last ones of each kernel in the trace)."""
     import torch
 
-    os.environ["MLD_AUTOTUNE"] = "0"
+    if not tuned:
+        os.environ["MLD_AUTOTUNE"] = "0"
     eng = _engine()
     from mld2p4_amd import problems
 
@@ -591,7 +594,7 @@ def traffic_probe(idim):
     p.build((irp, ja, val))
     for kind in ("residual", "pre", "post", "krylov_spmv"):
         p.time_step(0, kind, 2)  # one warm-up launch sequence + 2 timed
-    for kind in ("restrict", "prolong"):
+    for kind in ("restrict", "prolong", "residual"):
         p.time_step(1, kind, 2)
     torch.cuda.synchronize()
     log("traffic probe done")
@@ -611,6 +614,7 @@ def main():
     ap.add_argument("--sections", default="c1,run_b,diffusion,batched,adjoint",
                     help="extra sections to run (comma list; empty for none)")
     ap.add_argument("--small", action="store_true", help="small sizes for the extra sections (quick check)")
+    ap.add_argument("--tuned", action="store_true", help="traffic probe with storage autotuning on")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run the kernels whose HBM traffic is counted under rocprofv3 --pmc: a 1 GiB copy "
                          "(the calibration kernel), then the fine-level steps of the benchmark's hierarchy")
@@ -620,7 +624,7 @@ def main():
               "How the path would shard is a design only.", file=sys.stderr)
         sys.exit(2)
     if args.traffic_probe:
-        traffic_probe(args.idim)
+        traffic_probe(args.idim, tuned=args.tuned)
         return
 
     import torch

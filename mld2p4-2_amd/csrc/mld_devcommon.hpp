@@ -618,6 +618,9 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
       if (t < tb) { tb = t; xb = xm; }
     }
     m.xmap = xb;
+    if (std::getenv("MLD_VERBOSE"))
+      std::fprintf(stderr, "[mld] %d x %d operator (%s): row-block order %d, %.1f us\n", h.nr, h.nc,
+                   m.fmt == FMT_DIA ? "DIA" : "tuned layout", xb, tb * 1e3);
   }
   HIPCHECK(hipStreamSynchronize(s));  // the timing launches are done before tmp is freed
   return m;
