@@ -46,7 +46,8 @@ struct DMat {
   int iters = 0;        // CSR: entries per lane loaded up front (0, 2, 4, 8)
   int rpl = 1;          // block products (32 columns): columns per lane (1, 2, 4)
   DiaOff off{};         // DIA: diagonal offsets (w diagonals)
-  int xmap = 1;         // row-block order: 0 identity, 1 chunks of 8192 rows, 2 one range per XCD
+  int xmap = 1;         // row-block order: 0 identity, 1 / 3 / 4 chunks of 8192 / 32768 / 131072 rows,
+                        // 2 one range per XCD
   int* irp = nullptr;   // CSR row pointers
   int* sptr = nullptr;  // SELL slice offsets (nslices + 1)
   int* perm = nullptr;  // SELL position -> row
@@ -385,6 +386,8 @@ static int spmv_blocks(const DMat<V>& m) {
 static inline int xcd_chunk(int xmap, int nb, int rows_per_block) {
   if (xmap == 1) return std::max(1, 8192 / rows_per_block);
   if (xmap == 2) return std::max(1, nb / 8);
+  if (xmap == 3) return std::max(1, 32768 / rows_per_block);
+  if (xmap == 4) return std::max(1, 131072 / rows_per_block);
   return 0;
 }
 
@@ -604,15 +607,15 @@ static DMat<V> upload_tuned(const HCsr& h, DevArena& ar, hipStream_t s, int forc
   FmtChoice fb = dia ? FmtChoice{FMT_DIA, 1, (int)offs.size(), offs} : choose_format(h, bforce);
   if (blanes > 0) { fb.fmt = FMT_CSR; fb.lanes = blanes; }
   DMat<V> m = build_mat<V>(h, src, ar, s, fb, bforce, &layouts);
-  // row-block order of the chosen layout: 8192-row chunks per XCD (default),
-  // identity, or one contiguous range per XCD; the default wins ties within 2 %
-  // (MLD_XMAP=0/1/2 fixes it instead)
+  // row-block order of the chosen layout: chunks of 8192 rows per XCD
+  // (default), identity, one contiguous range per XCD, chunks of 32768 or
+  // 131072 rows; the default wins ties within 2 % (MLD_XMAP=0..4 fixes it)
   if (const char* e = std::getenv("MLD_XMAP")) {
-    m.xmap = std::max(0, std::min(2, std::atoi(e)));
+    m.xmap = std::max(0, std::min(4, std::atoi(e)));
   } else {
     float tb = 1e30f;
     int xb = 1;
-    for (int xm : {1, 0, 2}) {
+    for (int xm : {1, 0, 2, 3, 4}) {
       m.xmap = xm;
       const float t = time_spmv(m, x, y, s, e0, e1) * (xm == 1 ? 1.0f : 1.02f);
       if (t < tb) { tb = t; xb = xm; }

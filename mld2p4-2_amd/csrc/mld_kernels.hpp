@@ -340,8 +340,9 @@ struct Red {  // reduction buffers of a kernel with dot outputs
 // chunk = C > 0, row blocks are handed out in chunks of C consecutive blocks,
 // XCD x taking chunks x, x+8, x+16, ...; blocks past the last full period of
 // 8C keep their index.  chunk = 0 is the identity.  The launcher picks C per
-// operator (autotuned among the identity, 8192-row chunks and one contiguous
-// range per XCD): with 8192-row chunks an XCD's rows recur every 65536 rows,
+// operator (autotuned among the identity, chunks of 8192, 32768 and 131072
+// rows and one contiguous range per XCD): with 8192-row chunks an XCD's rows
+// recur every 65536 rows,
 // so for a 256^2-plane grid the planes a block gathers sit in that XCD's L2
 // (fine-level traffic 1.20x -> 1.01x of the algorithmic bytes); operators
 // whose rows gather from far apart (restriction) prefer a contiguous range.
