@@ -570,7 +570,14 @@ class DevPrecT : public DevPrec {
       {
         // lanes per row of the triangular solves, from the mean row length
         const double avg = (double)std::max(H.l.nnz(), H.u.nnz()) / std::max(1, H.l.nr);
-        S.tri_lanes = avg <= 1.5 ? 1 : (avg <= 6.0 ? 4 : (avg <= 24.0 ? 16 : 32));
+        // one lane per row up to 3.5 entries per row: the 7-point fine level's
+        // factors (3 per row) then take 256 rows per chunk instead of 64, a
+        // quarter of the ticket draws (run-B V-cycle 47.4 -> 45.8 ms at 256^3)
+        static const double one_lane_max = [] {  // MLD_TRI1_MAX overrides it, for measurements
+          const char* e = std::getenv("MLD_TRI1_MAX");
+          return e ? std::atof(e) : 3.5;
+        }();
+        S.tri_lanes = avg <= one_lane_max ? 1 : (avg <= 6.0 ? 4 : (avg <= 24.0 ? 16 : 32));
       }
       S.lrows = ar_.upload(H.lsched_rows.data(), H.lsched_rows.size(), s_);
       S.urows = ar_.upload(H.usched_rows.data(), H.usched_rows.size(), s_);
