@@ -600,6 +600,32 @@ def traffic_probe(idim, tuned=False):
     log("traffic probe done")
 
 
+def block_probe(idim=192, nrhs=32):
+    """Launch sequence for counter runs of the block V-cycle: the same three
+    1 GiB copies (calibration), then 3 block applications (FP32 preconditioner,
+    L1-JACOBI 2/2, storage autotuning on) of nrhs right-hand sides at idim^3;
+    the last application's kernels are the measured ones."""
+    import torch
+
+    eng = _engine()
+    from mld2p4_amd import problems
+
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    irp, ja, val = problems.pde3d_poisson(idim)
+    p = make_prec(eng, "L1-JACOBI", 2, "mixed")
+    p.build((irp, ja, val))
+    X = torch.ones((nrhs, len(irp) - 1), dtype=torch.float64, device="cuda:0")
+    for _ in range(3):
+        p.apply_block(X)
+    torch.cuda.synchronize()
+    log("block probe done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -618,6 +644,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run the kernels whose HBM traffic is counted under rocprofv3 --pmc: a 1 GiB copy "
                          "(the calibration kernel), then the fine-level steps of the benchmark's hierarchy")
+    ap.add_argument("--block-probe", action="store_true",
+                    help="only run the kernels of block V-cycles (192^3, 32 right-hand sides) for rocprofv3 --pmc, "
+                         "after the same 1 GiB calibration copies")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box here has one GPU). "
@@ -625,6 +654,9 @@ def main():
         sys.exit(2)
     if args.traffic_probe:
         traffic_probe(args.idim, tuned=args.tuned)
+        return
+    if args.block_probe:
+        block_probe()
         return
 
     import torch

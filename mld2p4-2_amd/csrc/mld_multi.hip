@@ -53,53 +53,66 @@ __device__ __forceinline__ bv_t<V, R> bvld(const V* p, int64_t i) {
   return *reinterpret_cast<const bv_t<V, R>*>(p + i);
 }
 
-template <typename V>
+// Stores of the epilogues: NT = non-temporal (outputs of the large DIA
+// levels, not re-read by the same kernel; see stnt)
+template <bool NT, typename V>
+__device__ __forceinline__ void bst(V* p, int64_t i, V v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p + i);
+  else p[i] = v;
+}
+template <bool NT, typename V, int R>
+__device__ __forceinline__ void bvst(V* p, int64_t i, bv_t<V, R> v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<bv_t<V, R>*>(p + i));
+  else bvat<V, R>(p, i) = v;
+}
+
+template <typename V, bool NT = false>
 struct BSet {
   V* __restrict__ y;
-  __device__ __forceinline__ void operator()(int, int64_t i, V acc) const { y[i] = acc; }
+  __device__ __forceinline__ void operator()(int, int64_t i, V acc) const { bst<NT>(y, i, acc); }
   template <int R>
-  __device__ __forceinline__ void vec(int, int64_t i, bv_t<V, R> a) const { bvat<V, R>(y, i) = a; }
+  __device__ __forceinline__ void vec(int, int64_t i, bv_t<V, R> a) const { bvst<NT, V, R>(y, i, a); }
 };
-template <typename V>
+template <typename V, bool NT = false>
 struct BAdd {
   V* __restrict__ y;
-  __device__ __forceinline__ void operator()(int, int64_t i, V acc) const { y[i] = y[i] + acc; }
+  __device__ __forceinline__ void operator()(int, int64_t i, V acc) const { bst<NT>(y, i, y[i] + acc); }
   template <int R>
   __device__ __forceinline__ void vec(int, int64_t i, bv_t<V, R> a) const {
-    bvat<V, R>(y, i) = bvld<V, R>(y, i) + a;
+    bvst<NT, V, R>(y, i, bvld<V, R>(y, i) + a);
   }
 };
-template <typename V>
+template <typename V, bool NT = false>
 struct BRes {
   const V* __restrict__ b;
   V* __restrict__ r;
-  __device__ __forceinline__ void operator()(int, int64_t i, V acc) const { r[i] = b[i] - acc; }
+  __device__ __forceinline__ void operator()(int, int64_t i, V acc) const { bst<NT>(r, i, b[i] - acc); }
   template <int R>
   __device__ __forceinline__ void vec(int, int64_t i, bv_t<V, R> a) const {
-    bvat<V, R>(r, i) = bvld<V, R>(b, i) - a;
+    bvst<NT, V, R>(r, i, bvld<V, R>(b, i) - a);
   }
 };
-template <typename V, bool ADD>
+template <typename V, bool ADD, bool NT = false>
 struct BJac {  // t = d_row*(b - ND y);  y = t  or  y = y + t
   const V* __restrict__ d;
   const V* __restrict__ b;
   V* __restrict__ y;
   __device__ __forceinline__ void operator()(int row, int64_t i, V acc) const {
     const V t = d[row] * (b[i] - acc);
-    if (ADD) y[i] = y[i] + t;
-    else y[i] = t;
+    if (ADD) bst<NT>(y, i, y[i] + t);
+    else bst<NT>(y, i, t);
   }
   template <int R>
   __device__ __forceinline__ void vec(int row, int64_t i, bv_t<V, R> a) const {
     const bv_t<V, R> t = d[row] * (bvld<V, R>(b, i) - a);
-    if (ADD) bvat<V, R>(y, i) = bvld<V, R>(y, i) + t;
-    else bvat<V, R>(y, i) = t;
+    if (ADD) bvst<NT, V, R>(y, i, bvld<V, R>(y, i) + t);
+    else bvst<NT, V, R>(y, i, t);
   }
 };
 
 // Last post-smoothing sweep of the fine level writing the cycle's FP64
 // output directly: yn = yin + d_row*(b - ND y); yout = (double)yn.
-template <typename V>
+template <typename V, bool NT = false>
 struct BJacOut {
   const V* __restrict__ d;
   const V* __restrict__ b;
@@ -107,14 +120,50 @@ struct BJacOut {
   double* __restrict__ yout;
   __device__ __forceinline__ void operator()(int row, int64_t i, V acc) const {
     const V t = d[row] * (b[i] - acc);
-    yout[i] = (double)(yin[i] + t);
+    bst<NT>(yout, i, (double)(yin[i] + t));
   }
   template <int R>
   __device__ __forceinline__ void vec(int row, int64_t i, bv_t<V, R> a) const {
     const bv_t<V, R> t = d[row] * (bvld<V, R>(b, i) - a);
-    bvat<double, R>(yout, i) = __builtin_convertvector(bvld<V, R>(yin, i) + t, bv_t<double, R>);
+    bvst<NT, double, R>(yout, i, __builtin_convertvector(bvld<V, R>(yin, i) + t, bv_t<double, R>));
   }
 };
+
+// The NT form of a block epilogue (same data, non-temporal stores)
+template <class E>
+struct BNtForm { using type = void; };
+template <typename V>
+struct BNtForm<BSet<V, false>> {
+  using type = BSet<V, true>;
+  static type make(const BSet<V, false>& e) { return type{e.y}; }
+};
+template <typename V>
+struct BNtForm<BAdd<V, false>> {
+  using type = BAdd<V, true>;
+  static type make(const BAdd<V, false>& e) { return type{e.y}; }
+};
+template <typename V>
+struct BNtForm<BRes<V, false>> {
+  using type = BRes<V, true>;
+  static type make(const BRes<V, false>& e) { return type{e.b, e.r}; }
+};
+template <typename V, bool ADD>
+struct BNtForm<BJac<V, ADD, false>> {
+  using type = BJac<V, ADD, true>;
+  static type make(const BJac<V, ADD, false>& e) { return type{e.d, e.b, e.y}; }
+};
+template <typename V>
+struct BNtForm<BJacOut<V, false>> {
+  using type = BJacOut<V, true>;
+  static type make(const BJacOut<V, false>& e) { return type{e.d, e.b, e.yin, e.yout}; }
+};
+// block DIA levels from this many rows store their outputs non-temporally
+// (MLD_BNT_MIN_ROWS overrides it, for measurements)
+constexpr int BNT_MIN_ROWS = 1 << 20;
+static int bnt_min_rows() {
+  const char* e = std::getenv("MLD_BNT_MIN_ROWS");
+  return e ? std::atoi(e) : BNT_MIN_ROWS;
+}
 
 // Y = A X for row-interleaved X (nc x 32) and Y (nr x 32), CSR A.  A group
 // of 32/RPL lanes owns one row; each lane owns RPL adjacent columns.  The
@@ -1222,6 +1271,12 @@ class MultiT : public MultiEngine {
     // the first Jacobi sweep from zero uses ND D (no d .* x gathers here)
     if (dx) throw DevError("bspmm: d .* x gathers are not used by block products");
     if (m.fmt == FMT_DIA) {
+      if constexpr (!std::is_void<typename BNtForm<E>::type>::value) {
+        if (m.nr >= bnt_min_rows()) {
+          bdia_launch<T, false>(m, x, BNtForm<E>::make(e), dx, rpl, nb, xc);
+          return;
+        }
+      }
       bdia_launch<T, false>(m, x, e, dx, rpl, nb, xc);
       return;
     }
@@ -1256,15 +1311,22 @@ class MultiT : public MultiEngine {
       return;
     }
     if (m.nr < 4096 || !autotune_enabled()) return;
-    T* x = nullptr;
-    T* y = nullptr;
-    HIPCHECK(hipMalloc(&x, sizeof(T) * (size_t)std::max(1, m.nc) * MB));
-    HIPCHECK(hipMalloc(&y, sizeof(T) * (size_t)m.nr * MB));
+    // the temporaries are released also when a timing launch throws
+    DevArena tmp;
+    T* x = tmp.alloc<T>((size_t)std::max(1, m.nc) * MB);
+    T* y = tmp.alloc<T>((size_t)m.nr * MB);
     fill64_kernel<T><<<nblk64((int64_t)std::max(1, m.nc) * MB), BS, 0, s_>>>((int64_t)std::max(1, m.nc) * MB, x,
                                                                               (T)1);
-    hipEvent_t e0, e1;
-    HIPCHECK(hipEventCreate(&e0));
-    HIPCHECK(hipEventCreate(&e1));
+    struct Events {
+      hipEvent_t e[2] = {nullptr, nullptr};
+      ~Events() {
+        for (hipEvent_t v : e)
+          if (v) (void)hipEventDestroy(v);
+      }
+    } ev;
+    HIPCHECK(hipEventCreate(&ev.e[0]));
+    HIPCHECK(hipEventCreate(&ev.e[1]));
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     int best_r = 1, best_x = 0;
     float tbest = 1e30f;
     for (int xm : {1, 0, 2}) {
@@ -1281,6 +1343,9 @@ class MultiT : public MultiEngine {
           HIPCHECK(hipEventElapsedTime(&t, e0, e1));
           tmin = std::min(tmin, t);
         }
+        if (std::getenv("MLD_VERBOSE"))
+          std::fprintf(stderr, "[mld] block product %d x %d (%s): %d columns per lane, row-block order %d, %.1f us\n",
+                       m.nr, m.nc, m.fmt == FMT_DIA ? "DIA" : "CSR", rpl, xm, tmin / 3 * 1e3);
         if (tmin < 0.98f * tbest) {
           tbest = tmin;
           best_r = rpl;
@@ -1288,10 +1353,7 @@ class MultiT : public MultiEngine {
         }
       }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(x);
-    (void)hipFree(y);
+    HIPCHECK(hipStreamSynchronize(s_));  // the timing launches are done before tmp is freed
     m.rpl = best_r;
     m.xmap = best_x;
   }
