@@ -570,14 +570,22 @@ class DevPrecT : public DevPrec {
       {
         // lanes per row of the triangular solves, from the mean row length
         const double avg = (double)std::max(H.l.nnz(), H.u.nnz()) / std::max(1, H.l.nr);
-        // one lane per row up to 3.5 entries per row: the 7-point fine level's
-        // factors (3 per row) then take 256 rows per chunk instead of 64, a
-        // quarter of the ticket draws (run-B V-cycle 47.4 -> 45.8 ms at 256^3)
-        static const double one_lane_max = [] {  // MLD_TRI1_MAX overrides it, for measurements
+        // Lanes per row: one up to 3.5 entries per row (the 7-point fine
+        // level's factors, 3 per row: 256 rows per chunk instead of 64, a
+        // quarter of the ticket draws; run-B V-cycle 47.4 -> 45.8 ms at
+        // 256^3), four up to 6, else 32 (the Galerkin level's factors, about
+        // 15 per row: 8 rows per chunk; 16 lanes 45.8, 32 lanes 41.4, 64 lanes
+        // 48.4 ms).  MLD_TRI1_MAX and MLD_TRI_LANES override them, for
+        // measurements.
+        static const double one_lane_max = [] {
           const char* e = std::getenv("MLD_TRI1_MAX");
           return e ? std::atof(e) : 3.5;
         }();
-        S.tri_lanes = avg <= one_lane_max ? 1 : (avg <= 6.0 ? 4 : (avg <= 24.0 ? 16 : 32));
+        S.tri_lanes = avg <= one_lane_max ? 1 : (avg <= 6.0 ? 4 : 32);
+        if (const char* e = std::getenv("MLD_TRI_LANES")) {
+          const int l = std::atoi(e);
+          if (avg > one_lane_max && (l == 4 || l == 16 || l == 32)) S.tri_lanes = l;
+        }
       }
       S.lrows = ar_.upload(H.lsched_rows.data(), H.lsched_rows.size(), s_);
       S.urows = ar_.upload(H.usched_rows.data(), H.usched_rows.size(), s_);
@@ -829,7 +837,8 @@ class DevPrecT : public DevPrec {
   }
   // Workgroups of a sync-free substitution: enough to hold the chunks of
   // SF_AHEAD level sets at once, at least SF_MIN_BLOCKS, at most SF_BLOCKS
-  // (MLD_SF_AHEAD, MLD_SF_BLOCKS override; 0 ahead: no such limit).  Every
+  // (MLD_SF_AHEAD, MLD_SF_MIN_BLOCKS, MLD_SF_BLOCKS override; 0 ahead: no
+  // such limit).  Every
   // workgroup beyond that only spins on values not yet written, and the
   // spinning loads crowd the L2 that the producers' stores pass through: at
   // 256^3 (run B) the V-cycle takes 89 ms with 2048 workgroups per solve,
@@ -838,8 +847,10 @@ class DevPrecT : public DevPrec {
     int ahead = SF_AHEAD, cap = SF_BLOCKS;
     if (const char* e = std::getenv("MLD_SF_AHEAD")) ahead = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("MLD_SF_BLOCKS")) cap = std::max(1, std::atoi(e));
+    int lo = SF_MIN_BLOCKS;
+    if (const char* e = std::getenv("MLD_SF_MIN_BLOCKS")) lo = std::max(1, std::atoi(e));
     int nb = std::min(nch, cap);
-    if (ahead > 0 && nlv > 0) nb = std::min(nb, std::max(SF_MIN_BLOCKS, ahead * ((nch + nlv - 1) / nlv)));
+    if (ahead > 0 && nlv > 0) nb = std::min(nb, std::max(lo, ahead * ((nch + nlv - 1) / nlv)));
     return std::max(1, nb);
   }
 
