@@ -80,6 +80,11 @@ struct DSmoother {
   int *lch = nullptr, *uch = nullptr, *ltch = nullptr, *utch = nullptr;
   int nlch = 0, nuch = 0, nltch = 0, nutch = 0;
   int* sf_ticket = nullptr;
+  // ILU(0) factors of a 7-point stencil on a gnx x gny x gnz grid (natural
+  // order): their values by stencil position for trsv_grid_kernel (null when
+  // the factors have another pattern)
+  V *lg = nullptr, *ug = nullptr;
+  int gnx = 0, gny = 0, gnz = 0;
   // dense LU
   double* lu = nullptr; int* piv = nullptr;
   double* dinv = nullptr;  // DENSE_LU: explicit inverse (row-major)

@@ -603,6 +603,17 @@ class DevPrecT : public DevPrec {
         S.sf_ticket = ar_.alloc<int>(1);
         HIPCHECK(hipMemsetAsync(S.sf_ticket, 0, sizeof(int), s_));
       }
+      if (H.l.nr > SMALL_TRI) {  // ILU(0), GS, BWGS
+        std::vector<V> lg, ug;
+        if (grid_factors(H.l, H.u, S.gnx, S.gny, S.gnz, lg, ug)) {
+          S.lg = ar_.upload(lg.data(), lg.size(), s_);
+          S.ug = ar_.upload(ug.data(), ug.size(), s_);
+          if (std::getenv("MLD_VERBOSE"))
+            std::fprintf(stderr, "[mld] %d-row %s factors of a 7-point stencil on a %d x %d x %d grid: "
+                                 "grid substitutions\n", H.l.nr, H.solver == ILU_N ? "ILU(0)" : "Gauss-Seidel",
+                         S.gnx, S.gny, S.gnz);
+        }
+      }
       HIPCHECK(hipStreamSynchronize(s_));
     }
     if (H.solver == DENSE_LU) {
@@ -770,6 +781,12 @@ class DevPrecT : public DevPrec {
       HIPCHECK(hipGetLastError());
       return;
     }
+    if (!TR && S.lg && sync_free() && grid_trsv()) {
+      grid_launch<0, 2, false>(S, n, S.lg, x, nullptr, t1, nullptr);
+      if (beta) grid_launch<1, 1, true>(S, n, S.ug, t1, S.dv, t2, y);
+      else grid_launch<1, 2, true>(S, n, S.ug, t1, S.dv, t2, y);
+      return;
+    }
     if (sync_free()) {
       const int* c1 = TR ? S.utch : S.lch;
       const int* c2 = TR ? S.ltch : S.uch;
@@ -823,6 +840,21 @@ class DevPrecT : public DevPrec {
     HIPCHECK(hipGetLastError());
   }
 
+  // The grid form of one substitution (trsv_grid_kernel): t filled with the
+  // sentinel, then one wavefront per tile of 8 x 8 grid lines.
+  template <int SC, int BT, bool BACK>
+  void grid_launch(DSmoother<V>& S, int n, const V* g, const V* b, const V* d, V* t, V* y) {
+    sf_fill_kernel<V><<<nblk(n), BS, 0, s_>>>(n, t);
+    const int ntiles = ((S.gny + 7) / 8) * ((S.gnz + 7) / 8);
+    int xs = 1, nwg = GRID_TRSV_BLOCKS;  // (MLD_GRID_XS, MLD_GRID_WG override them, for measurements)
+    if (const char* e = std::getenv("MLD_GRID_XS")) xs = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("MLD_GRID_WG")) nwg = std::max(1, std::atoi(e));
+    nwg = std::min(ntiles, nwg);
+    trsv_grid_kernel<V, SC, BT, BACK><<<nwg * xs, 64, 0, s_>>>(S.gnx, S.gny, S.gnz, g, b, d, t, y, S.sf_ticket,
+                                                              scal_ + S_ABORT, SF_SPIN_MAX, xs);
+    HIPCHECK(hipGetLastError());
+  }
+  static constexpr int GRID_TRSV_BLOCKS = 2048;  // one wavefront each
   // One sync-free substitution over a whole schedule (nch chunks) of an
   // n-row factor: t filled with the sentinel, then the solve.
   template <int SC, int BT, bool TR, int LN>
@@ -855,6 +887,56 @@ class DevPrecT : public DevPrec {
   }
 
   // The level-scheduled launches instead (MLD_TRSV=level).
+  // The ILU(0) factors l (strict lower) and u (strict upper) of a 7-point
+  // stencil on an nx x ny x nz grid in natural order: every row r = i +
+  // nx (j + ny k) of l holds exactly the columns r - nx ny (k > 0), r - nx
+  // (j > 0), r - 1 (i > 0), in this order, and of u r + 1, r + nx, r + nx ny
+  // (i < nx - 1, j < ny - 1, k < nz - 1); nx, ny, nz >= 2.  Then the values
+  // by stencil position (0 where a row has no such entry) for
+  // trsv_grid_kernel, and true.
+  static bool grid_factors(const HCsr& l, const HCsr& u, int& nx, int& ny, int& nz, std::vector<V>& lg,
+                           std::vector<V>& ug) {
+    const int64_t n = l.nr;
+    if (n < 8 || u.nr != n || l.irp[n] - l.irp[n - 1] != 3) return false;
+    const int64_t c0 = l.ja[l.irp[n - 1]], c1 = l.ja[l.irp[n - 1] + 1];
+    const int64_t sx = (n - 1) - c1, sxy = (n - 1) - c0;  // nx and nx ny from the last row
+    if (sx < 2 || sxy <= sx || sxy % sx != 0 || n % sxy != 0) return false;
+    if (sxy / sx < 2 || n / sxy < 2 || sxy > INT32_MAX || n / sxy > INT32_MAX) return false;
+    nx = (int)sx;
+    ny = (int)(sxy / sx);
+    nz = (int)(n / sxy);
+    lg.assign(3 * (size_t)n, (V)0);
+    ug.assign(3 * (size_t)n, (V)0);
+    for (int64_t r = 0; r < n; ++r) {
+      const int64_t i = r % sx, j = (r / sx) % ny, k = r / sxy;
+      int64_t q = l.irp[r];
+      const int64_t lo[3] = {r - sxy, r - sx, r - 1};
+      const bool lh[3] = {k > 0, j > 0, i > 0};
+      for (int e = 0; e < 3; ++e)
+        if (lh[e]) {
+          if (q >= l.irp[r + 1] || l.ja[q] != lo[e]) return false;
+          lg[(size_t)e * n + r] = (V)l.val[q++];
+        }
+      if (q != l.irp[r + 1]) return false;
+      q = u.irp[r];
+      const int64_t uo[3] = {r + 1, r + sx, r + sxy};
+      const bool uh[3] = {i < sx - 1, j < ny - 1, k < nz - 1};
+      for (int e = 0; e < 3; ++e)
+        if (uh[e]) {
+          if (q >= u.irp[r + 1] || u.ja[q] != uo[e]) return false;
+          ug[(size_t)e * n + r] = (V)u.val[q++];
+        }
+      if (q != u.irp[r + 1]) return false;
+    }
+    return true;
+  }
+  // the grid form of the ILU(0) substitutions (MLD_TRSV_GRID=0: the general
+  // sync-free form, for measurements and tests)
+  static bool grid_trsv() {
+    const char* e = std::getenv("MLD_TRSV_GRID");
+    return !(e && std::atoi(e) == 0);
+  }
+
   static bool sync_free() {
     static const bool sf = [] {
       const char* e = std::getenv("MLD_TRSV");
@@ -917,6 +999,19 @@ class DevPrecT : public DevPrec {
     };
     if (D.n <= SMALL_TRI) {  // whole solve in one workgroup
       run(0, nl);
+      return;
+    }
+    if (S.lg && sync_free() && grid_trsv()) {
+      const V* g = lower ? S.lg : S.ug;
+      if (lower) {
+        if (bt == 0) grid_launch<2, 0, false>(S, D.n, g, b, S.dv, t, y);
+        else if (bt == 1) grid_launch<2, 1, false>(S, D.n, g, b, S.dv, t, y);
+        else grid_launch<2, 2, false>(S, D.n, g, b, S.dv, t, y);
+      } else {
+        if (bt == 0) grid_launch<2, 0, true>(S, D.n, g, b, S.dv, t, y);
+        else if (bt == 1) grid_launch<2, 1, true>(S, D.n, g, b, S.dv, t, y);
+        else grid_launch<2, 2, true>(S, D.n, g, b, S.dv, t, y);
+      }
       return;
     }
     if (sync_free()) {

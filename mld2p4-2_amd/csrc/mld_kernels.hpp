@@ -991,6 +991,84 @@ __global__ __launch_bounds__(BS) void trsv_sf_kernel(int nchunks, const int* __r
   }
 }
 
+// Triangular solve with a 7-point ILU(0) factor on an nx x ny x nz grid in
+// natural order (lower: columns r - nx ny, r - nx, r - 1 where they exist;
+// BACK, upper: r + 1, r + nx, r + nx ny), g holding the factor's values by
+// stencil position (g[0..n), g[n..2n), g[2n..3n) in ascending column order).
+// One wavefront per tile of 8 x 8 grid lines along x: lane (jj, kk) owns the
+// line (j, k) = (8 J + jj, 8 K + kk) and the tile steps through its
+// diagonal planes i + jj + kk = s (mirrored for BACK).  A row's neighbours in
+// j and k inside the tile were computed by the neighbouring lanes in the
+// previous step and come by cross-lane reads; its neighbour in i is the
+// lane's own previous value; only the tile's two faces wait for values in
+// memory (sentinel and bounded wait as in trsv_sf_kernel).  Tiles are drawn
+// through a ticket in an order in which both tiles a tile waits for were
+// drawn before it.  Per row the arithmetic of tri_row_sf (products added in
+// ascending column order; t_i = b_i - acc, d_i b_i - acc or, SC = 2, the
+// Gauss-Seidel (b_i - acc) / d_i), so the values are those of the other
+// substitutions bit for bit.
+template <typename V, int SC, int BETA, bool BACK>
+__global__ __launch_bounds__(64) void trsv_grid_kernel(int nx, int ny, int nz, const V* __restrict__ g,
+                                                       const V* __restrict__ b, const V* __restrict__ d, V* t,
+                                                       V* y, int* ticket, double* abort, int spin_max, int xs) {
+  // xs > 1: only every xs-th workgroup draws tiles (the others leave at once)
+  if (blockIdx.x % xs != 0) return;
+  const int nwg = ((int)gridDim.x + xs - 1) / xs;  // workgroups that draw
+  const int lane = threadIdx.x;
+  const int jj = lane & 7, kk = lane >> 3;
+  const int ntj = (ny + 7) >> 3, ntk = (nz + 7) >> 3, ntiles = ntj * ntk;
+  const int64_t n = (int64_t)nx * ny * nz, pl = (int64_t)nx * ny;
+  const int mj = BACK ? 7 - jj : jj, mk = BACK ? 7 - kk : kk;  // position in the solve's order
+  const int ljn = BACK ? (lane + 1) & 63 : (lane + 63) & 63;  // lane of the neighbour in j
+  const int lkn = BACK ? (lane + 8) & 63 : (lane + 56) & 63;  // lane of the neighbour in k
+  for (;;) {
+    int c = 0;
+    if (lane == 0) {
+      c = atomicAdd(ticket, 1);
+      if (c == ntiles + nwg - 1) atomicExch(ticket, 0);  // every draw of this launch is done
+    }
+    c = __shfl(c, 0);
+    if (c >= ntiles) return;
+    const int tix = BACK ? ntiles - 1 - c : c;
+    const int j = (tix % ntj) * 8 + jj, k = (tix / ntj) * 8 + kk;
+    const bool valid = j < ny && k < nz;
+    const bool hasj = BACK ? j + 1 < ny : j > 0;
+    const bool hask = BACK ? k + 1 < nz : k > 0;
+    const bool facej = hasj && mj == 0, facek = hask && mk == 0;
+    const int64_t line = (int64_t)nx * (j + (int64_t)ny * k);
+    const int64_t oj = BACK ? nx : -(int64_t)nx, ok = BACK ? pl : -pl;  // neighbour offsets in j and k
+    V prev = (V)0;
+    for (int st = 0; st < nx + 14; ++st) {
+      const V vj = __shfl(prev, ljn), vk = __shfl(prev, lkn);
+      const int mi = st - mj - mk;
+      if (valid && mi >= 0 && mi < nx) {
+        const int i = BACK ? nx - 1 - mi : mi;
+        const int64_t r = line + i;
+        const bool hasi = BACK ? i + 1 < nx : i > 0;
+        const V g0 = g[r], g1 = g[n + r], g2 = g[2 * n + r];
+        const V bi = (SC == 1) ? d[r] * b[r] : b[r];
+        const V tj = facej ? sf_wait(t + r + oj, abort, spin_max) : vj;
+        const V tk = facek ? sf_wait(t + r + ok, abort, spin_max) : vk;
+        V acc = (V)0;
+        if (!BACK) {
+          if (hask) acc = acc + g0 * tk;
+          if (hasj) acc = acc + g1 * tj;
+          if (hasi) acc = acc + g2 * prev;
+        } else {
+          if (hasi) acc = acc + g0 * prev;
+          if (hasj) acc = acc + g1 * tj;
+          if (hask) acc = acc + g2 * tk;
+        }
+        const V ti = (SC == 2) ? (bi - acc) / d[r] : bi - acc;
+        if (BETA == 0) y[r] = ti;
+        else if (BETA == 1) y[r] = y[r] + ti;
+        __hip_atomic_store(t + r, ti, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        prev = ti;
+      }
+    }
+  }
+}
+
 // Whole ILU(0) apply for a small matrix in one workgroup: level sets
 // processed in order with a barrier in between.
 template <typename V, int BETA, bool TR, int LANES, bool LDS = false>

@@ -146,6 +146,45 @@ def test_ilu_level_scheduled_bitexact(eng, orc, idim, beta):
             assert (y.cpu().numpy() == yo).all(), (lev, trans)
 
 
+@pytest.mark.parametrize("idim", [30, 41, 52])
+def test_ilu_grid_substitution_bitexact(eng, orc, idim, capfd, monkeypatch):
+    """ILU(0) factors of the 7-point stencil on the fine level are recognised
+    as a grid (the upload says so under MLD_VERBOSE) and solved by
+    trsv_grid_kernel (tiles of 8 x 8 grid lines; idim 30, 41, 52 leave
+    partial tiles): 'N' applications with beta 0 and 1 identical to the
+    oracle's sequential substitution, and to the general sync-free form
+    (MLD_TRSV_GRID=0) in FP64 and in FP32."""
+    _say(f"[ILU grid] idim={idim}")
+    monkeypatch.setenv("MLD_VERBOSE", "1")
+    a = orc.gen_pde3d(idim)
+    A = a.to_scipy()
+    p = engine_prec(eng, A, "BJAC", 1)
+    out = capfd.readouterr().err
+    assert f"on a {idim} x {idim} x {idim} grid: grid substitutions" in out
+    po = oracle_prec(orc, a, "BJAC", 1)
+    rng = np.random.default_rng(17)
+    n = idim ** 3
+    x = rng.standard_normal(n)
+    y0 = rng.standard_normal(n)
+    for beta in (0, 1, 0):
+        monkeypatch.delenv("MLD_TRSV_GRID", raising=False)
+        y = dev(y0.copy())
+        p.level_smooth(0, dev(x), y, beta=beta, sweeps=1, trans="N")
+        yo = po.smooth(0, x, alpha=1.0, beta=float(beta), y=y0, sweeps=1, trans="N")
+        assert (y.cpu().numpy() == yo).all(), beta
+        monkeypatch.setenv("MLD_TRSV_GRID", "0")
+        y2 = dev(y0.copy())
+        p.level_smooth(0, dev(x), y2, beta=beta, sweeps=1, trans="N")
+        assert torch.equal(y, y2), beta
+    monkeypatch.delenv("MLD_TRSV_GRID", raising=False)
+    p32 = engine_prec(eng, A, "BJAC", 1, precision="mixed")
+    x32 = dev(x)
+    z1 = p32.apply(x32)
+    monkeypatch.setenv("MLD_TRSV_GRID", "0")
+    z2 = p32.apply(x32)
+    assert torch.equal(z1, z2)
+
+
 @pytest.mark.parametrize("idim", [14, 30])
 def test_gauss_seidel_level_bitexact(eng, orc, idim):
     """GS (forward) and BWGS (backward) smoothers of the default ML
@@ -170,6 +209,38 @@ def test_gauss_seidel_level_bitexact(eng, orc, idim):
                 assert (y.cpu().numpy() == yo).all(), (which, sweeps, beta)
     with pytest.raises(eng.MldError):
         p.level_smooth(0, dev(x), dev(y0.copy()), trans="T")
+
+
+@pytest.mark.parametrize("idim", [41])
+def test_gauss_seidel_grid_substitution_bitexact(eng, orc, idim, capfd, monkeypatch):
+    """GS and BWGS sweeps on the fine level run as grid substitutions
+    (trsv_grid_kernel with the division by the stored diagonal): identical to
+    the oracle's sequential sweeps and to the general sync-free form
+    (MLD_TRSV_GRID=0), 1 and 3 sweeps, beta 0 and 1."""
+    _say(f"[GS grid] idim={idim}")
+    monkeypatch.setenv("MLD_VERBOSE", "1")
+    a = orc.gen_pde3d(idim)
+    A = a.to_scipy()
+    p = engine_prec(eng, A, "FBGS", 1)
+    out = capfd.readouterr().err
+    assert f"Gauss-Seidel factors of a 7-point stencil on a {idim} x {idim} x {idim} grid" in out
+    po = oracle_prec(orc, a, "FBGS", 1)
+    rng = np.random.default_rng(23)
+    n = A.shape[0]
+    x = rng.standard_normal(n)
+    y0 = rng.standard_normal(n)
+    for which in (0, 1):
+        for sweeps in (1, 3):
+            for beta in (0, 1):
+                monkeypatch.delenv("MLD_TRSV_GRID", raising=False)
+                y = dev(y0.copy())
+                p.level_smooth(0, dev(x), y, beta=beta, sweeps=sweeps, which=which)
+                yo = po.smooth(0, x, alpha=1.0, beta=float(beta), y=y0, sweeps=sweeps, which=which)
+                assert (y.cpu().numpy() == yo).all(), (which, sweeps, beta)
+                monkeypatch.setenv("MLD_TRSV_GRID", "0")
+                y2 = dev(y0.copy())
+                p.level_smooth(0, dev(x), y2, beta=beta, sweeps=sweeps, which=which)
+                assert torch.equal(y, y2), (which, sweeps, beta)
 
 
 @pytest.mark.parametrize("nsw", [1, 2])
