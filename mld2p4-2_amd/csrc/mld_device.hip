@@ -606,8 +606,12 @@ class DevPrecT : public DevPrec {
       if (H.l.nr > SMALL_TRI) {  // ILU(0), GS, BWGS
         std::vector<V> lg, ug;
         if (grid_factors(H.l, H.u, S.gnx, S.gny, S.gnz, lg, ug)) {
-          S.lg = ar_.upload(lg.data(), lg.size(), s_);
-          S.ug = ar_.upload(ug.data(), ug.size(), s_);
+          if (H.solver == ILU_N && !H.dv.empty()) {
+            skew_upload(S, lg, ug, H.dv);
+          } else {
+            S.lg = ar_.upload(lg.data(), lg.size(), s_);
+            S.ug = ar_.upload(ug.data(), ug.size(), s_);
+          }
           if (std::getenv("MLD_VERBOSE"))
             std::fprintf(stderr, "[mld] %d-row %s factors of a 7-point stencil on a %d x %d x %d grid: "
                                  "grid substitutions\n", H.l.nr, H.solver == ILU_N ? "ILU(0)" : "Gauss-Seidel",
@@ -781,10 +785,10 @@ class DevPrecT : public DevPrec {
       HIPCHECK(hipGetLastError());
       return;
     }
-    if (!TR && S.lg && sync_free() && grid_trsv()) {
-      grid_launch<0, 2, false>(S, n, S.lg, x, nullptr, t1, nullptr);
-      if (beta) grid_launch<1, 1, true>(S, n, S.ug, t1, S.dv, t2, y);
-      else grid_launch<1, 2, true>(S, n, S.ug, t1, S.dv, t2, y);
+    if (!TR && S.lgs && sync_free() && grid_trsv()) {
+      skew_launch<false, 2>(S, x, S.tgs, nullptr);
+      if (beta) skew_launch<true, 1>(S, S.tgs, t2, y);
+      else skew_launch<true, 2>(S, S.tgs, t2, y);
       return;
     }
     if (sync_free()) {
@@ -855,6 +859,19 @@ class DevPrecT : public DevPrec {
     HIPCHECK(hipGetLastError());
   }
   static constexpr int GRID_TRSV_BLOCKS = 2048;  // one wavefront each
+  // The ILU(0) pair on a grid with the operands in the tiles' step order:
+  // forward (L, t in step order) from the input in natural order, backward (U,
+  // d b from step order) into the natural-order output.
+  template <bool BACK, int BT>
+  void skew_launch(DSmoother<V>& S, const V* in, V* t, V* y) {
+    const int64_t nfill = BACK ? (int64_t)S.gnx * S.gny * S.gnz : S.gsz;
+    sf_fill_kernel<V><<<(int)((nfill + BS - 1) / BS), BS, 0, s_>>>((int)nfill, t);
+    const int ntiles = ((S.gny + 7) / 8) * ((S.gnz + 7) / 8);
+    const int nwg = std::min(ntiles, GRID_TRSV_BLOCKS);
+    trsv_grid_skew_kernel<V, BT, BACK><<<nwg, 64, 0, s_>>>(S.gnx, S.gny, S.gnz, S.gsz, BACK ? S.ugs : S.lgs, S.dgs,
+                                                           in, t, y, S.sf_ticket, scal_ + S_ABORT, SF_SPIN_MAX);
+    HIPCHECK(hipGetLastError());
+  }
   // One sync-free substitution over a whole schedule (nch chunks) of an
   // n-row factor: t filled with the sentinel, then the solve.
   template <int SC, int BT, bool TR, int LN>
@@ -929,6 +946,41 @@ class DevPrecT : public DevPrec {
       if (q != u.irp[r + 1]) return false;
     }
     return true;
+  }
+  // Position of grid point r = i + nx (j + ny k) in the tiles' step order:
+  // tile T = j / 8 + ntj (k / 8), step s = i + j % 8 + k % 8 of the forward
+  // solve, lane j % 8 + 8 (k % 8): ((T ns + s) 64 + lane), ns = nx + 14.  At
+  // every step of either solve the 64 lanes of a tile touch 64 consecutive
+  // positions.
+  static int64_t skew_pos(int64_t r, int nx, int ny, int ntj) {
+    const int64_t i = r % nx, j = (r / nx) % ny, k = r / ((int64_t)nx * ny);
+    const int64_t tile = j / 8 + (int64_t)ntj * (k / 8);
+    const int64_t st = i + j % 8 + k % 8, lane = j % 8 + 8 * (k % 8);
+    return (tile * (nx + 14) + st) * 64 + lane;
+  }
+  // ILU(0) factors on a grid: values by stencil position (lg, ug: 3 n each),
+  // d, and the intermediate vector's buffer in the tiles' step order.
+  void skew_upload(DSmoother<V>& S, const std::vector<V>& lg, const std::vector<V>& ug,
+                   const std::vector<double>& dv) {
+    const int nx = S.gnx, ny = S.gny, nz = S.gnz;
+    const int ntj = (ny + 7) / 8, ntk = (nz + 7) / 8;
+    const int64_t n = (int64_t)nx * ny * nz, sz = (int64_t)ntj * ntk * (nx + 14) * 64;
+    std::vector<V> a(3 * (size_t)sz, (V)0), dd((size_t)sz, (V)0);
+    for (int64_t r = 0; r < n; ++r) {
+      const int64_t q = skew_pos(r, nx, ny, ntj);
+      for (int e = 0; e < 3; ++e) a[(size_t)e * sz + q] = lg[(size_t)e * n + r];
+      dd[q] = (V)dv[r];
+    }
+    S.lgs = ar_.upload(a.data(), a.size(), s_);
+    for (int64_t r = 0; r < n; ++r) {
+      const int64_t q = skew_pos(r, nx, ny, ntj);
+      for (int e = 0; e < 3; ++e) a[(size_t)e * sz + q] = ug[(size_t)e * n + r];
+    }
+    S.ugs = ar_.upload(a.data(), a.size(), s_);
+    S.dgs = ar_.upload(dd.data(), dd.size(), s_);
+    S.tgs = ar_.alloc<V>((size_t)sz);
+    S.gsz = sz;
+    HIPCHECK(hipStreamSynchronize(s_));
   }
   // the grid form of the ILU(0) substitutions (MLD_TRSV_GRID=0: the general
   // sync-free form, for measurements and tests)

@@ -1069,6 +1069,88 @@ __global__ __launch_bounds__(64) void trsv_grid_kernel(int nx, int ny, int nz, c
   }
 }
 
+// The ILU(0) pair of trsv_grid_kernel with the bulk operands in the tiles'
+// step order (position ((T (nx + 14) + s) 64 + lane) for the point that lane
+// (jj, kk) of tile T reaches at step s = i + jj + kk of the forward solve), so
+// that at every step the 64 lanes read 64 consecutive values instead of one
+// value from each of 64 grid lines.  Forward (!BACK, the unit lower factor,
+// t_i = b_i - acc): b in natural order, t in step order.  Backward (BACK, the
+// upper factor, t_i = d_i b_i - acc): b and d in step order, t (and y) in
+// natural order.  Values, waits and tile order as in trsv_grid_kernel.
+template <typename V, int BETA, bool BACK>
+__global__ __launch_bounds__(64) void trsv_grid_skew_kernel(int nx, int ny, int nz, int64_t sz,
+                                                            const V* __restrict__ g, const V* __restrict__ d,
+                                                            const V* __restrict__ b, V* t, V* y, int* ticket,
+                                                            double* abort, int spin_max) {
+  const int lane = threadIdx.x;
+  const int jj = lane & 7, kk = lane >> 3;
+  const int ntj = (ny + 7) >> 3, ntk = (nz + 7) >> 3, ntiles = ntj * ntk;
+  const int64_t pl = (int64_t)nx * ny, ns = nx + 14;
+  const int mj = BACK ? 7 - jj : jj, mk = BACK ? 7 - kk : kk;  // position in the solve's order
+  const int ljn = BACK ? (lane + 1) & 63 : (lane + 63) & 63;  // lane of the neighbour in j
+  const int lkn = BACK ? (lane + 8) & 63 : (lane + 56) & 63;  // lane of the neighbour in k
+  for (;;) {
+    int c = 0;
+    if (lane == 0) {
+      c = atomicAdd(ticket, 1);
+      if (c == ntiles + (int)gridDim.x - 1) atomicExch(ticket, 0);  // every draw of this launch is done
+    }
+    c = __shfl(c, 0);
+    if (c >= ntiles) return;
+    const int tix = BACK ? ntiles - 1 - c : c;
+    const int j = (tix % ntj) * 8 + jj, k = (tix / ntj) * 8 + kk;
+    const bool valid = j < ny && k < nz;
+    const bool hasj = BACK ? j + 1 < ny : j > 0;
+    const bool hask = BACK ? k + 1 < nz : k > 0;
+    const bool facej = hasj && mj == 0, facek = hask && mk == 0;
+    const int64_t line = (int64_t)nx * (j + (int64_t)ny * k);
+    const int64_t base = (int64_t)tix * ns * 64;  // this tile's first position
+    V prev = (V)0;
+    for (int st = 0; st < nx + 14; ++st) {
+      const V vj = __shfl(prev, ljn), vk = __shfl(prev, lkn);
+      const int mi = st - mj - mk;
+      if (valid && mi >= 0 && mi < nx) {
+        const int i = BACK ? nx - 1 - mi : mi;
+        const int64_t r = line + i;
+        const int64_t q = base + (int64_t)(i + jj + kk) * 64 + lane;  // position in step order
+        const bool hasi = BACK ? i + 1 < nx : i > 0;
+        const V g0 = g[q], g1 = g[sz + q], g2 = g[2 * sz + q];
+        const V bi = BACK ? d[q] * b[q] : b[r];
+        V tj = vj, tk = vk;
+        if (BACK) {
+          if (facej) tj = sf_wait(t + r + nx, abort, spin_max);
+          if (facek) tk = sf_wait(t + r + pl, abort, spin_max);
+        } else {
+          // (i, j - 1, k): lane (7, kk) of the tile before in j; (i, j, k - 1):
+          // lane (jj, 7) of the tile before in k; both at step i + 7 + (kk or jj)
+          if (facej) tj = sf_wait(t + base - ns * 64 + (int64_t)(i + 7 + kk) * 64 + 7 + 8 * kk, abort, spin_max);
+          if (facek)
+            tk = sf_wait(t + base - (int64_t)ntj * ns * 64 + (int64_t)(i + jj + 7) * 64 + jj + 56, abort, spin_max);
+        }
+        V acc = (V)0;
+        if (!BACK) {
+          if (hask) acc = acc + g0 * tk;
+          if (hasj) acc = acc + g1 * tj;
+          if (hasi) acc = acc + g2 * prev;
+        } else {
+          if (hasi) acc = acc + g0 * prev;
+          if (hasj) acc = acc + g1 * tj;
+          if (hask) acc = acc + g2 * tk;
+        }
+        const V ti = bi - acc;
+        if (BACK) {
+          if (BETA == 0) y[r] = ti;
+          else if (BETA == 1) y[r] = y[r] + ti;
+          __hip_atomic_store(t + r, ti, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          __hip_atomic_store(t + q, ti, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        prev = ti;
+      }
+    }
+  }
+}
+
 // Whole ILU(0) apply for a small matrix in one workgroup: level sets
 // processed in order with a barrier in between.
 template <typename V, int BETA, bool TR, int LANES, bool LDS = false>
