@@ -85,9 +85,9 @@ struct DSmoother {
   // the factors have another pattern)
   V *lg = nullptr, *ug = nullptr;
   int gnx = 0, gny = 0, gnz = 0;
-  // ILU(0) on such a grid: the factors' values, d and the intermediate vector
-  // in the tiles' step order (trsv_grid_skew_kernel), gsz entries each
-  V *lgs = nullptr, *ugs = nullptr, *dgs = nullptr, *tgs = nullptr;
+  // ILU(0) on such a grid: the factors' values, d and two work vectors in the
+  // tiles' step order (trsv_grid_skew_kernel), gsz entries each
+  V *lgs = nullptr, *ugs = nullptr, *dgs = nullptr, *tgs = nullptr, *xgs = nullptr;
   int64_t gsz = 0;
   // dense LU
   double* lu = nullptr; int* piv = nullptr;

@@ -786,9 +786,17 @@ class DevPrecT : public DevPrec {
       return;
     }
     if (!TR && S.lgs && sync_free() && grid_trsv()) {
-      skew_launch<false, 2>(S, x, S.tgs, nullptr);
-      if (beta) skew_launch<true, 1>(S, S.tgs, t2, y);
-      else skew_launch<true, 2>(S, S.tgs, t2, y);
+      // x into step order (xgs), L solve into tgs, U solve into xgs, result
+      // back into natural order
+      const int64_t ng = (int64_t)S.gnx * S.gny * S.gnz;
+      grid_to_steps_kernel<V><<<(int)((S.gsz + BS - 1) / BS), BS, 0, s_>>>(S.gnx, S.gny, S.gnz, S.gsz, x, S.xgs);
+      skew_launch<false>(S, S.xgs, S.tgs);
+      skew_launch<true>(S, S.tgs, S.xgs);
+      if (beta)
+        grid_from_steps_kernel<V, 1><<<(int)((ng + BS - 1) / BS), BS, 0, s_>>>(S.gnx, S.gny, S.gnz, S.xgs, y);
+      else
+        grid_from_steps_kernel<V, 0><<<(int)((ng + BS - 1) / BS), BS, 0, s_>>>(S.gnx, S.gny, S.gnz, S.xgs, y);
+      HIPCHECK(hipGetLastError());
       return;
     }
     if (sync_free()) {
@@ -859,17 +867,15 @@ class DevPrecT : public DevPrec {
     HIPCHECK(hipGetLastError());
   }
   static constexpr int GRID_TRSV_BLOCKS = 2048;  // one wavefront each
-  // The ILU(0) pair on a grid with the operands in the tiles' step order:
-  // forward (L, t in step order) from the input in natural order, backward (U,
-  // d b from step order) into the natural-order output.
-  template <bool BACK, int BT>
-  void skew_launch(DSmoother<V>& S, const V* in, V* t, V* y) {
-    const int64_t nfill = BACK ? (int64_t)S.gnx * S.gny * S.gnz : S.gsz;
-    sf_fill_kernel<V><<<(int)((nfill + BS - 1) / BS), BS, 0, s_>>>((int)nfill, t);
+  // One substitution of the ILU(0) pair on a grid, every operand in the
+  // tiles' step order: t filled with the sentinel, then the solve.
+  template <bool BACK>
+  void skew_launch(DSmoother<V>& S, const V* in, V* t) {
+    sf_fill_kernel<V><<<(int)((S.gsz + BS - 1) / BS), BS, 0, s_>>>((int)S.gsz, t);
     const int ntiles = ((S.gny + 7) / 8) * ((S.gnz + 7) / 8);
     const int nwg = std::min(ntiles, GRID_TRSV_BLOCKS);
-    trsv_grid_skew_kernel<V, BT, BACK><<<nwg, 64, 0, s_>>>(S.gnx, S.gny, S.gnz, S.gsz, BACK ? S.ugs : S.lgs, S.dgs,
-                                                           in, t, y, S.sf_ticket, scal_ + S_ABORT, SF_SPIN_MAX);
+    trsv_grid_skew_kernel<V, BACK><<<nwg, 64, 0, s_>>>(S.gnx, S.gny, S.gnz, S.gsz, BACK ? S.ugs : S.lgs, S.dgs, in,
+                                                       t, S.sf_ticket, scal_ + S_ABORT, SF_SPIN_MAX);
     HIPCHECK(hipGetLastError());
   }
   // One sync-free substitution over a whole schedule (nch chunks) of an
@@ -979,6 +985,7 @@ class DevPrecT : public DevPrec {
     S.ugs = ar_.upload(a.data(), a.size(), s_);
     S.dgs = ar_.upload(dd.data(), dd.size(), s_);
     S.tgs = ar_.alloc<V>((size_t)sz);
+    S.xgs = ar_.alloc<V>((size_t)sz);
     S.gsz = sz;
     HIPCHECK(hipStreamSynchronize(s_));
   }

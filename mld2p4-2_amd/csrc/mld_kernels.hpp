@@ -1069,23 +1069,23 @@ __global__ __launch_bounds__(64) void trsv_grid_kernel(int nx, int ny, int nz, c
   }
 }
 
-// The ILU(0) pair of trsv_grid_kernel with the bulk operands in the tiles'
-// step order (position ((T (nx + 14) + s) 64 + lane) for the point that lane
+// The ILU(0) pair of trsv_grid_kernel with every operand in the tiles' step
+// order (position ((T (nx + 14) + s) 64 + lane) for the point that lane
 // (jj, kk) of tile T reaches at step s = i + jj + kk of the forward solve), so
-// that at every step the 64 lanes read 64 consecutive values instead of one
-// value from each of 64 grid lines.  Forward (!BACK, the unit lower factor,
-// t_i = b_i - acc): b in natural order, t in step order.  Backward (BACK, the
-// upper factor, t_i = d_i b_i - acc): b and d in step order, t (and y) in
-// natural order.  Values, waits and tile order as in trsv_grid_kernel.
-template <typename V, int BETA, bool BACK>
+// that at every step the 64 lanes read and write 64 consecutive values
+// instead of one value in each of 64 grid lines.  Forward (!BACK, the unit
+// lower factor): t_i = b_i - acc.  Backward (BACK, the upper factor):
+// t_i = d_i b_i - acc.  Values, waits and tile order as in trsv_grid_kernel;
+// grid_to_steps_kernel / grid_from_steps_kernel convert the input and output.
+template <typename V, bool BACK>
 __global__ __launch_bounds__(64) void trsv_grid_skew_kernel(int nx, int ny, int nz, int64_t sz,
                                                             const V* __restrict__ g, const V* __restrict__ d,
-                                                            const V* __restrict__ b, V* t, V* y, int* ticket,
+                                                            const V* __restrict__ b, V* t, int* ticket,
                                                             double* abort, int spin_max) {
   const int lane = threadIdx.x;
   const int jj = lane & 7, kk = lane >> 3;
   const int ntj = (ny + 7) >> 3, ntk = (nz + 7) >> 3, ntiles = ntj * ntk;
-  const int64_t pl = (int64_t)nx * ny, ns = nx + 14;
+  const int64_t ns = nx + 14;
   const int mj = BACK ? 7 - jj : jj, mk = BACK ? 7 - kk : kk;  // position in the solve's order
   const int ljn = BACK ? (lane + 1) & 63 : (lane + 63) & 63;  // lane of the neighbour in j
   const int lkn = BACK ? (lane + 8) & 63 : (lane + 56) & 63;  // lane of the neighbour in k
@@ -1103,30 +1103,26 @@ __global__ __launch_bounds__(64) void trsv_grid_skew_kernel(int nx, int ny, int 
     const bool hasj = BACK ? j + 1 < ny : j > 0;
     const bool hask = BACK ? k + 1 < nz : k > 0;
     const bool facej = hasj && mj == 0, facek = hask && mk == 0;
-    const int64_t line = (int64_t)nx * (j + (int64_t)ny * k);
     const int64_t base = (int64_t)tix * ns * 64;  // this tile's first position
+    // the face neighbours' positions at i = 0: (i, j -/+ 1, k) is lane (7 or 0,
+    // kk) of the tile before / after in j, (i, j, k -/+ 1) lane (jj, 7 or 0)
+    // of the tile before / after in k
+    const int64_t fj0 = BACK ? base + ns * 64 + (int64_t)kk * 64 + 8 * kk
+                             : base - ns * 64 + (int64_t)(7 + kk) * 64 + 7 + 8 * kk;
+    const int64_t fk0 = BACK ? base + (int64_t)ntj * ns * 64 + (int64_t)jj * 64 + jj
+                             : base - (int64_t)ntj * ns * 64 + (int64_t)(jj + 7) * 64 + jj + 56;
     V prev = (V)0;
     for (int st = 0; st < nx + 14; ++st) {
       const V vj = __shfl(prev, ljn), vk = __shfl(prev, lkn);
       const int mi = st - mj - mk;
       if (valid && mi >= 0 && mi < nx) {
         const int i = BACK ? nx - 1 - mi : mi;
-        const int64_t r = line + i;
-        const int64_t q = base + (int64_t)(i + jj + kk) * 64 + lane;  // position in step order
+        const int64_t q = base + (int64_t)(i + jj + kk) * 64 + lane;  // this point's position
         const bool hasi = BACK ? i + 1 < nx : i > 0;
         const V g0 = g[q], g1 = g[sz + q], g2 = g[2 * sz + q];
-        const V bi = BACK ? d[q] * b[q] : b[r];
-        V tj = vj, tk = vk;
-        if (BACK) {
-          if (facej) tj = sf_wait(t + r + nx, abort, spin_max);
-          if (facek) tk = sf_wait(t + r + pl, abort, spin_max);
-        } else {
-          // (i, j - 1, k): lane (7, kk) of the tile before in j; (i, j, k - 1):
-          // lane (jj, 7) of the tile before in k; both at step i + 7 + (kk or jj)
-          if (facej) tj = sf_wait(t + base - ns * 64 + (int64_t)(i + 7 + kk) * 64 + 7 + 8 * kk, abort, spin_max);
-          if (facek)
-            tk = sf_wait(t + base - (int64_t)ntj * ns * 64 + (int64_t)(i + jj + 7) * 64 + jj + 56, abort, spin_max);
-        }
+        const V bi = BACK ? d[q] * b[q] : b[q];
+        const V tj = facej ? sf_wait(t + fj0 + (int64_t)i * 64, abort, spin_max) : vj;
+        const V tk = facek ? sf_wait(t + fk0 + (int64_t)i * 64, abort, spin_max) : vk;
         V acc = (V)0;
         if (!BACK) {
           if (hask) acc = acc + g0 * tk;
@@ -1138,17 +1134,45 @@ __global__ __launch_bounds__(64) void trsv_grid_skew_kernel(int nx, int ny, int 
           if (hask) acc = acc + g2 * tk;
         }
         const V ti = bi - acc;
-        if (BACK) {
-          if (BETA == 0) y[r] = ti;
-          else if (BETA == 1) y[r] = y[r] + ti;
-          __hip_atomic_store(t + r, ti, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-          __hip_atomic_store(t + q, ti, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        __hip_atomic_store(t + q, ti, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         prev = ti;
       }
     }
   }
+}
+// xs = x in the tiles' step order (0 at the positions no grid point takes);
+// one thread per position, the writes consecutive.
+template <typename V>
+__global__ __launch_bounds__(BS) void grid_to_steps_kernel(int nx, int ny, int nz, int64_t sz,
+                                                           const V* __restrict__ x, V* __restrict__ xs) {
+  const int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (q >= sz) return;
+  const int ntj = (ny + 7) >> 3;
+  const int64_t ns = nx + 14;
+  const int lane = (int)(q & 63);
+  const int64_t ts = q >> 6;
+  const int tile = (int)(ts / ns), st = (int)(ts % ns);
+  const int jj = lane & 7, kk = lane >> 3;
+  const int j = (tile % ntj) * 8 + jj, k = (tile / ntj) * 8 + kk, i = st - jj - kk;
+  V v = (V)0;
+  if (j < ny && k < nz && i >= 0 && i < nx) v = x[i + (int64_t)nx * (j + (int64_t)ny * k)];
+  xs[q] = v;
+}
+// y = ts (BETA 0) or y = y + ts (BETA 1) in natural order; one thread per
+// grid point, the writes consecutive.
+template <typename V, int BETA>
+__global__ __launch_bounds__(BS) void grid_from_steps_kernel(int nx, int ny, int nz, const V* __restrict__ ts,
+                                                             V* __restrict__ y) {
+  const int64_t r = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const int64_t pl = (int64_t)nx * ny;
+  if (r >= pl * nz) return;
+  const int ntj = (ny + 7) >> 3;
+  const int64_t i = r % nx, j = (r / nx) % ny, k = r / pl;
+  const int64_t tile = j / 8 + (int64_t)ntj * (k / 8);
+  const int64_t q = (tile * (nx + 14) + i + j % 8 + k % 8) * 64 + j % 8 + 8 * (k % 8);
+  const V v = ts[q];
+  if (BETA == 1) y[r] = y[r] + v;
+  else y[r] = v;
 }
 
 // Whole ILU(0) apply for a small matrix in one workgroup: level sets
