@@ -788,14 +788,13 @@ class DevPrecT : public DevPrec {
     if (!TR && S.lgs && sync_free() && grid_trsv()) {
       // x into step order (xgs), L solve into tgs, U solve into xgs, result
       // back into natural order
-      const int64_t ng = (int64_t)S.gnx * S.gny * S.gnz;
-      grid_to_steps_kernel<V><<<(int)((S.gsz + BS - 1) / BS), BS, 0, s_>>>(S.gnx, S.gny, S.gnz, S.gsz, x, S.xgs);
+      const int ntiles = ((S.gny + 7) / 8) * ((S.gnz + 7) / 8);
+      const int nsc = (S.gnx + 14 + 63) / 64, nic = (S.gnx + 63) / 64;  // 64 steps / points per workgroup
+      grid_to_steps_kernel<V><<<ntiles * nsc, BS, 0, s_>>>(S.gnx, S.gny, S.gnz, x, S.xgs);
       skew_launch<false>(S, S.xgs, S.tgs);
       skew_launch<true>(S, S.tgs, S.xgs);
-      if (beta)
-        grid_from_steps_kernel<V, 1><<<(int)((ng + BS - 1) / BS), BS, 0, s_>>>(S.gnx, S.gny, S.gnz, S.xgs, y);
-      else
-        grid_from_steps_kernel<V, 0><<<(int)((ng + BS - 1) / BS), BS, 0, s_>>>(S.gnx, S.gny, S.gnz, S.xgs, y);
+      if (beta) grid_from_steps_kernel<V, 1><<<ntiles * nic, BS, 0, s_>>>(S.gnx, S.gny, S.gnz, S.xgs, y);
+      else grid_from_steps_kernel<V, 0><<<ntiles * nic, BS, 0, s_>>>(S.gnx, S.gny, S.gnz, S.xgs, y);
       HIPCHECK(hipGetLastError());
       return;
     }

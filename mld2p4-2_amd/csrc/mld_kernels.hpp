@@ -1140,39 +1140,64 @@ __global__ __launch_bounds__(64) void trsv_grid_skew_kernel(int nx, int ny, int 
     }
   }
 }
-// xs = x in the tiles' step order (0 at the positions no grid point takes);
-// one thread per position, the writes consecutive.
+// xs = x in the tiles' step order (0 at the positions no grid point takes).
+// One workgroup per tile and 64 steps: the 78 values of each of the tile's 64
+// grid lines that those steps take are read line by line into LDS, then the
+// 64 steps' positions are written row by row (both sides consecutive).
 template <typename V>
-__global__ __launch_bounds__(BS) void grid_to_steps_kernel(int nx, int ny, int nz, int64_t sz,
-                                                           const V* __restrict__ x, V* __restrict__ xs) {
-  const int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (q >= sz) return;
-  const int ntj = (ny + 7) >> 3;
+__global__ __launch_bounds__(BS) void grid_to_steps_kernel(int nx, int ny, int nz, const V* __restrict__ x,
+                                                           V* __restrict__ xs) {
+  constexpr int SC = 64, W = SC + 14;
+  __shared__ V sh[64][W + 1];
+  const int ntj = (ny + 7) >> 3, nsc = (nx + 14 + SC - 1) / SC;
+  const int tile = blockIdx.x / nsc, s0 = (blockIdx.x % nsc) * SC;
   const int64_t ns = nx + 14;
-  const int lane = (int)(q & 63);
-  const int64_t ts = q >> 6;
-  const int tile = (int)(ts / ns), st = (int)(ts % ns);
-  const int jj = lane & 7, kk = lane >> 3;
-  const int j = (tile % ntj) * 8 + jj, k = (tile / ntj) * 8 + kk, i = st - jj - kk;
-  V v = (V)0;
-  if (j < ny && k < nz && i >= 0 && i < nx) v = x[i + (int64_t)nx * (j + (int64_t)ny * k)];
-  xs[q] = v;
+  const int J = tile % ntj, K = tile / ntj;
+  for (int e = threadIdx.x; e < 64 * W; e += BS) {  // line l, i = s0 - 14 + w
+    const int l = e / W, w = e % W;
+    const int j = J * 8 + (l & 7), k = K * 8 + (l >> 3), i = s0 - 14 + w;
+    V v = (V)0;
+    if (j < ny && k < nz && i >= 0 && i < nx) v = x[i + (int64_t)nx * (j + (int64_t)ny * k)];
+    sh[l][w] = v;
+  }
+  __syncthreads();
+  const int64_t base = (int64_t)tile * ns * 64;
+  for (int e = threadIdx.x; e < SC * 64; e += BS) {  // step s0 + e / 64, lane e % 64
+    const int st = s0 + e / 64, l = e % 64;
+    if (st >= ns) break;
+    const int w = st - (l & 7) - (l >> 3) - (s0 - 14);  // i - (s0 - 14), in [0, W)
+    xs[base + (int64_t)st * 64 + l] = sh[l][w];
+  }
 }
-// y = ts (BETA 0) or y = y + ts (BETA 1) in natural order; one thread per
-// grid point, the writes consecutive.
+// y = ts (BETA 0) or y = y + ts (BETA 1) in natural order.  One workgroup per
+// tile and 64 points of its lines: the 78 steps holding them are read row by
+// row into LDS, then each line's 64 points are written (both sides
+// consecutive).
 template <typename V, int BETA>
 __global__ __launch_bounds__(BS) void grid_from_steps_kernel(int nx, int ny, int nz, const V* __restrict__ ts,
                                                              V* __restrict__ y) {
-  const int64_t r = (int64_t)blockIdx.x * BS + threadIdx.x;
-  const int64_t pl = (int64_t)nx * ny;
-  if (r >= pl * nz) return;
-  const int ntj = (ny + 7) >> 3;
-  const int64_t i = r % nx, j = (r / nx) % ny, k = r / pl;
-  const int64_t tile = j / 8 + (int64_t)ntj * (k / 8);
-  const int64_t q = (tile * (nx + 14) + i + j % 8 + k % 8) * 64 + j % 8 + 8 * (k % 8);
-  const V v = ts[q];
-  if (BETA == 1) y[r] = y[r] + v;
-  else y[r] = v;
+  constexpr int IC = 64, W = IC + 14;
+  __shared__ V sh[W][64 + 1];
+  const int ntj = (ny + 7) >> 3, nic = (nx + IC - 1) / IC;
+  const int tile = blockIdx.x / nic, i0 = (blockIdx.x % nic) * IC;
+  const int64_t ns = nx + 14;
+  const int J = tile % ntj, K = tile / ntj;
+  const int64_t base = (int64_t)tile * ns * 64;
+  for (int e = threadIdx.x; e < W * 64; e += BS) {  // step i0 + e / 64, lane e % 64
+    const int w = e / 64, l = e % 64;
+    const int st = i0 + w;
+    sh[w][l] = st < ns ? ts[base + (int64_t)st * 64 + l] : (V)0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * IC; e += BS) {  // line e / IC, i = i0 + e % IC
+    const int l = e / IC, i = i0 + e % IC;
+    const int j = J * 8 + (l & 7), k = K * 8 + (l >> 3);
+    if (j >= ny || k >= nz || i >= nx) continue;
+    const V v = sh[i - i0 + (l & 7) + (l >> 3)][l];
+    const int64_t r = i + (int64_t)nx * (j + (int64_t)ny * k);
+    if (BETA == 1) y[r] = y[r] + v;
+    else y[r] = v;
+  }
 }
 
 // Whole ILU(0) apply for a small matrix in one workgroup: level sets
