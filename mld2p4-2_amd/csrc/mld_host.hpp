@@ -35,8 +35,8 @@ enum : int {
   SMOOTH_PRE = 1, SMOOTH_POST = 2, SMOOTH_BOTH = 3,
 };
 
-// Error codes returned through the C-ABI (reference info codes where one
-// exists, see SURVEY of mld_d_hierarchy_bld.f90:113-118 etc.).
+// Error codes returned through the C-ABI (the reference's info codes where
+// one exists, e.g. mld_d_hierarchy_bld.f90:113-118).
 enum : int {
   E_OK = 0,
   E_NOT_INIT = 3111,        // init not called (mld_d_hierarchy_bld.f90:113-118)

@@ -203,7 +203,7 @@ def test_problem_generator_matches_oracle(eng, orc):
 def test_byte_model(eng):
     from mld2p4_amd import perfmodel
 
-    # fine-level residual at 256^3 (survey figure without row pointers: ELL)
+    # fine-level residual at 256^3 in CSR: values, column indices, row pointers, three vectors
     n, nnz = 256 ** 3, 117047296
     assert perfmodel.residual_bytes(n, nnz, 8, 8, csr=True) == 1874329604
     # two fused l1-Jacobi sweeps from a zero guess: ND + x + d + y
