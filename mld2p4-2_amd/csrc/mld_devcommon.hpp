@@ -80,13 +80,12 @@ struct DSmoother {
   int *lch = nullptr, *uch = nullptr, *ltch = nullptr, *utch = nullptr;
   int nlch = 0, nuch = 0, nltch = 0, nutch = 0;
   int* sf_ticket = nullptr;
-  // ILU(0) factors of a 7-point stencil on a gnx x gny x gnz grid (natural
-  // order): their values by stencil position for trsv_grid_kernel (null when
-  // the factors have another pattern)
-  V *lg = nullptr, *ug = nullptr;
+  // Triangular factors (ILU(0), or the parts of Gauss-Seidel) of a 7-point
+  // stencil on a gnx x gny x gnz grid in natural order: their values by
+  // stencil position, d and two work vectors in the tiles' step order for
+  // trsv_grid_kernel, gsz entries each (null when the factors have another
+  // pattern)
   int gnx = 0, gny = 0, gnz = 0;
-  // ILU(0) on such a grid: the factors' values, d and two work vectors in the
-  // tiles' step order (trsv_grid_skew_kernel), gsz entries each
   V *lgs = nullptr, *ugs = nullptr, *dgs = nullptr, *tgs = nullptr, *xgs = nullptr;
   int64_t gsz = 0;
   // dense LU

@@ -605,13 +605,8 @@ class DevPrecT : public DevPrec {
       }
       if (H.l.nr > SMALL_TRI) {  // ILU(0), GS, BWGS
         std::vector<V> lg, ug;
-        if (grid_factors(H.l, H.u, S.gnx, S.gny, S.gnz, lg, ug)) {
-          if (H.solver == ILU_N && !H.dv.empty()) {
-            skew_upload(S, lg, ug, H.dv);
-          } else {
-            S.lg = ar_.upload(lg.data(), lg.size(), s_);
-            S.ug = ar_.upload(ug.data(), ug.size(), s_);
-          }
+        if (!H.dv.empty() && grid_factors(H.l, H.u, S.gnx, S.gny, S.gnz, lg, ug)) {
+          skew_upload(S, lg, ug, H.dv);
           if (std::getenv("MLD_VERBOSE"))
             std::fprintf(stderr, "[mld] %d-row %s factors of a 7-point stencil on a %d x %d x %d grid: "
                                  "grid substitutions\n", H.l.nr, H.solver == ILU_N ? "ILU(0)" : "Gauss-Seidel",
@@ -785,17 +780,13 @@ class DevPrecT : public DevPrec {
       HIPCHECK(hipGetLastError());
       return;
     }
-    if (!TR && S.lgs && sync_free() && grid_trsv()) {
+    if (!TR && S.lgs && S.solver == ILU_N && sync_free() && grid_trsv()) {
       // x into step order (xgs), L solve into tgs, U solve into xgs, result
       // back into natural order
-      const int ntiles = ((S.gny + 7) / 8) * ((S.gnz + 7) / 8);
-      const int nsc = (S.gnx + 14 + 63) / 64, nic = (S.gnx + 63) / 64;  // 64 steps / points per workgroup
-      grid_to_steps_kernel<V><<<ntiles * nsc, BS, 0, s_>>>(S.gnx, S.gny, S.gnz, x, S.xgs);
-      skew_launch<false>(S, S.xgs, S.tgs);
-      skew_launch<true>(S, S.tgs, S.xgs);
-      if (beta) grid_from_steps_kernel<V, 1><<<ntiles * nic, BS, 0, s_>>>(S.gnx, S.gny, S.gnz, S.xgs, y);
-      else grid_from_steps_kernel<V, 0><<<ntiles * nic, BS, 0, s_>>>(S.gnx, S.gny, S.gnz, S.xgs, y);
-      HIPCHECK(hipGetLastError());
+      grid_in(S, x);
+      grid_launch<0, false>(S, S.xgs, S.tgs);
+      grid_launch<1, true>(S, S.tgs, S.xgs);
+      grid_out(S, S.xgs, nullptr, y, beta ? 1 : 0);
       return;
     }
     if (sync_free()) {
@@ -851,30 +842,31 @@ class DevPrecT : public DevPrec {
     HIPCHECK(hipGetLastError());
   }
 
-  // The grid form of one substitution (trsv_grid_kernel): t filled with the
-  // sentinel, then one wavefront per tile of 8 x 8 grid lines.
-  template <int SC, int BT, bool BACK>
-  void grid_launch(DSmoother<V>& S, int n, const V* g, const V* b, const V* d, V* t, V* y) {
-    sf_fill_kernel<V><<<nblk(n), BS, 0, s_>>>(n, t);
-    const int ntiles = ((S.gny + 7) / 8) * ((S.gnz + 7) / 8);
-    int xs = 1, nwg = GRID_TRSV_BLOCKS;  // (MLD_GRID_XS, MLD_GRID_WG override them, for measurements)
-    if (const char* e = std::getenv("MLD_GRID_XS")) xs = std::max(1, std::atoi(e));
-    if (const char* e = std::getenv("MLD_GRID_WG")) nwg = std::max(1, std::atoi(e));
-    nwg = std::min(ntiles, nwg);
-    trsv_grid_kernel<V, SC, BT, BACK><<<nwg * xs, 64, 0, s_>>>(S.gnx, S.gny, S.gnz, g, b, d, t, y, S.sf_ticket,
-                                                              scal_ + S_ABORT, SF_SPIN_MAX, xs);
-    HIPCHECK(hipGetLastError());
-  }
   static constexpr int GRID_TRSV_BLOCKS = 2048;  // one wavefront each
-  // One substitution of the ILU(0) pair on a grid, every operand in the
-  // tiles' step order: t filled with the sentinel, then the solve.
-  template <bool BACK>
-  void skew_launch(DSmoother<V>& S, const V* in, V* t) {
+  // One grid substitution (trsv_grid_kernel), every operand in the tiles'
+  // step order: t filled with the sentinel, then the solve.
+  template <int SC, bool BACK>
+  void grid_launch(DSmoother<V>& S, const V* in, V* t) {
     sf_fill_kernel<V><<<(int)((S.gsz + BS - 1) / BS), BS, 0, s_>>>((int)S.gsz, t);
     const int ntiles = ((S.gny + 7) / 8) * ((S.gnz + 7) / 8);
     const int nwg = std::min(ntiles, GRID_TRSV_BLOCKS);
-    trsv_grid_skew_kernel<V, BACK><<<nwg, 64, 0, s_>>>(S.gnx, S.gny, S.gnz, S.gsz, BACK ? S.ugs : S.lgs, S.dgs, in,
-                                                       t, S.sf_ticket, scal_ + S_ABORT, SF_SPIN_MAX);
+    trsv_grid_kernel<V, SC, BACK><<<nwg, 64, 0, s_>>>(S.gnx, S.gny, S.gnz, S.gsz, BACK ? S.ugs : S.lgs, S.dgs, in, t,
+                                                      S.sf_ticket, scal_ + S_ABORT, SF_SPIN_MAX);
+    HIPCHECK(hipGetLastError());
+  }
+  // x into the step order (S.xgs)
+  void grid_in(DSmoother<V>& S, const V* x) {
+    const int ntiles = ((S.gny + 7) / 8) * ((S.gnz + 7) / 8);
+    grid_to_steps_kernel<V><<<ntiles * ((S.gnx + 14 + 63) / 64), BS, 0, s_>>>(S.gnx, S.gny, S.gnz, x, S.xgs);
+    HIPCHECK(hipGetLastError());
+  }
+  // ts back into natural order: t = ts (t not null) and y by bt (0: y = ts,
+  // 1: y += ts, 2: y untouched)
+  void grid_out(DSmoother<V>& S, const V* ts, V* t, V* y, int bt) {
+    const int nb = ((S.gny + 7) / 8) * ((S.gnz + 7) / 8) * ((S.gnx + 63) / 64);
+    if (bt == 0) grid_from_steps_kernel<V, 0><<<nb, BS, 0, s_>>>(S.gnx, S.gny, S.gnz, ts, t, y);
+    else if (bt == 1) grid_from_steps_kernel<V, 1><<<nb, BS, 0, s_>>>(S.gnx, S.gny, S.gnz, ts, t, y);
+    else grid_from_steps_kernel<V, 2><<<nb, BS, 0, s_>>>(S.gnx, S.gny, S.gnz, ts, t, y);
     HIPCHECK(hipGetLastError());
   }
   // One sync-free substitution over a whole schedule (nch chunks) of an
@@ -915,7 +907,7 @@ class DevPrecT : public DevPrec {
   // (j > 0), r - 1 (i > 0), in this order, and of u r + 1, r + nx, r + nx ny
   // (i < nx - 1, j < ny - 1, k < nz - 1); nx, ny, nz >= 2.  Then the values
   // by stencil position (0 where a row has no such entry) for
-  // trsv_grid_kernel, and true.
+  // trsv_grid_kernel (in the tiles' step order: skew_upload), and true.
   static bool grid_factors(const HCsr& l, const HCsr& u, int& nx, int& ny, int& nz, std::vector<V>& lg,
                            std::vector<V>& ug) {
     const int64_t n = l.nr;
@@ -1059,17 +1051,13 @@ class DevPrecT : public DevPrec {
       run(0, nl);
       return;
     }
-    if (S.lg && sync_free() && grid_trsv()) {
-      const V* g = lower ? S.lg : S.ug;
-      if (lower) {
-        if (bt == 0) grid_launch<2, 0, false>(S, D.n, g, b, S.dv, t, y);
-        else if (bt == 1) grid_launch<2, 1, false>(S, D.n, g, b, S.dv, t, y);
-        else grid_launch<2, 2, false>(S, D.n, g, b, S.dv, t, y);
-      } else {
-        if (bt == 0) grid_launch<2, 0, true>(S, D.n, g, b, S.dv, t, y);
-        else if (bt == 1) grid_launch<2, 1, true>(S, D.n, g, b, S.dv, t, y);
-        else grid_launch<2, 2, true>(S, D.n, g, b, S.dv, t, y);
-      }
+    if (S.lgs && sync_free() && grid_trsv()) {
+      // b into step order, solve, y back (and t, which only a following
+      // sweep reads)
+      grid_in(S, b);
+      if (lower) grid_launch<2, false>(S, S.xgs, S.tgs);
+      else grid_launch<2, true>(S, S.xgs, S.tgs);
+      grid_out(S, S.tgs, bt == 2 ? t : nullptr, y, bt);
       return;
     }
     if (sync_free()) {

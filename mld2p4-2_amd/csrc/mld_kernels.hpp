@@ -991,97 +991,33 @@ __global__ __launch_bounds__(BS) void trsv_sf_kernel(int nchunks, const int* __r
   }
 }
 
-// Triangular solve with a 7-point ILU(0) factor on an nx x ny x nz grid in
-// natural order (lower: columns r - nx ny, r - nx, r - 1 where they exist;
-// BACK, upper: r + 1, r + nx, r + nx ny), g holding the factor's values by
-// stencil position (g[0..n), g[n..2n), g[2n..3n) in ascending column order).
-// One wavefront per tile of 8 x 8 grid lines along x: lane (jj, kk) owns the
-// line (j, k) = (8 J + jj, 8 K + kk) and the tile steps through its
-// diagonal planes i + jj + kk = s (mirrored for BACK).  A row's neighbours in
-// j and k inside the tile were computed by the neighbouring lanes in the
-// previous step and come by cross-lane reads; its neighbour in i is the
-// lane's own previous value; only the tile's two faces wait for values in
-// memory (sentinel and bounded wait as in trsv_sf_kernel).  Tiles are drawn
-// through a ticket in an order in which both tiles a tile waits for were
-// drawn before it.  Per row the arithmetic of tri_row_sf (products added in
-// ascending column order; t_i = b_i - acc, d_i b_i - acc or, SC = 2, the
-// Gauss-Seidel (b_i - acc) / d_i), so the values are those of the other
-// substitutions bit for bit.
-template <typename V, int SC, int BETA, bool BACK>
-__global__ __launch_bounds__(64) void trsv_grid_kernel(int nx, int ny, int nz, const V* __restrict__ g,
-                                                       const V* __restrict__ b, const V* __restrict__ d, V* t,
-                                                       V* y, int* ticket, double* abort, int spin_max, int xs) {
-  // xs > 1: only every xs-th workgroup draws tiles (the others leave at once)
-  if (blockIdx.x % xs != 0) return;
-  const int nwg = ((int)gridDim.x + xs - 1) / xs;  // workgroups that draw
-  const int lane = threadIdx.x;
-  const int jj = lane & 7, kk = lane >> 3;
-  const int ntj = (ny + 7) >> 3, ntk = (nz + 7) >> 3, ntiles = ntj * ntk;
-  const int64_t n = (int64_t)nx * ny * nz, pl = (int64_t)nx * ny;
-  const int mj = BACK ? 7 - jj : jj, mk = BACK ? 7 - kk : kk;  // position in the solve's order
-  const int ljn = BACK ? (lane + 1) & 63 : (lane + 63) & 63;  // lane of the neighbour in j
-  const int lkn = BACK ? (lane + 8) & 63 : (lane + 56) & 63;  // lane of the neighbour in k
-  for (;;) {
-    int c = 0;
-    if (lane == 0) {
-      c = atomicAdd(ticket, 1);
-      if (c == ntiles + nwg - 1) atomicExch(ticket, 0);  // every draw of this launch is done
-    }
-    c = __shfl(c, 0);
-    if (c >= ntiles) return;
-    const int tix = BACK ? ntiles - 1 - c : c;
-    const int j = (tix % ntj) * 8 + jj, k = (tix / ntj) * 8 + kk;
-    const bool valid = j < ny && k < nz;
-    const bool hasj = BACK ? j + 1 < ny : j > 0;
-    const bool hask = BACK ? k + 1 < nz : k > 0;
-    const bool facej = hasj && mj == 0, facek = hask && mk == 0;
-    const int64_t line = (int64_t)nx * (j + (int64_t)ny * k);
-    const int64_t oj = BACK ? nx : -(int64_t)nx, ok = BACK ? pl : -pl;  // neighbour offsets in j and k
-    V prev = (V)0;
-    for (int st = 0; st < nx + 14; ++st) {
-      const V vj = __shfl(prev, ljn), vk = __shfl(prev, lkn);
-      const int mi = st - mj - mk;
-      if (valid && mi >= 0 && mi < nx) {
-        const int i = BACK ? nx - 1 - mi : mi;
-        const int64_t r = line + i;
-        const bool hasi = BACK ? i + 1 < nx : i > 0;
-        const V g0 = g[r], g1 = g[n + r], g2 = g[2 * n + r];
-        const V bi = (SC == 1) ? d[r] * b[r] : b[r];
-        const V tj = facej ? sf_wait(t + r + oj, abort, spin_max) : vj;
-        const V tk = facek ? sf_wait(t + r + ok, abort, spin_max) : vk;
-        V acc = (V)0;
-        if (!BACK) {
-          if (hask) acc = acc + g0 * tk;
-          if (hasj) acc = acc + g1 * tj;
-          if (hasi) acc = acc + g2 * prev;
-        } else {
-          if (hasi) acc = acc + g0 * prev;
-          if (hasj) acc = acc + g1 * tj;
-          if (hask) acc = acc + g2 * tk;
-        }
-        const V ti = (SC == 2) ? (bi - acc) / d[r] : bi - acc;
-        if (BETA == 0) y[r] = ti;
-        else if (BETA == 1) y[r] = y[r] + ti;
-        __hip_atomic_store(t + r, ti, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        prev = ti;
-      }
-    }
-  }
-}
-
-// The ILU(0) pair of trsv_grid_kernel with every operand in the tiles' step
-// order (position ((T (nx + 14) + s) 64 + lane) for the point that lane
-// (jj, kk) of tile T reaches at step s = i + jj + kk of the forward solve), so
-// that at every step the 64 lanes read and write 64 consecutive values
-// instead of one value in each of 64 grid lines.  Forward (!BACK, the unit
-// lower factor): t_i = b_i - acc.  Backward (BACK, the upper factor):
-// t_i = d_i b_i - acc.  Values, waits and tile order as in trsv_grid_kernel;
-// grid_to_steps_kernel / grid_from_steps_kernel convert the input and output.
-template <typename V, bool BACK>
-__global__ __launch_bounds__(64) void trsv_grid_skew_kernel(int nx, int ny, int nz, int64_t sz,
-                                                            const V* __restrict__ g, const V* __restrict__ d,
-                                                            const V* __restrict__ b, V* t, int* ticket,
-                                                            double* abort, int spin_max) {
+// Triangular solve with a factor of the 7-point stencil on an nx x ny x nz
+// grid in natural order (lower: columns r - nx ny, r - nx, r - 1 where they
+// exist; BACK, upper: r + 1, r + nx, r + nx ny): the ILU(0) factors (SC 0,
+// unit lower: t_i = b_i - acc; SC 1, upper with the D scaling:
+// t_i = d_i b_i - acc) or the triangular parts of Gauss-Seidel (SC 2:
+// t_i = (b_i - acc) / d_i).  One wavefront per tile of 8 x 8 grid lines
+// along x: lane (jj, kk) owns the line (j, k) = (8 J + jj, 8 K + kk) and the
+// tile steps through its diagonal planes i + jj + kk = s (mirrored for
+// BACK).  A row's neighbours in j and k inside the tile were computed by the
+// neighbouring lanes in the previous step and come by cross-lane reads; its
+// neighbour in i is the lane's own previous value; only the tile's two faces
+// wait for values in memory (sentinel and bounded wait as in trsv_sf_kernel).
+// Tiles are drawn through a ticket in an order in which both tiles a tile
+// waits for were drawn before it.  Per row the arithmetic of tri_row_sf
+// (products added in ascending column order), so the values are those of
+// the other substitutions bit for bit.
+// Every operand is in the tiles' step order: position ((T (nx + 14) + s) 64
+// + lane) for the point that lane (jj, kk) of tile T reaches at step
+// s = i + jj + kk of the forward solve (g: the factor's values by stencil
+// position, 3 arrays of sz; d, b and t), so that at every step the 64 lanes
+// read and write 64 consecutive values instead of one value in each of 64
+// grid lines; grid_to_steps_kernel / grid_from_steps_kernel convert the
+// input and the output.
+template <typename V, int SC, bool BACK>
+__global__ __launch_bounds__(64) void trsv_grid_kernel(int nx, int ny, int nz, int64_t sz, const V* __restrict__ g,
+                                                       const V* __restrict__ d, const V* __restrict__ b, V* t,
+                                                       int* ticket, double* abort, int spin_max) {
   const int lane = threadIdx.x;
   const int jj = lane & 7, kk = lane >> 3;
   const int ntj = (ny + 7) >> 3, ntk = (nz + 7) >> 3, ntiles = ntj * ntk;
@@ -1120,7 +1056,7 @@ __global__ __launch_bounds__(64) void trsv_grid_skew_kernel(int nx, int ny, int 
         const int64_t q = base + (int64_t)(i + jj + kk) * 64 + lane;  // this point's position
         const bool hasi = BACK ? i + 1 < nx : i > 0;
         const V g0 = g[q], g1 = g[sz + q], g2 = g[2 * sz + q];
-        const V bi = BACK ? d[q] * b[q] : b[q];
+        const V bi = (SC == 1) ? d[q] * b[q] : b[q];
         const V tj = facej ? sf_wait(t + fj0 + (int64_t)i * 64, abort, spin_max) : vj;
         const V tk = facek ? sf_wait(t + fk0 + (int64_t)i * 64, abort, spin_max) : vk;
         V acc = (V)0;
@@ -1133,7 +1069,7 @@ __global__ __launch_bounds__(64) void trsv_grid_skew_kernel(int nx, int ny, int 
           if (hasj) acc = acc + g1 * tj;
           if (hask) acc = acc + g2 * tk;
         }
-        const V ti = bi - acc;
+        const V ti = (SC == 2) ? (bi - acc) / d[q] : bi - acc;
         __hip_atomic_store(t + q, ti, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         prev = ti;
       }
@@ -1169,13 +1105,14 @@ __global__ __launch_bounds__(BS) void grid_to_steps_kernel(int nx, int ny, int n
     xs[base + (int64_t)st * 64 + l] = sh[l][w];
   }
 }
-// y = ts (BETA 0) or y = y + ts (BETA 1) in natural order.  One workgroup per
+// ts back into natural order: t = ts (when t is not null) and y = ts
+// (BETA 0), y = y + ts (BETA 1) or y untouched (BETA 2).  One workgroup per
 // tile and 64 points of its lines: the 78 steps holding them are read row by
 // row into LDS, then each line's 64 points are written (both sides
 // consecutive).
 template <typename V, int BETA>
 __global__ __launch_bounds__(BS) void grid_from_steps_kernel(int nx, int ny, int nz, const V* __restrict__ ts,
-                                                             V* __restrict__ y) {
+                                                             V* __restrict__ t, V* __restrict__ y) {
   constexpr int IC = 64, W = IC + 14;
   __shared__ V sh[W][64 + 1];
   const int ntj = (ny + 7) >> 3, nic = (nx + IC - 1) / IC;
@@ -1195,8 +1132,9 @@ __global__ __launch_bounds__(BS) void grid_from_steps_kernel(int nx, int ny, int
     if (j >= ny || k >= nz || i >= nx) continue;
     const V v = sh[i - i0 + (l & 7) + (l >> 3)][l];
     const int64_t r = i + (int64_t)nx * (j + (int64_t)ny * k);
+    if (t) t[r] = v;
     if (BETA == 1) y[r] = y[r] + v;
-    else y[r] = v;
+    else if (BETA == 0) y[r] = v;
   }
 }
 
