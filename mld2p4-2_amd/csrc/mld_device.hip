@@ -905,7 +905,8 @@ class DevPrecT : public DevPrec {
   // stencil on an nx x ny x nz grid in natural order: every row r = i +
   // nx (j + ny k) of l holds exactly the columns r - nx ny (k > 0), r - nx
   // (j > 0), r - 1 (i > 0), in this order, and of u r + 1, r + nx, r + nx ny
-  // (i < nx - 1, j < ny - 1, k < nz - 1); nx, ny, nz >= 2.  Then the values
+  // (i < nx - 1, j < ny - 1, k < nz - 1); nx, ny, nz >= 2, at least 16 tiles of
+  // 8 x 8 grid lines.  Then the values
   // by stencil position (0 where a row has no such entry) for
   // trsv_grid_kernel (in the tiles' step order: skew_upload), and true.
   static bool grid_factors(const HCsr& l, const HCsr& u, int& nx, int& ny, int& nz, std::vector<V>& lg,
@@ -919,6 +920,9 @@ class DevPrecT : public DevPrec {
     nx = (int)sx;
     ny = (int)(sxy / sx);
     nz = (int)(n / sxy);
+    // at least 16 tiles of 8 x 8 lines (thinner grids leave the tiles' chain
+    // of waits with too little work beside it: the general form then)
+    if (((ny + 7) / 8) * ((nz + 7) / 8) < 16) return false;
     lg.assign(3 * (size_t)n, (V)0);
     ug.assign(3 * (size_t)n, (V)0);
     for (int64_t r = 0; r < n; ++r) {
