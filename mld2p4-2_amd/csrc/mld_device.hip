@@ -8,7 +8,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <atomic>
 #include <cstdio>
+#include <thread>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -909,6 +911,18 @@ class DevPrecT : public DevPrec {
   // 8 x 8 grid lines.  Then the values
   // by stencil position (0 where a row has no such entry) for
   // trsv_grid_kernel (in the tiles' step order: skew_upload), and true.
+  // f(r0, r1) over [0, n) in contiguous ranges on host threads
+  // (OMP_NUM_THREADS, default 8); f allocates nothing and does not throw
+  template <class F>
+  static void host_ranges(int64_t n, F f) {
+    int nt = 8;
+    if (const char* e = std::getenv("OMP_NUM_THREADS")) nt = std::atoi(e);
+    nt = (int)std::min<int64_t>(std::max(1, std::min(nt, 64)), std::max<int64_t>(1, n / 65536));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(f, n * t / nt, n * (t + 1) / nt);
+    f((int64_t)0, n / nt);
+    for (auto& th : pool) th.join();
+  }
   static bool grid_factors(const HCsr& l, const HCsr& u, int& nx, int& ny, int& nz, std::vector<V>& lg,
                            std::vector<V>& ug) {
     const int64_t n = l.nr;
@@ -925,28 +939,33 @@ class DevPrecT : public DevPrec {
     if (((ny + 7) / 8) * ((nz + 7) / 8) < 16) return false;
     lg.assign(3 * (size_t)n, (V)0);
     ug.assign(3 * (size_t)n, (V)0);
-    for (int64_t r = 0; r < n; ++r) {
-      const int64_t i = r % sx, j = (r / sx) % ny, k = r / sxy;
-      int64_t q = l.irp[r];
-      const int64_t lo[3] = {r - sxy, r - sx, r - 1};
-      const bool lh[3] = {k > 0, j > 0, i > 0};
-      for (int e = 0; e < 3; ++e)
-        if (lh[e]) {
-          if (q >= l.irp[r + 1] || l.ja[q] != lo[e]) return false;
-          lg[(size_t)e * n + r] = (V)l.val[q++];
-        }
-      if (q != l.irp[r + 1]) return false;
-      q = u.irp[r];
-      const int64_t uo[3] = {r + 1, r + sx, r + sxy};
-      const bool uh[3] = {i < sx - 1, j < ny - 1, k < nz - 1};
-      for (int e = 0; e < 3; ++e)
-        if (uh[e]) {
-          if (q >= u.irp[r + 1] || u.ja[q] != uo[e]) return false;
-          ug[(size_t)e * n + r] = (V)u.val[q++];
-        }
-      if (q != u.irp[r + 1]) return false;
-    }
-    return true;
+    std::atomic<bool> bad{false};
+    const int gy = ny, gz = nz;
+    host_ranges(n, [&](int64_t r0, int64_t r1) {
+      for (int64_t r = r0; r < r1 && !bad.load(std::memory_order_relaxed); ++r) {
+        const int64_t i = r % sx, j = (r / sx) % gy, k = r / sxy;
+        int64_t q = l.irp[r];
+        const int64_t lo[3] = {r - sxy, r - sx, r - 1};
+        const bool lh[3] = {k > 0, j > 0, i > 0};
+        bool ok = true;
+        for (int e = 0; e < 3 && ok; ++e)
+          if (lh[e]) {
+            if (q >= l.irp[r + 1] || l.ja[q] != lo[e]) ok = false;
+            else lg[(size_t)e * n + r] = (V)l.val[q++];
+          }
+        if (!ok || q != l.irp[r + 1]) { bad = true; break; }
+        q = u.irp[r];
+        const int64_t uo[3] = {r + 1, r + sx, r + sxy};
+        const bool uh[3] = {i < sx - 1, j < gy - 1, k < gz - 1};
+        for (int e = 0; e < 3 && ok; ++e)
+          if (uh[e]) {
+            if (q >= u.irp[r + 1] || u.ja[q] != uo[e]) ok = false;
+            else ug[(size_t)e * n + r] = (V)u.val[q++];
+          }
+        if (!ok || q != u.irp[r + 1]) { bad = true; break; }
+      }
+    });
+    return !bad;
   }
   // Position of grid point r = i + nx (j + ny k) in the tiles' step order:
   // tile T = j / 8 + ntj (k / 8), step s = i + j % 8 + k % 8 of the forward
@@ -967,16 +986,21 @@ class DevPrecT : public DevPrec {
     const int ntj = (ny + 7) / 8, ntk = (nz + 7) / 8;
     const int64_t n = (int64_t)nx * ny * nz, sz = (int64_t)ntj * ntk * (nx + 14) * 64;
     std::vector<V> a(3 * (size_t)sz, (V)0), dd((size_t)sz, (V)0);
-    for (int64_t r = 0; r < n; ++r) {
-      const int64_t q = skew_pos(r, nx, ny, ntj);
-      for (int e = 0; e < 3; ++e) a[(size_t)e * sz + q] = lg[(size_t)e * n + r];
-      dd[q] = (V)dv[r];
-    }
+    host_ranges(n, [&](int64_t r0, int64_t r1) {  // skew_pos is one-to-one: the ranges write apart
+      for (int64_t r = r0; r < r1; ++r) {
+        const int64_t q = skew_pos(r, nx, ny, ntj);
+        for (int e = 0; e < 3; ++e) a[(size_t)e * sz + q] = lg[(size_t)e * n + r];
+        dd[q] = (V)dv[r];
+      }
+    });
     S.lgs = ar_.upload(a.data(), a.size(), s_);
-    for (int64_t r = 0; r < n; ++r) {
-      const int64_t q = skew_pos(r, nx, ny, ntj);
-      for (int e = 0; e < 3; ++e) a[(size_t)e * sz + q] = ug[(size_t)e * n + r];
-    }
+    HIPCHECK(hipStreamSynchronize(s_));  // a is refilled below
+    host_ranges(n, [&](int64_t r0, int64_t r1) {
+      for (int64_t r = r0; r < r1; ++r) {
+        const int64_t q = skew_pos(r, nx, ny, ntj);
+        for (int e = 0; e < 3; ++e) a[(size_t)e * sz + q] = ug[(size_t)e * n + r];
+      }
+    });
     S.ugs = ar_.upload(a.data(), a.size(), s_);
     S.dgs = ar_.upload(dd.data(), dd.size(), s_);
     S.tgs = ar_.alloc<V>((size_t)sz);
