@@ -299,13 +299,24 @@ def section_run_b(eng, torch, problems, irp, ja, val, n, steps):
     t_vc = time_vcycle(torch, p, n, reps=5)
     # dependency depth: level sets of each ILU(0) solve per level, times the
     # solves per V-cycle (pre + post sweeps, each an L and a U solve)
-    levels, steps_vc = [], 0
+    levels, steps_vc, steps_sched = [], 0, 0
     for l in range(p.nlev):
         i = p.level_info(l)
         dl, du = p.tri_levels(l)
         apps = i["sweeps_pre"] + (i["sweeps_post"] if l < p.nlev - 1 else 0)
         levels.append({"n": i["n"], "levels_lower": dl, "levels_upper": du, "ilu_applications": apps})
         steps_vc += apps * (dl + du)
+        if l > 0:
+            steps_sched += apps * (dl + du)
+    # the fine level's factors are those of the 7-point grid: grid
+    # substitutions (one wavefront per 8 x 8 tile of grid lines) instead of its
+    # level sets; one application timed both ways
+    fine_grid_ms = p.time_step(0, "pre", 5)
+    os.environ["MLD_TRSV_GRID"] = "0"
+    try:
+        fine_sched_ms = p.time_step(0, "pre", 5)
+    finally:
+        os.environ.pop("MLD_TRSV_GRID", None)
     del p
     release(torch)
     floor_us = trsv_chain_step(eng, torch)
@@ -314,7 +325,13 @@ def section_run_b(eng, torch, problems, irp, ja, val, n, steps):
            "ilu_levels": levels, "dependent_steps_per_vcycle": steps_vc,
            "us_per_dependent_step": round(t_vc * 1e6 / steps_vc, 3),
            "chain_step_us": round(floor_us, 3),
-           "vcycle_lower_bound_ms": round(steps_vc * floor_us * 1e-3, 3)}
+           "vcycle_lower_bound_ms": round(steps_vc * floor_us * 1e-3, 3),
+           "fine_ilu_application_ms": {"grid_substitutions": round(fine_grid_ms, 3),
+                                       "level_sets": round(fine_sched_ms, 3)},
+           "level_set_steps_levels_1_on": steps_sched,
+           "levels_1_on_lower_bound_ms": round(steps_sched * floor_us * 1e-3, 3),
+           "note": "dependent steps and the lower bound count every level's level sets; the fine level runs as "
+                   "grid substitutions, whose time is fine_ilu_application_ms.grid_substitutions per application"}
     log(f"run B: {steps_vc} dependent steps per V-cycle, {t_vc * 1e6 / steps_vc:.2f} us each; "
         f"one step of a pure chain {floor_us:.2f} us -> lower bound {steps_vc * floor_us * 1e-3:.1f} ms")
     return out
