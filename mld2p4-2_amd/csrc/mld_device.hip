@@ -978,8 +978,9 @@ class DevPrecT : public DevPrec {
     const int64_t st = i + j % 8 + k % 8, lane = j % 8 + 8 * (k % 8);
     return (tile * (nx + 14) + st) * 64 + lane;
   }
-  // ILU(0) factors on a grid: values by stencil position (lg, ug: 3 n each),
-  // d, and the intermediate vector's buffer in the tiles' step order.
+  // Factors on a grid (ILU(0), or the triangular parts of Gauss-Seidel):
+  // values by stencil position (lg, ug: 3 n each) and d in the tiles' step
+  // order, and two work vectors of that order.
   void skew_upload(DSmoother<V>& S, const std::vector<V>& lg, const std::vector<V>& ug,
                    const std::vector<double>& dv) {
     const int nx = S.gnx, ny = S.gny, nz = S.gnz;
@@ -1008,8 +1009,8 @@ class DevPrecT : public DevPrec {
     S.gsz = sz;
     HIPCHECK(hipStreamSynchronize(s_));
   }
-  // the grid form of the ILU(0) substitutions (MLD_TRSV_GRID=0: the general
-  // sync-free form, for measurements and tests)
+  // the grid form of the ILU(0) and Gauss-Seidel substitutions
+  // (MLD_TRSV_GRID=0: the general sync-free form, for measurements and tests)
   static bool grid_trsv() {
     const char* e = std::getenv("MLD_TRSV_GRID");
     return !(e && std::atoi(e) == 0);
